@@ -1,0 +1,163 @@
+"""Throughput benchmark of the MI355X 2D MHD-GLM step.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--n 4096] [--precision fp64]
+
+Workload: Orszag-Tang on an n x n periodic grid (default 4096^2, the size the
+headline metric is quoted on), MUSCL + HLLD + mixed GLM, SSPRK2, CFL 0.25,
+alpha 0.18, HLLD tolerance 1e-4.  One "step" is one full SSPRK2 step of the
+whole grid (two fused stage kernels and one control kernel); the state stays
+resident in HBM.  value = n*n*K / (wall time of K steps) in cell-updates/s,
+the reference's "Mcell-steps/s" unit (mhd2d_glm_cpu.cpp:589-591) times 1e6.
+
+Also reported on the same JSON line:
+  roofline     -- the dominant (stage) kernel's algorithmic HBM bytes per
+                  launch over its average duration measured here with HIP
+                  events on the solver stream, against 8 TB/s;
+  cpu_baseline -- the reference CPU build (oracle/_ref, serial, one core) on
+                  a bounded sample of the same workload at 128^2.
+"""
+import argparse
+import json
+import os
+import re
+import subprocess
+import sys
+import tempfile
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "precision-study_amd"))
+
+HBM_PEAK_GBS = 8000.0
+METRIC = "cell-updates/s, 2D MHD-GLM HLLD 4096²; mixed vs FP64 speedup at L2≤1e-5"
+# Compulsory HBM bytes per cell of each stage kernel (9 variables):
+#   stage 1 reads U, writes U1;  stage 2 reads U1 and U, writes U.
+STAGE_BYTES_PER_CELL = {1: 2 * 9, 2: 3 * 9}
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def cpu_baseline(t_end: float):
+    """Reference CPU build (or the oracle if the reference build is absent),
+    OT 128^2 FP64 to t_end, one thread."""
+    ref = os.path.join(ROOT, "oracle", "_ref", "mhd2d_glm_cpu_fp64")
+    kind = "reference"
+    if not os.path.exists(ref):
+        ref = os.path.join(ROOT, "oracle", "build", "mhd2d_glm_oracle_fp64")
+        kind = "port"
+    if not os.path.exists(ref):
+        return None
+    with tempfile.TemporaryDirectory() as d:
+        t0 = time.perf_counter()
+        r = subprocess.run([ref, "128", "128", str(t_end), "0.25", "muscl", "hlld", "0.18"],
+                           env=dict(os.environ, OUTDIR=d), capture_output=True, text=True,
+                           timeout=600)
+        wall = time.perf_counter() - t0
+    if r.returncode != 0:
+        return None
+    m = re.search(r"Time loop wall time = (\S+) s \((\S+) Mcell-steps/s\)", r.stdout)
+    steps = int(re.search(r"Done\. Steps: (\d+)", r.stdout).group(1))
+    return {
+        "value": float(m.group(2)) * 1e6,
+        "unit": "cell-updates/s",
+        "cores": 1,
+        "kind": kind,
+        "sample": f"Orszag-Tang 128x128 FP64 MUSCL+HLLD to t={t_end} ({steps} SSPRK2 steps, "
+                  f"{float(m.group(1)):.1f} s loop, {wall:.1f} s process), serial g++ -O2 build",
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--n", type=int, default=4096)
+    ap.add_argument("--precision", choices=("fp64", "fp32"), default="fp64")
+    ap.add_argument("--tile", type=int, default=-1)
+    ap.add_argument("--cpu-t-end", type=float, default=0.2)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py: this solver runs on one GPU (a single grid per device); "
+              "multi-GPU runs are not supported -- use --gpus 1.", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    import mhdglm
+
+    if not torch.cuda.is_available():
+        print("bench.py: no GPU visible", file=sys.stderr)
+        sys.exit(2)
+    prec = mhdglm.PREC_FP64 if args.precision == "fp64" else mhdglm.PREC_FP32
+    esize = 8 if prec == mhdglm.PREC_FP64 else 4
+    n = args.n
+    log(f"bench: OT {n}x{n} {args.precision}, warmup {args.warmup}, steps {args.steps}")
+    solver = mhdglm.Solver(mhdglm.SolverConfig(nx=n, ny=n, precision=prec, tile=args.tile))
+    solver.init_problem("ot")
+    t_end = 0.5
+    solver.enqueue_steps(t_end, args.warmup)
+    torch.cuda.synchronize()
+    log("bench: warmup done")
+
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    solver.enqueue_steps(t_end, args.steps)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    st = solver.status()
+    if st.code != 0 or st.steps != args.warmup + args.steps:
+        print(f"bench: run did not complete the timed steps: {st.as_dict()}", file=sys.stderr)
+        sys.exit(1)
+    wall = t1 - t0
+    value = n * n * args.steps / wall
+    log(f"bench: {args.steps} steps in {wall:.3f} s -> {value:.4e} cell-updates/s")
+
+    ms = solver.time_kernels(t_end, 20)
+    solver.close()
+    stage = 1 if ms[0] >= ms[1] else 2
+    alg_bytes = STAGE_BYTES_PER_CELL[stage] * esize * n * n
+    achieved = alg_bytes / (ms[stage - 1] * 1e-3) / 1e9
+    traffic = None
+    tfile = os.path.join(ROOT, "profiles", f"traffic_stage_ot{n}_{args.precision}.json")
+    if os.path.exists(tfile):
+        with open(tfile) as f:
+            traffic = json.load(f).get(f"stage{stage}_bytes_per_launch")
+    roofline = {
+        "bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+        "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+        "kernel": f"k_stage<{stage}> ({STAGE_BYTES_PER_CELL[stage] * esize} B/cell compulsory)",
+        "kernel_ms": {"stage1": round(ms[0], 4), "stage2": round(ms[1], 4),
+                      "finish": round(ms[2], 4)},
+    }
+    cpu = None if args.no_cpu else cpu_baseline(args.cpu_t_end)
+    out = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "cell-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": wall / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64" if prec == mhdglm.PREC_FP64 else "f32",
+        "data": "synthetic: Orszag-Tang initial condition evaluated on the host",
+        "config": {
+            "workload": f"Orszag-Tang {n}x{n}, MUSCL+HLLD+GLM SSPRK2, {args.precision}",
+            "nx": n, "ny": n, "problem": "ot", "precision": args.precision,
+            "cfl": 0.25, "alpha": 0.18, "hlld_tol": 1e-4,
+            "step": "one SSPRK2 step of the whole grid",
+        },
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,131 @@
+/* mhd_glm.h -- C ABI of the MI355X 2D ideal-MHD + mixed-GLM solver.
+ *
+ * The reference solver has no library boundary: its drivers are
+ * executables (mhd2d_glm_cpu.cpp:417, mhd2d_glm_gpu.cu:353) whose loop calls
+ * header-only physics (include/mhd/mhd_glm_common.hpp).  This ABI is the seam
+ * between a host driver (CLI, initial conditions, .dat output) and the HIP
+ * step, so the process contract of those executables (argv, OUTDIR, stdout,
+ * exit codes) is kept by a thin host program on top of it, and the
+ * reference's by-array operator signatures are exported as batched calls.
+ *
+ * Conventions: plain C types only; every int return is 0 on success or a
+ * code of the reference executables (1 usage, 2 non-positive signal rate,
+ * 3 non-physical stage-1 state, 4 non-physical state, 9 device error);
+ * host buffers belong to the caller, device memory to the context; a
+ * context owns one HIP stream and is used by one host thread at a time.
+ *
+ * State arrays exchanged with the host are structure-of-arrays: 9 planes
+ * (rho, mx, my, mz, E, Bx, By, Bz, psi), plane k starting at element
+ * k*ld, cell (i,j) of plane k at k*ld + j*nx + i.  Element type is double
+ * for MHD_PREC_FP64 and float for MHD_PREC_FP32.
+ */
+#ifndef MHD_GLM_H
+#define MHD_GLM_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum mhd_precision { MHD_PREC_FP64 = 0, MHD_PREC_FP32 = 1 };
+
+/* Run configuration.  Mirrors the reference CLI
+ * `nx ny t_end cfl order solver alpha [problem]` (mhd2d_glm_gpu.cu:354-374)
+ * plus the compile-time switches of the reference build: precision
+ * (base.hpp:27-31, -DUSE_FLOAT) and HLLD tolerance
+ * (mhd_common.hpp:77-79, -DMHD_HLLD_TOLERANCE_VALUE). */
+typedef struct mhd_cfg {
+  int nx, ny;        /* grid cells, >= 2 each */
+  double cfl;        /* in (0,1) */
+  double alpha;      /* GLM damping parameter, > 0 */
+  int muscl;         /* 1: minmod MUSCL ("muscl"), 0: first order ("first") */
+  int hlld;          /* 1: HLLD with HLL fallback ("hlld"), 0: HLL ("hll") */
+  int precision;     /* enum mhd_precision */
+  double hlld_tol;   /* HLLD degeneracy tolerance (protocol value 1e-4) */
+  int tile;          /* stage-kernel tile shape id; -1 = default */
+  int batch_steps;   /* steps launched per host check; <= 0 = default */
+} mhd_cfg;
+
+/* Progress and diagnostics of a context. */
+typedef struct mhd_status {
+  int code;                 /* 0 running/finished, 2/3/4 halted, 9 device */
+  int done;                 /* 1 once t >= t_end */
+  int steps;                /* completed SSPRK2 steps since set_state */
+  double t;                 /* simulation time (value of the run's type) */
+  double dt;                /* dt of the next step, or of the failed step */
+  double ch_last;           /* c_h of the last step taken (or failed) */
+  unsigned long long x_evaluations, y_evaluations; /* HLLD flux evaluations */
+  unsigned long long degenerate, nonphysical;      /* HLLD->HLL fallbacks */
+  int bad_i, bad_j;         /* first unusable cell on halt 3/4, else -1 */
+  int fail_step;            /* step index of the failure */
+} mhd_status;
+
+typedef struct mhd_ctx mhd_ctx;
+
+/* Creates a solver on the current HIP device.  Replaces the setup of
+ * mhd2d_glm_gpu.cu:390-474 (grid, allocations).  Returns 0, 1 (invalid
+ * configuration) or 9 (device error); *out is NULL on failure. */
+int mhd_create(const mhd_cfg* cfg, mhd_ctx** out);
+void mhd_destroy(mhd_ctx* ctx);
+/* Last error message of the context (static text if ctx is NULL). */
+const char* mhd_last_error(const mhd_ctx* ctx);
+
+/* Initial condition of a named problem ("ot", "kh", "khb") evaluated on the
+ * host exactly as the reference does (mhd2d_glm_cpu.cpp:486-505,
+ * mhd2d_glm_gpu.cu:415-457) into a caller SoA buffer (no device needed). */
+int mhd_problem_state(const char* problem, int nx, int ny, int precision,
+                      void* soa9, size_t ld);
+
+/* Uploads a state and restarts time at t=0 with cleared counters; the
+ * first step's c_h and dt are computed on the device. */
+int mhd_set_state(mhd_ctx* ctx, const void* soa9, size_t ld);
+/* Same as mhd_set_state with mhd_problem_state's initial condition. */
+int mhd_init_problem(mhd_ctx* ctx, const char* problem);
+
+/* Advances until t >= t_end, a failure, or max_steps more steps (max_steps
+ * < 0: no limit).  This is the time loop of mhd2d_glm_cpu.cpp:513-570.
+ * Returns the status code (0 or 2/3/4/9) and fills *st if non-NULL. */
+int mhd_advance(mhd_ctx* ctx, double t_end, long long max_steps,
+                mhd_status* st);
+int mhd_get_status(mhd_ctx* ctx, mhd_status* st);
+
+/* Downloads the current state U (which = 0) or the stage-1 state U1
+ * (which = 1, the state the reference prints on an exit-3 failure). */
+int mhd_get_state(mhd_ctx* ctx, int which, void* soa9, size_t ld);
+
+/* Device pointer of state plane 0 (planes are ctx-internal `ld` apart,
+ * reported in *ld) for callers that keep data on the device. */
+int mhd_device_state(mhd_ctx* ctx, void** dev_ptr, size_t* ld);
+/* HIP stream of the context (as void*: hipStream_t). */
+void* mhd_stream(mhd_ctx* ctx);
+/* Enqueues max_steps steps on the context stream without any host
+ * synchronisation or status read-back (for timing and graph capture). */
+int mhd_enqueue_steps(mhd_ctx* ctx, double t_end, int nsteps);
+
+/* Measurement (no reference counterpart; the reference times its loop with
+ * a host wall clock, mhd2d_glm_cpu.cpp:512,571-572): runs nsteps steps with
+ * HIP events recorded on the context stream around each kernel and writes
+ * the average duration in milliseconds of the stage-1, stage-2 and finish
+ * kernels to avg_ms[0..2]. */
+int mhd_time_kernels(mhd_ctx* ctx, double t_end, int nsteps, double* avg_ms);
+
+/* Operator-level calls: the reference's by-array physics functions, batched
+ * over n states on the device.  Arrays are host AoS (state s, slot k at
+ * s*9+k) of the precision's element type.
+ *   mhd_glm_flux_batch  ~ mhdGlmFluxX/Y (mhd_glm_common.hpp:69-126);
+ *                         dir 0 = x, 1 = y; status[s] = HLLD status
+ *   mhd_cons_to_prim_batch ~ mhdGlmConsToPrim (mhd_glm_common.hpp:43-50)
+ *   mhd_fast_speed_batch   ~ mhdFastSpeedX/Y (mhd_common.hpp:206-238) */
+int mhd_glm_flux_batch(int precision, int dir, int n, const void* UL,
+                       const void* UR, double ch, int hlld, double hlld_tol,
+                       void* F, int* status);
+int mhd_cons_to_prim_batch(int precision, int n, const void* U, void* W);
+int mhd_fast_speed_batch(int precision, int dir, int n, const void* W,
+                         void* out);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* MHD_GLM_H */

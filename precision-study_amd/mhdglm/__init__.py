@@ -1,0 +1,268 @@
+"""Python binding of the MI355X MHD-GLM solver (ctypes over include/mhd_glm.h).
+
+Mirrors the reference's interfaces for the 2D MHD-GLM path:
+  * :class:`Solver` -- the time loop of mhd2d_glm_cpu.cpp:513-570 behind the
+    C ABI (create / set_state / advance / get_state);
+  * :func:`glm_flux`, :func:`cons_to_prim`, :func:`fast_speed` -- the
+    by-array physics functions of include/mhd/mhd_glm_common.hpp and
+    include/mhd/mhd_common.hpp, batched on the device;
+  * :func:`run_cli` -- the command-line drop-in (argv, OUTDIR, .dat).
+
+The library is loaded from this package's ``lib/`` directory (built by
+``make -C precision-study_amd``).  There is no CPU fallback: a missing
+library or device raises.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+from dataclasses import dataclass
+
+import numpy as np
+
+PKG_ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+LIB_PATH = os.path.join(PKG_ROOT, "lib", "libmhdglm.so")
+CLI_PATH = os.path.join(PKG_ROOT, "bin", "mhd2d_glm_amd")
+
+PREC_FP64 = 0
+PREC_FP32 = 1
+DTYPES = {PREC_FP64: np.float64, PREC_FP32: np.float32}
+
+# Public symbols of include/mhd_glm.h.
+EXPORTED = (
+    "mhd_create", "mhd_destroy", "mhd_last_error", "mhd_problem_state",
+    "mhd_set_state", "mhd_init_problem", "mhd_advance", "mhd_get_status",
+    "mhd_get_state", "mhd_device_state", "mhd_stream", "mhd_enqueue_steps",
+    "mhd_time_kernels", "mhd_glm_flux_batch", "mhd_cons_to_prim_batch",
+    "mhd_fast_speed_batch",
+)
+
+
+class MhdError(RuntimeError):
+    """A non-zero return of the C ABI (code = reference exit code)."""
+
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"mhd error {code}: {msg}")
+        self.code = code
+
+
+class Cfg(ctypes.Structure):
+    _fields_ = [
+        ("nx", ctypes.c_int), ("ny", ctypes.c_int),
+        ("cfl", ctypes.c_double), ("alpha", ctypes.c_double),
+        ("muscl", ctypes.c_int), ("hlld", ctypes.c_int),
+        ("precision", ctypes.c_int), ("hlld_tol", ctypes.c_double),
+        ("tile", ctypes.c_int), ("batch_steps", ctypes.c_int),
+    ]
+
+
+class Status(ctypes.Structure):
+    _fields_ = [
+        ("code", ctypes.c_int), ("done", ctypes.c_int), ("steps", ctypes.c_int),
+        ("t", ctypes.c_double), ("dt", ctypes.c_double), ("ch_last", ctypes.c_double),
+        ("x_evaluations", ctypes.c_ulonglong), ("y_evaluations", ctypes.c_ulonglong),
+        ("degenerate", ctypes.c_ulonglong), ("nonphysical", ctypes.c_ulonglong),
+        ("bad_i", ctypes.c_int), ("bad_j", ctypes.c_int), ("fail_step", ctypes.c_int),
+    ]
+
+    def as_dict(self) -> dict:
+        return {name: getattr(self, name) for name, _ in self._fields_}
+
+
+_LIB = None
+
+
+def load(path: str = LIB_PATH) -> ctypes.CDLL:
+    """Loads the solver library (raises if it was not built)."""
+    global _LIB
+    if _LIB is not None:
+        return _LIB
+    if not os.path.exists(path):
+        raise FileNotFoundError(
+            f"{path} is missing: build it with `make -C {PKG_ROOT}` "
+            "(or __graft_entry__.build())")
+    lib = ctypes.CDLL(path)
+    P, I, D, S = ctypes.c_void_p, ctypes.c_int, ctypes.c_double, ctypes.c_size_t
+    lib.mhd_create.argtypes = [ctypes.POINTER(Cfg), ctypes.POINTER(P)]
+    lib.mhd_create.restype = I
+    lib.mhd_destroy.argtypes = [P]
+    lib.mhd_destroy.restype = None
+    lib.mhd_last_error.argtypes = [P]
+    lib.mhd_last_error.restype = ctypes.c_char_p
+    lib.mhd_problem_state.argtypes = [ctypes.c_char_p, I, I, I, P, S]
+    lib.mhd_problem_state.restype = I
+    lib.mhd_set_state.argtypes = [P, P, S]
+    lib.mhd_set_state.restype = I
+    lib.mhd_init_problem.argtypes = [P, ctypes.c_char_p]
+    lib.mhd_init_problem.restype = I
+    lib.mhd_advance.argtypes = [P, D, ctypes.c_longlong, ctypes.POINTER(Status)]
+    lib.mhd_advance.restype = I
+    lib.mhd_get_status.argtypes = [P, ctypes.POINTER(Status)]
+    lib.mhd_get_status.restype = I
+    lib.mhd_get_state.argtypes = [P, I, P, S]
+    lib.mhd_get_state.restype = I
+    lib.mhd_device_state.argtypes = [P, ctypes.POINTER(P), ctypes.POINTER(S)]
+    lib.mhd_device_state.restype = I
+    lib.mhd_stream.argtypes = [P]
+    lib.mhd_stream.restype = P
+    lib.mhd_enqueue_steps.argtypes = [P, D, I]
+    lib.mhd_enqueue_steps.restype = I
+    lib.mhd_time_kernels.argtypes = [P, D, I, ctypes.POINTER(ctypes.c_double)]
+    lib.mhd_time_kernels.restype = I
+    lib.mhd_glm_flux_batch.argtypes = [I, I, I, P, P, D, I, D, P, P]
+    lib.mhd_glm_flux_batch.restype = I
+    lib.mhd_cons_to_prim_batch.argtypes = [I, I, P, P]
+    lib.mhd_cons_to_prim_batch.restype = I
+    lib.mhd_fast_speed_batch.argtypes = [I, I, I, P, P]
+    lib.mhd_fast_speed_batch.restype = I
+    _LIB = lib
+    return lib
+
+
+def _ptr(a: np.ndarray) -> ctypes.c_void_p:
+    return ctypes.c_void_p(a.ctypes.data)
+
+
+def problem_state(problem: str, nx: int, ny: int, precision: int = PREC_FP64) -> np.ndarray:
+    """Host-evaluated initial state, shape (9, ny, nx) (no device needed)."""
+    lib = load()
+    out = np.empty((9, ny, nx), dtype=DTYPES[precision])
+    rc = lib.mhd_problem_state(problem.encode(), nx, ny, precision, _ptr(out), nx * ny)
+    if rc:
+        raise MhdError(rc, f"unknown problem {problem!r} or bad size")
+    return out
+
+
+@dataclass
+class SolverConfig:
+    nx: int
+    ny: int
+    cfl: float = 0.25
+    alpha: float = 0.18
+    muscl: bool = True
+    hlld: bool = True
+    precision: int = PREC_FP64
+    hlld_tol: float = 1.0e-4
+    tile: int = -1
+    batch_steps: int = 0
+
+
+class Solver:
+    """One solver context (device state + stream) behind the C ABI."""
+
+    def __init__(self, cfg: SolverConfig):
+        self.lib = load()
+        self.cfg = cfg
+        c = Cfg(cfg.nx, cfg.ny, cfg.cfl, cfg.alpha, int(cfg.muscl), int(cfg.hlld),
+                cfg.precision, cfg.hlld_tol, cfg.tile, cfg.batch_steps)
+        h = ctypes.c_void_p()
+        rc = self.lib.mhd_create(ctypes.byref(c), ctypes.byref(h))
+        if rc:
+            raise MhdError(rc, "mhd_create failed")
+        self.h = h
+        self.dtype = DTYPES[cfg.precision]
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.mhd_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def _check(self, rc: int, allow=(0,)) -> int:
+        if rc not in allow:
+            raise MhdError(rc, self.lib.mhd_last_error(self.h).decode())
+        return rc
+
+    def set_state(self, u: np.ndarray):
+        u = np.ascontiguousarray(u, dtype=self.dtype).reshape(9, -1)
+        assert u.shape[1] == self.cfg.nx * self.cfg.ny
+        self._check(self.lib.mhd_set_state(self.h, _ptr(u), u.shape[1]))
+
+    def init_problem(self, problem: str = "ot"):
+        self._check(self.lib.mhd_init_problem(self.h, problem.encode()))
+
+    def advance(self, t_end: float, max_steps: int = -1) -> Status:
+        st = Status()
+        self._check(self.lib.mhd_advance(self.h, t_end, max_steps, ctypes.byref(st)),
+                    allow=(0, 2, 3, 4))
+        return st
+
+    def status(self) -> Status:
+        st = Status()
+        self._check(self.lib.mhd_get_status(self.h, ctypes.byref(st)), allow=(0, 2, 3, 4))
+        return st
+
+    def get_state(self, which: int = 0) -> np.ndarray:
+        out = np.empty((9, self.cfg.ny, self.cfg.nx), dtype=self.dtype)
+        self._check(self.lib.mhd_get_state(self.h, which, _ptr(out), self.cfg.nx * self.cfg.ny))
+        return out
+
+    def enqueue_steps(self, t_end: float, nsteps: int):
+        self._check(self.lib.mhd_enqueue_steps(self.h, t_end, nsteps))
+
+    def time_kernels(self, t_end: float, nsteps: int):
+        """Average ms of (stage-1, stage-2, finish) kernels over nsteps steps."""
+        out = (ctypes.c_double * 3)()
+        self._check(self.lib.mhd_time_kernels(self.h, t_end, nsteps, out))
+        return tuple(out)
+
+    def stream(self) -> int:
+        return self.lib.mhd_stream(self.h)
+
+
+def _aos(a, dtype) -> np.ndarray:
+    a = np.ascontiguousarray(a, dtype=dtype)
+    assert a.ndim == 2 and a.shape[1] == 9
+    return a
+
+
+def glm_flux(UL, UR, ch: float, direction: int = 0, hlld: bool = True,
+             precision: int = PREC_FP64, hlld_tol: float = 1.0e-4):
+    """Batched mhdGlmFluxX/Y on the device: (F[n,9], status[n])."""
+    lib = load()
+    dt = DTYPES[precision]
+    a, b = _aos(UL, dt), _aos(UR, dt)
+    n = a.shape[0]
+    F = np.empty((n, 9), dtype=dt)
+    st = np.empty(n, dtype=np.int32)
+    rc = lib.mhd_glm_flux_batch(precision, direction, n, _ptr(a), _ptr(b), ch, int(hlld),
+                                hlld_tol, _ptr(F), _ptr(st))
+    if rc:
+        raise MhdError(rc, "mhd_glm_flux_batch failed")
+    return F, st
+
+
+def cons_to_prim(U, precision: int = PREC_FP64) -> np.ndarray:
+    """Batched mhdGlmConsToPrim on the device."""
+    lib = load()
+    dt = DTYPES[precision]
+    a = _aos(U, dt)
+    W = np.empty_like(a)
+    rc = lib.mhd_cons_to_prim_batch(precision, a.shape[0], _ptr(a), _ptr(W))
+    if rc:
+        raise MhdError(rc, "mhd_cons_to_prim_batch failed")
+    return W
+
+
+def fast_speed(W, direction: int = 0, precision: int = PREC_FP64) -> np.ndarray:
+    """Batched mhdFastSpeedX/Y on the device."""
+    lib = load()
+    dt = DTYPES[precision]
+    a = _aos(W, dt)
+    out = np.empty(a.shape[0], dtype=dt)
+    rc = lib.mhd_fast_speed_batch(precision, direction, a.shape[0], _ptr(a), _ptr(out))
+    if rc:
+        raise MhdError(rc, "mhd_fast_speed_batch failed")
+    return out
+
+
+def run_cli(args, outdir: str, precision: str = "fp64", env=None, timeout=600):
+    """Runs the drop-in executable; returns the CompletedProcess."""
+    e = dict(os.environ if env is None else env)
+    e["OUTDIR"] = outdir
+    e["MHD_PRECISION"] = precision
+    return subprocess.run([CLI_PATH] + [str(x) for x in args], env=e, capture_output=True,
+                          text=True, timeout=timeout)

@@ -1,0 +1,75 @@
+"""CPU tests of the product's host side: C ABI surface, host-evaluated
+initial conditions, CLI argument handling.  No device call is made."""
+import os
+import re
+import subprocess
+
+import numpy as np
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = os.path.join(ROOT, "include", "mhd_glm.h")
+UINT = {np.float64: np.uint64, np.float32: np.uint32}
+
+
+def declared_functions():
+    text = open(HEADER).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(mhd_[a-z0-9_]+)\s*\(", text)))
+
+
+def test_library_exports_every_declared_symbol(mhd):
+    lib = mhd.load()
+    names = declared_functions()
+    assert len(names) >= 15
+    missing = [n for n in names if not hasattr(lib, n)]
+    assert not missing, missing
+    assert set(names) == set(mhd.EXPORTED)
+
+
+def test_header_is_plain_c():
+    text = open(HEADER).read()
+    for banned in ("torch", "hipStream_t", "std::", "template"):
+        assert banned not in re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    r = subprocess.run(["gcc", "-std=c99", "-fsyntax-only", "-x", "c", HEADER], capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
+
+
+@pytest.mark.parametrize("problem", ["ot", "kh", "khb"])
+@pytest.mark.parametrize("dtype,prec", [(np.float64, 0), (np.float32, 1)])
+def test_initial_state_matches_oracle(mhd, oracle, problem, dtype, prec):
+    nx, ny = 48, 40
+    u = mhd.problem_state(problem, nx, ny, prec)
+    ref = np.moveaxis(oracle.init(problem, nx, ny, dtype=dtype), -1, 0)
+    assert u.dtype == dtype
+    assert np.array_equal(u.view(UINT[dtype]), np.ascontiguousarray(ref).view(UINT[dtype]))
+
+
+def test_problem_state_rejects_unknown(mhd):
+    with pytest.raises(mhd.MhdError):
+        mhd.problem_state("sod", 8, 8)
+
+
+def run_cli(mhd, args, tmp_path, precision="fp64"):
+    return mhd.run_cli(args, str(tmp_path), precision=precision)
+
+
+@pytest.mark.parametrize("args,msg", [
+    (["0", "8"], "invalid grid or final time"),
+    (["8", "8", "-1"], "invalid grid or final time"),
+    (["8", "8", "0.5", "1.5"], "CFL must be in (0, 1)"),
+    (["8", "8", "0.5", "0.25", "weno"], "order must be 'muscl' or 'first'"),
+    (["8", "8", "0.5", "0.25", "muscl", "roe"], "solver must be 'hll' or 'hlld'"),
+    (["8", "8", "0.5", "0.25", "muscl", "hlld", "0"], "alpha must be positive"),
+    (["8", "8", "0.5", "0.25", "muscl", "hlld", "0.18", "sod"], "problem must be"),
+    (["1", "8"], "nx, ny >= 2"),
+])
+def test_cli_usage_errors(mhd, tmp_path, args, msg):
+    r = run_cli(mhd, args, tmp_path)
+    assert r.returncode == 1
+    assert msg in r.stderr
+
+
+def test_cli_rejects_bad_precision(mhd, tmp_path):
+    r = run_cli(mhd, ["8", "8"], tmp_path, precision="fp16")
+    assert r.returncode == 1
