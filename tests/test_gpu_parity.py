@@ -33,10 +33,15 @@ def say(msg):
 
 
 def assert_bitwise(a, b, what=""):
+    """Bitwise equality, except that NaNs compare equal whatever their sign
+    and payload: IEEE 754 leaves those unspecified, x86 SSE produces a
+    negative default NaN and CDNA a positive one.  NaN positions must match."""
     a, b = np.ascontiguousarray(a), np.ascontiguousarray(b)
     assert a.shape == b.shape and a.dtype == b.dtype
+    na, nb = np.isnan(a), np.isnan(b)
+    assert np.array_equal(na, nb), f"{what}: NaN positions differ"
     ba, bb = a.view(UINT[a.dtype.type]), b.view(UINT[b.dtype.type])
-    bad = np.argwhere(ba != bb)
+    bad = np.argwhere((ba != bb) & ~na)
     assert bad.size == 0, f"{what}: {len(bad)} mismatches, first {tuple(bad[0])}: {a[tuple(bad[0])]!r} vs {b[tuple(bad[0])]!r}"
 
 
