@@ -9,7 +9,14 @@ whole grid (two fused stage kernels and one control kernel); the state stays
 resident in HBM.  value = n*n*K / (wall time of K steps) in cell-updates/s,
 the reference's "Mcell-steps/s" unit (mhd2d_glm_cpu.cpp:589-591) times 1e6.
 
-Also reported on the same JSON line:
+`value` is the plain FP64 path (bitwise the reference CPU build).  Also
+reported on the same JSON line:
+  forms        -- throughput of the other arithmetic forms on the same
+                  workload (FP64 with FMA, mixed, mixed + compensated, FP32)
+                  and the mixed-vs-FP64 speedup;
+  accuracy     -- relative L2 against FP64 of rho, p, |B| and psi after whole
+                  Orszag-Tang runs to t=0.5 at --accuracy-n (default 1024^2),
+                  for each form, beside the FP64 noise floor (FMA vs no FMA);
   roofline     -- the dominant (stage) kernel's algorithmic HBM bytes per
                   launch over its average duration measured here with HIP
                   events on the solver stream, against 8 TB/s;
@@ -69,16 +76,71 @@ def cpu_baseline(t_end: float):
     }
 
 
+def time_form(mhdglm, torch, form, n, steps, warmup, tile=-1):
+    """Times `steps` SSPRK2 steps of OT n^2 in one arithmetic form."""
+    log(f"bench: OT {n}x{n} {form}, warmup {warmup}, steps {steps}")
+    solver = mhdglm.Solver(mhdglm.SolverConfig(nx=n, ny=n, precision=mhdglm.PRECISIONS[form],
+                                               tile=tile))
+    solver.init_problem("ot")
+    t_end = 0.5
+    solver.enqueue_steps(t_end, warmup)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    solver.enqueue_steps(t_end, steps)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    st = solver.status()
+    if st.code != 0 or st.steps != warmup + steps:
+        print(f"bench: {form} run did not complete the timed steps: {st.as_dict()}",
+              file=sys.stderr)
+        sys.exit(1)
+    wall = t1 - t0
+    value = n * n * steps / wall
+    ms = solver.time_kernels(t_end, 20)
+    solver.close()
+    log(f"bench: {form}: {steps} steps in {wall:.3f} s -> {value:.4e} cell-updates/s")
+    return {"value": value, "wall": wall, "kernel_ms": ms}
+
+
+def accuracy_check(mhdglm, n, forms):
+    """Whole OT n^2 runs to t=0.5 in FP64 and each form: relative L2 vs FP64."""
+    from mhdglm import compare
+
+    def run(form):
+        t0 = time.perf_counter()
+        s = mhdglm.Solver(mhdglm.SolverConfig(nx=n, ny=n, precision=mhdglm.PRECISIONS[form]))
+        s.init_problem("ot")
+        st = s.advance(0.5)
+        u = s.get_state(0).astype("float64")
+        s.close()
+        log(f"bench: accuracy run {form} {n}^2: {st.steps} steps, {time.perf_counter() - t0:.1f} s")
+        return u, st
+
+    u64, s64 = run("fp64")
+    out = {"n": n, "t_end": 0.5, "fp64_steps": s64.steps, "relL2_vs_fp64": {}}
+    for form in forms:
+        u, st = run(form)
+        errs = compare.rel_l2_state(u64, u, ("rho", "p", "|B|", "psi"))
+        out["relL2_vs_fp64"][form] = {k: float(f"{v:.3e}") for k, v in errs.items()}
+        out["relL2_vs_fp64"][form]["steps"] = st.steps
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--n", type=int, default=4096)
-    ap.add_argument("--precision", choices=("fp64", "fp32"), default="fp64")
+    ap.add_argument("--precision", choices=("fp64", "fp32", "mixed", "mixed_kahan", "fp64_fma"),
+                    default="fp64")
     ap.add_argument("--tile", type=int, default=-1)
     ap.add_argument("--cpu-t-end", type=float, default=0.2)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--forms", default="fp64_fma,mixed,mixed_kahan,fp32",
+                    help="other forms timed beside --precision ('' for none)")
+    ap.add_argument("--accuracy-n", type=int, default=1024,
+                    help="grid of the whole-run accuracy check (0 to skip)")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="only run a plain device copy of this many MiB (HBM counter calibration)")
     args = ap.parse_args()
@@ -105,32 +167,10 @@ def main():
     if not torch.cuda.is_available():
         print("bench.py: no GPU visible", file=sys.stderr)
         sys.exit(2)
-    prec = mhdglm.PREC_FP64 if args.precision == "fp64" else mhdglm.PREC_FP32
-    esize = 8 if prec == mhdglm.PREC_FP64 else 4
     n = args.n
-    log(f"bench: OT {n}x{n} {args.precision}, warmup {args.warmup}, steps {args.steps}")
-    solver = mhdglm.Solver(mhdglm.SolverConfig(nx=n, ny=n, precision=prec, tile=args.tile))
-    solver.init_problem("ot")
-    t_end = 0.5
-    solver.enqueue_steps(t_end, args.warmup)
-    torch.cuda.synchronize()
-    log("bench: warmup done")
-
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    solver.enqueue_steps(t_end, args.steps)
-    torch.cuda.synchronize()
-    t1 = time.perf_counter()
-    st = solver.status()
-    if st.code != 0 or st.steps != args.warmup + args.steps:
-        print(f"bench: run did not complete the timed steps: {st.as_dict()}", file=sys.stderr)
-        sys.exit(1)
-    wall = t1 - t0
-    value = n * n * args.steps / wall
-    log(f"bench: {args.steps} steps in {wall:.3f} s -> {value:.4e} cell-updates/s")
-
-    ms = solver.time_kernels(t_end, 20)
-    solver.close()
+    res = time_form(mhdglm, torch, args.precision, n, args.steps, args.warmup, args.tile)
+    value, wall, ms = res["value"], res["wall"], res["kernel_ms"]
+    esize = 4 if args.precision == "fp32" else 8
     stage = 1 if ms[0] >= ms[1] else 2
     alg_bytes = STAGE_BYTES_PER_CELL[stage] * esize * n * n
     achieved = alg_bytes / (ms[stage - 1] * 1e-3) / 1e9
@@ -146,6 +186,19 @@ def main():
         "kernel_ms": {"stage1": round(ms[0], 4), "stage2": round(ms[1], 4),
                       "finish": round(ms[2], 4)},
     }
+    forms = {args.precision: round(value, 1)}
+    for form in [f for f in args.forms.split(",") if f and f != args.precision]:
+        forms[form] = round(time_form(mhdglm, torch, form, n, args.steps, args.warmup,
+                                      args.tile)["value"], 1)
+    if "mixed" in forms and args.precision in ("fp64", "mixed"):
+        forms["mixed_vs_fp64_speedup"] = round(forms["mixed"] / forms.get("fp64", value), 3)
+    if "mixed" in forms and "fp64_fma" in forms:
+        forms["mixed_vs_fp64_fma_speedup"] = round(forms["mixed"] / forms["fp64_fma"], 3)
+    accuracy = None
+    if args.accuracy_n:
+        accuracy = accuracy_check(mhdglm, args.accuracy_n,
+                                  [f for f in ("fp64_fma", "mixed", "mixed_kahan", "fp32")
+                                   if f in forms or f == "fp64_fma"])
     cpu = None if args.no_cpu else cpu_baseline(args.cpu_t_end)
     out = {
         "metric": METRIC,
@@ -158,7 +211,10 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64" if prec == mhdglm.PREC_FP64 else "f32",
+        "dtype": {"fp64": "f64", "fp32": "f32", "fp64_fma": "f64",
+                  "mixed": "f32 flux / f64 state+update", "mixed_kahan": "f32 flux / f32x2 state"}[args.precision],
+        "forms": forms,
+        "accuracy": accuracy,
         "data": "synthetic: Orszag-Tang initial condition evaluated on the host",
         "config": {
             "workload": f"Orszag-Tang {n}x{n}, MUSCL+HLLD+GLM SSPRK2, {args.precision}",

@@ -16,8 +16,8 @@
  *
  * State arrays exchanged with the host are structure-of-arrays: 9 planes
  * (rho, mx, my, mz, E, Bx, By, Bz, psi), plane k starting at element
- * k*ld, cell (i,j) of plane k at k*ld + j*nx + i.  Element type is double
- * for MHD_PREC_FP64 and float for MHD_PREC_FP32.
+ * k*ld, cell (i,j) of plane k at k*ld + j*nx + i.  Element type is float
+ * for MHD_PREC_FP32 and double for every other form.
  */
 #ifndef MHD_GLM_H
 #define MHD_GLM_H
@@ -28,7 +28,20 @@
 extern "C" {
 #endif
 
-enum mhd_precision { MHD_PREC_FP64 = 0, MHD_PREC_FP32 = 1 };
+/* Arithmetic form of the step.
+ *   FP64, FP32   the reference's two builds (base.hpp:27-31), FP contraction
+ *                off: bitwise the reference CPU build of that precision.
+ *   MIXED        FP32 reconstruction, HLLD and GLM; FP64 pressure recovery
+ *                where FP32 would cancel (low beta / high Mach); FP64 state,
+ *                update accumulation, dt and time.
+ *   MIXED_KAHAN  as MIXED but the state is FP32 + FP32 compensation.
+ *   FP64_FMA     FP64 with FMA contraction (relative L2 <= 1e-12 of FP64).
+ * No reference counterpart exists for the last three except the reference's
+ * own mixed build with the opposite allocation (mhd2d_glm_gpu_mixed.cu:4-8). */
+enum mhd_precision {
+  MHD_PREC_FP64 = 0, MHD_PREC_FP32 = 1, MHD_PREC_MIXED = 2,
+  MHD_PREC_MIXED_KAHAN = 3, MHD_PREC_FP64_FMA = 4
+};
 
 /* Run configuration.  Mirrors the reference CLI
  * `nx ny t_end cfl order solver alpha [problem]` (mhd2d_glm_gpu.cu:354-374)
@@ -45,6 +58,8 @@ typedef struct mhd_cfg {
   double hlld_tol;   /* HLLD degeneracy tolerance (protocol value 1e-4) */
   int tile;          /* stage-kernel tile shape id; -1 = default */
   int batch_steps;   /* steps launched per host check; <= 0 = default */
+  double lowbeta_ratio; /* MIXED*: FP64 recovery where E > ratio * e_int;
+                           <= 0 = default (16) */
 } mhd_cfg;
 
 /* Progress and diagnostics of a context. */
@@ -102,6 +117,16 @@ void* mhd_stream(mhd_ctx* ctx);
 /* Enqueues max_steps steps on the context stream without any host
  * synchronisation or status read-back (for timing and graph capture). */
 int mhd_enqueue_steps(mhd_ctx* ctx, double t_end, int nsteps);
+
+/* Binary checkpoint of the whole solver state (no reference counterpart:
+ * the reference writes only the final .dat, whose primitives cannot restore
+ * U bitwise, mhd2d_glm_cpu.cpp:648-672).  The file holds a header (magic,
+ * version, nx, ny, precision), the device control block (t, dt, c_h, step,
+ * counters) and the raw state planes; loading it into a context of the same
+ * nx, ny and precision continues the run bitwise.  Returns 0, 1 (mismatch or
+ * I/O error) or 9. */
+int mhd_checkpoint_save(mhd_ctx* ctx, const char* path);
+int mhd_checkpoint_load(mhd_ctx* ctx, const char* path);
 
 /* Measurement (no reference counterpart; the reference times its loop with
  * a host wall clock, mhd2d_glm_cpu.cpp:512,571-572): runs nsteps steps with

@@ -29,6 +29,7 @@ using mhd::GridDesc;
 struct mhd_ctx {
   mhd_cfg cfg;
   GridDesc grid;
+  const mhd::Launchers* L = nullptr;
   hipStream_t stream = nullptr;
   void* u = nullptr;       // device state U   (9 planes)
   void* u1 = nullptr;      // device state U1  (9 planes)
@@ -45,9 +46,18 @@ namespace {
 
 const char* kNoCtx = "mhd: no context";
 
-size_t elem_size(int prec) { return prec == MHD_PREC_FP32 ? sizeof(float) : sizeof(double); }
-size_t ctrl_size(int prec) {
-  return prec == MHD_PREC_FP32 ? sizeof(Ctrl<float>) : sizeof(Ctrl<double>);
+// Element type of host state buffers: float for FP32, double otherwise.
+size_t io_elem(int prec) { return prec == MHD_PREC_FP32 ? sizeof(float) : sizeof(double); }
+
+const mhd::Launchers* launchers_for(int prec) {
+  switch (prec) {
+    case MHD_PREC_FP64: return mhd::launchers_exact(mhd::MODE_FP64);
+    case MHD_PREC_FP32: return mhd::launchers_exact(mhd::MODE_FP32);
+    case MHD_PREC_MIXED: return mhd::launchers_mixed(mhd::MODE_MIXED);
+    case MHD_PREC_MIXED_KAHAN: return mhd::launchers_mixed(mhd::MODE_MIXED_KAHAN);
+    case MHD_PREC_FP64_FMA: return mhd::launchers_fma();
+    default: return nullptr;
+  }
 }
 
 int fail(mhd_ctx* c, int code, const std::string& msg) {
@@ -108,7 +118,7 @@ void fill_status(const mhd_ctx* c, const Ctrl<R>* h, mhd_status* st) {
 }
 
 int read_ctrl(mhd_ctx* c) {
-  HIP_OR_FAIL(c, hipMemcpyAsync(c->ctrl_host, c->ctrl, ctrl_size(c->cfg.precision),
+  HIP_OR_FAIL(c, hipMemcpyAsync(c->ctrl_host, c->ctrl, c->L->ctrl_bytes,
                                 hipMemcpyDeviceToHost, c->stream));
   HIP_OR_FAIL(c, hipStreamSynchronize(c->stream));
   return 0;
@@ -116,7 +126,7 @@ int read_ctrl(mhd_ctx* c) {
 
 int status_of(mhd_ctx* c, mhd_status* st) {
   mhd_status tmp;
-  if (c->cfg.precision == MHD_PREC_FP32)
+  if (c->L->time_is_float)
     fill_status(c, (const Ctrl<float>*)c->ctrl_host, &tmp);
   else
     fill_status(c, (const Ctrl<double>*)c->ctrl_host, &tmp);
@@ -124,18 +134,12 @@ int status_of(mhd_ctx* c, mhd_status* st) {
   return tmp.code;
 }
 
-hipError_t enqueue_step(mhd_ctx* c) {
-  if (c->cfg.precision == MHD_PREC_FP32)
-    return mhd::launch_step<float>(c->grid, (float*)c->u, (float*)c->u1,
-                                   (Ctrl<float>*)c->ctrl, c->stream);
-  return mhd::launch_step<double>(c->grid, (double*)c->u, (double*)c->u1,
-                                  (Ctrl<double>*)c->ctrl, c->stream);
+hipError_t enqueue_step(mhd_ctx* c, hipEvent_t* ev = nullptr) {
+  return c->L->step(c->grid, c->u, c->u1, c->ctrl, c->stream, ev);
 }
 
 hipError_t enqueue_target(mhd_ctx* c, double t_end) {
-  if (c->cfg.precision == MHD_PREC_FP32)
-    return mhd::launch_set_target<float>((Ctrl<float>*)c->ctrl, t_end, c->stream);
-  return mhd::launch_set_target<double>((Ctrl<double>*)c->ctrl, t_end, c->stream);
+  return c->L->set_target(c->ctrl, t_end, c->stream);
 }
 
 // Builds (once) a graph of `batch` steps on the context stream.
@@ -195,13 +199,15 @@ int mhd_create(const mhd_cfg* cfg, mhd_ctx** out) {
   if (cfg->nx < 2 || cfg->ny < 2) return 1;
   if (!(cfg->cfl > 0.0 && cfg->cfl < 1.0)) return 1;
   if (!(cfg->alpha > 0.0)) return 1;
-  if (cfg->precision != MHD_PREC_FP64 && cfg->precision != MHD_PREC_FP32) return 1;
+  if (!launchers_for(cfg->precision)) return 1;
   if ((long long)cfg->nx * cfg->ny > (long long)INT_MAX) return 1;
   mhd_ctx* c = new mhd_ctx();
   c->cfg = *cfg;
+  c->L = launchers_for(cfg->precision);
+  if (!(c->cfg.lowbeta_ratio > 0.0)) c->cfg.lowbeta_ratio = 16.0;
   if (c->cfg.batch_steps <= 0) c->cfg.batch_steps = 32;
   if (c->cfg.hlld_tol <= 0.0) c->cfg.hlld_tol = 1.0e-4;
-  if (c->cfg.tile < 0 || c->cfg.tile >= mhd::TILE_COUNT) c->cfg.tile = mhd::TILE_16x16;
+  if (c->cfg.tile < 0 || c->cfg.tile >= mhd::TILE_COUNT) c->cfg.tile = mhd::kDefaultTile;
   // Planes start on 512-byte boundaries.
   const size_t cells = (size_t)cfg->nx * cfg->ny;
   c->plane = (cells + 63) / 64 * 64;
@@ -213,13 +219,14 @@ int mhd_create(const mhd_cfg* cfg, mhd_ctx** out) {
   c->grid.alpha = cfg->alpha;
   c->grid.tol = c->cfg.hlld_tol;
   c->grid.tile = c->cfg.tile;
-  const size_t bytes = 9 * c->plane * elem_size(cfg->precision);
+  c->grid.lowbeta_ratio = c->cfg.lowbeta_ratio;
+  const size_t bytes = (size_t)c->L->state_planes * c->plane * c->L->state_elem;
   hipError_t e;
   if ((e = hipStreamCreate(&c->stream)) != hipSuccess ||
       (e = hipMalloc(&c->u, bytes)) != hipSuccess ||
       (e = hipMalloc(&c->u1, bytes)) != hipSuccess ||
-      (e = hipMalloc(&c->ctrl, ctrl_size(cfg->precision))) != hipSuccess ||
-      (e = hipHostMalloc(&c->ctrl_host, ctrl_size(cfg->precision), hipHostMallocDefault)) !=
+      (e = hipMalloc(&c->ctrl, c->L->ctrl_bytes)) != hipSuccess ||
+      (e = hipHostMalloc(&c->ctrl_host, c->L->ctrl_bytes, hipHostMallocDefault)) !=
           hipSuccess) {
     std::fprintf(stderr, "mhd_create: %s\n", hipGetErrorString(e));
     mhd_destroy(c);
@@ -247,7 +254,7 @@ int mhd_problem_state(const char* problem, int nx, int ny, int precision,
   if (ld < (size_t)nx * ny) return 1;
   if (precision == MHD_PREC_FP32)
     problem_soa<float>(problem, nx, ny, (float*)soa9, ld);
-  else if (precision == MHD_PREC_FP64)
+  else if (launchers_for(precision))
     problem_soa<double>(problem, nx, ny, (double*)soa9, ld);
   else
     return 1;
@@ -257,28 +264,37 @@ int mhd_problem_state(const char* problem, int nx, int ny, int precision,
 int mhd_set_state(mhd_ctx* c, const void* soa9, size_t ld) {
   if (!c) return 1;
   if (!soa9 || ld < (size_t)c->cfg.nx * c->cfg.ny) return fail(c, 1, "mhd_set_state: bad buffer");
-  const size_t es = elem_size(c->cfg.precision);
   const size_t cells = (size_t)c->cfg.nx * c->cfg.ny;
-  for (int k = 0; k < 9; k++)
-    HIP_OR_FAIL(c, hipMemcpyAsync((char*)c->u + k * c->plane * es,
-                                  (const char*)soa9 + k * ld * es, cells * es,
-                                  hipMemcpyHostToDevice, c->stream));
-  if (c->cfg.precision == MHD_PREC_FP32) {
+  const size_t es = c->L->state_elem;
+  if (c->L->state_planes == 18) {
+    // Compensated form: plane k holds hi = (float)x, plane 9+k lo = (float)(x - hi).
+    std::vector<float> hl(18 * cells);
+    const double* src = (const double*)soa9;
+    for (int k = 0; k < 9; k++)
+      for (size_t q = 0; q < cells; q++) {
+        const double x = src[k * ld + q];
+        const float hi = (float)x;
+        hl[k * cells + q] = hi;
+        hl[(9 + k) * cells + q] = (float)(x - (double)hi);
+      }
+    for (int k = 0; k < 18; k++)
+      HIP_OR_FAIL(c, hipMemcpy((char*)c->u + k * c->plane * es, hl.data() + k * cells,
+                               cells * es, hipMemcpyHostToDevice));
+  } else {
+    for (int k = 0; k < 9; k++)
+      HIP_OR_FAIL(c, hipMemcpyAsync((char*)c->u + k * c->plane * es,
+                                    (const char*)soa9 + k * ld * es, cells * es,
+                                    hipMemcpyHostToDevice, c->stream));
+  }
+  if (c->L->time_is_float) {
     init_ctrl((Ctrl<float>*)c->ctrl_host, c->cfg);
   } else {
     init_ctrl((Ctrl<double>*)c->ctrl_host, c->cfg);
   }
-  HIP_OR_FAIL(c, hipMemcpyAsync(c->ctrl, c->ctrl_host, ctrl_size(c->cfg.precision),
+  HIP_OR_FAIL(c, hipMemcpyAsync(c->ctrl, c->ctrl_host, c->L->ctrl_bytes,
                                 hipMemcpyHostToDevice, c->stream));
-  hipError_t e;
-  if (c->cfg.precision == MHD_PREC_FP32) {
-    e = mhd::launch_speed<float>(c->grid, (const float*)c->u, (Ctrl<float>*)c->ctrl, c->stream);
-    if (e == hipSuccess) e = mhd::launch_commit_speed<float>((Ctrl<float>*)c->ctrl, c->stream);
-  } else {
-    e = mhd::launch_speed<double>(c->grid, (const double*)c->u, (Ctrl<double>*)c->ctrl,
-                                  c->stream);
-    if (e == hipSuccess) e = mhd::launch_commit_speed<double>((Ctrl<double>*)c->ctrl, c->stream);
-  }
+  hipError_t e = c->L->speed(c->grid, c->u, c->ctrl, c->stream);
+  if (e == hipSuccess) e = c->L->commit(c->ctrl, c->stream);
   if (e != hipSuccess) return hip_fail(c, e, "initial signal speed");
   if (int rc = read_ctrl(c)) return rc;
   c->have_state = true;
@@ -289,7 +305,7 @@ int mhd_init_problem(mhd_ctx* c, const char* problem) {
   if (!c) return 1;
   if (!mhd::problem_known(problem)) return fail(c, 1, "mhd_init_problem: unknown problem");
   const size_t cells = (size_t)c->cfg.nx * c->cfg.ny;
-  std::vector<char> host(9 * cells * elem_size(c->cfg.precision));
+  std::vector<char> host(9 * cells * io_elem(c->cfg.precision));
   mhd_problem_state(problem, c->cfg.nx, c->cfg.ny, c->cfg.precision, host.data(), cells);
   return mhd_set_state(c, host.data(), cells);
 }
@@ -355,9 +371,21 @@ int mhd_get_state(mhd_ctx* c, int which, void* soa9, size_t ld) {
   if (!c) return 1;
   if (!soa9 || ld < (size_t)c->cfg.nx * c->cfg.ny || (which != 0 && which != 1))
     return fail(c, 1, "mhd_get_state: bad arguments");
-  const size_t es = elem_size(c->cfg.precision);
+  const size_t es = c->L->state_elem;
   const size_t cells = (size_t)c->cfg.nx * c->cfg.ny;
   const char* src = (const char*)(which == 0 ? c->u : c->u1);
+  if (c->L->state_planes == 18) {
+    std::vector<float> hl(18 * cells);
+    for (int k = 0; k < 18; k++)
+      HIP_OR_FAIL(c, hipMemcpyAsync(hl.data() + k * cells, src + k * c->plane * es, cells * es,
+                                    hipMemcpyDeviceToHost, c->stream));
+    HIP_OR_FAIL(c, hipStreamSynchronize(c->stream));
+    double* out = (double*)soa9;
+    for (int k = 0; k < 9; k++)
+      for (size_t q = 0; q < cells; q++)
+        out[k * ld + q] = (double)hl[k * cells + q] + (double)hl[(9 + k) * cells + q];
+    return 0;
+  }
   for (int k = 0; k < 9; k++)
     HIP_OR_FAIL(c, hipMemcpyAsync((char*)soa9 + k * ld * es, src + k * c->plane * es,
                                   cells * es, hipMemcpyDeviceToHost, c->stream));
@@ -374,6 +402,71 @@ int mhd_device_state(mhd_ctx* c, void** dev_ptr, size_t* ld) {
 
 void* mhd_stream(mhd_ctx* c) { return c ? (void*)c->stream : nullptr; }
 
+namespace {
+const char kCkMagic[8] = {'M', 'H', 'D', 'G', 'L', 'M', 'C', 'K'};
+struct CkHeader {
+  char magic[8];
+  int version, nx, ny, precision, planes, elem, ctrl_bytes, pad;
+};
+}  // namespace
+
+int mhd_checkpoint_save(mhd_ctx* c, const char* path) {
+  if (!c || !path) return 1;
+  if (!c->have_state) return fail(c, 1, "mhd_checkpoint_save: no state");
+  const size_t cells = (size_t)c->cfg.nx * c->cfg.ny;
+  const size_t es = c->L->state_elem;
+  std::vector<char> buf((size_t)c->L->state_planes * cells * es);
+  if (int rc = read_ctrl(c)) return rc;
+  for (int k = 0; k < c->L->state_planes; k++)
+    HIP_OR_FAIL(c, hipMemcpyAsync(buf.data() + k * cells * es, (const char*)c->u + k * c->plane * es,
+                                  cells * es, hipMemcpyDeviceToHost, c->stream));
+  HIP_OR_FAIL(c, hipStreamSynchronize(c->stream));
+  CkHeader h;
+  std::memset(&h, 0, sizeof(h));
+  std::memcpy(h.magic, kCkMagic, 8);
+  h.version = 1;
+  h.nx = c->cfg.nx;
+  h.ny = c->cfg.ny;
+  h.precision = c->cfg.precision;
+  h.planes = c->L->state_planes;
+  h.elem = (int)es;
+  h.ctrl_bytes = (int)c->L->ctrl_bytes;
+  FILE* f = std::fopen(path, "wb");
+  if (!f) return fail(c, 1, std::string("mhd_checkpoint_save: cannot open ") + path);
+  bool ok = std::fwrite(&h, sizeof(h), 1, f) == 1 &&
+            std::fwrite(c->ctrl_host, c->L->ctrl_bytes, 1, f) == 1 &&
+            std::fwrite(buf.data(), 1, buf.size(), f) == buf.size();
+  ok = (std::fclose(f) == 0) && ok;
+  return ok ? 0 : fail(c, 1, "mhd_checkpoint_save: write failed");
+}
+
+int mhd_checkpoint_load(mhd_ctx* c, const char* path) {
+  if (!c || !path) return 1;
+  FILE* f = std::fopen(path, "rb");
+  if (!f) return fail(c, 1, std::string("mhd_checkpoint_load: cannot open ") + path);
+  CkHeader h;
+  const size_t cells = (size_t)c->cfg.nx * c->cfg.ny;
+  const size_t es = c->L->state_elem;
+  bool ok = std::fread(&h, sizeof(h), 1, f) == 1 && std::memcmp(h.magic, kCkMagic, 8) == 0 &&
+            h.version == 1 && h.nx == c->cfg.nx && h.ny == c->cfg.ny &&
+            h.precision == c->cfg.precision && h.planes == c->L->state_planes &&
+            h.elem == (int)es && h.ctrl_bytes == (int)c->L->ctrl_bytes;
+  std::vector<char> buf;
+  if (ok) {
+    buf.resize((size_t)h.planes * cells * es);
+    ok = std::fread(c->ctrl_host, c->L->ctrl_bytes, 1, f) == 1 &&
+         std::fread(buf.data(), 1, buf.size(), f) == buf.size();
+  }
+  std::fclose(f);
+  if (!ok) return fail(c, 1, "mhd_checkpoint_load: file does not match this context");
+  for (int k = 0; k < h.planes; k++)
+    HIP_OR_FAIL(c, hipMemcpy((char*)c->u + k * c->plane * es, buf.data() + k * cells * es,
+                             cells * es, hipMemcpyHostToDevice));
+  HIP_OR_FAIL(c, hipMemcpy(c->ctrl, c->ctrl_host, c->L->ctrl_bytes, hipMemcpyHostToDevice));
+  c->have_state = true;
+  return 0;
+}
+
 int mhd_time_kernels(mhd_ctx* c, double t_end, int nsteps, double* avg_ms) {
   if (!c || !avg_ms || nsteps <= 0) return 1;
   if (!c->have_state) return fail(c, 1, "mhd_time_kernels: no state");
@@ -383,12 +476,7 @@ int mhd_time_kernels(mhd_ctx* c, double t_end, int nsteps, double* avg_ms) {
   for (auto& x : ev) HIP_OR_FAIL(c, hipEventCreate(&x));
   int rc = 0;
   for (int q = 0; q < nsteps && rc == 0; q++) {
-    if (c->cfg.precision == MHD_PREC_FP32)
-      e = mhd::launch_step_timed<float>(c->grid, (float*)c->u, (float*)c->u1,
-                                        (Ctrl<float>*)c->ctrl, c->stream, &ev[(size_t)q * 4]);
-    else
-      e = mhd::launch_step_timed<double>(c->grid, (double*)c->u, (double*)c->u1,
-                                         (Ctrl<double>*)c->ctrl, c->stream, &ev[(size_t)q * 4]);
+    e = enqueue_step(c, &ev[(size_t)q * 4]);
     if (e != hipSuccess) rc = hip_fail(c, e, "timed step launch");
   }
   if (rc == 0) {

@@ -1,6 +1,6 @@
 // mhd_kernels.h -- internal interface between the host library (mhd_api.cpp,
-// compiled by the host C++ compiler) and the HIP kernels (mhd_kernels.hip).
-// Not part of the public C ABI (include/mhd_glm.h).
+// compiled by the host C++ compiler) and the HIP kernels (k_exact.hip,
+// k_fma.hip, k_mixed.hip).  Not part of the public C ABI (include/mhd_glm.h).
 #ifndef MHD_KERNELS_H
 #define MHD_KERNELS_H
 
@@ -11,12 +11,20 @@
 
 namespace mhd {
 
-enum Precision { PREC_FP64 = 0, PREC_FP32 = 1 };
+// Arithmetic forms of the step (values match enum mhd_precision).
+enum Mode {
+  MODE_FP64 = 0,         // FP64, contraction off: bitwise the reference CPU build
+  MODE_FP32 = 1,         // FP32, contraction off: bitwise the reference FP32 build
+  MODE_MIXED = 2,        // FP32 flux/reconstruction/GLM, FP64 state + update + time
+  MODE_MIXED_KAHAN = 3,  // as MIXED with an FP32 + FP32 compensated state
+  MODE_FP64_FMA = 4,     // FP64 with FMA contraction
+  MODE_COUNT = 5
+};
 enum RunState { RUN_ACTIVE = 0, RUN_DONE = 1, RUN_HALTED = 2 };
 
 // Device-resident step control for one solver instance.  T is the type of
-// time, dt, c_h and the CFL arithmetic: the arithmetic type of the run
-// (float for FP32, double for FP64), as in the reference (Real t, dt).
+// time, dt, c_h and the CFL arithmetic: float for FP32 (the reference keeps
+// t and dt in Real), double otherwise.
 template <class T>
 struct Ctrl {
   T t, dt, ch, t_end, cfl, dx, dy;
@@ -32,53 +40,65 @@ struct Ctrl {
   int fail_cell;                  // cell reported on halt
   int fail_step;
   T fail_t, fail_dt, fail_ch;
+  // Per-block partial maxima of the signal speed, spread over kSpeedSlots
+  // addresses (block b uses slot b % kSpeedSlots) so that tens of thousands
+  // of blocks do not serialise on one atomic; folded into speed_acc by the
+  // next control kernel.
+  static constexpr int kSpeedSlots = 64;
+  unsigned long long speed_slot[kSpeedSlots];
 };
 
 // Launch parameters of one solver instance (one grid).
 struct GridDesc {
   int nx, ny;
-  size_t plane;      // elements per variable plane (>= nx*ny)
-  int muscl, hlld;   // reconstruction order / Riemann solver
-  double alpha;      // GLM damping parameter
-  double tol;        // HLLD degeneracy tolerance
-  int tile;          // tile-shape id (see mhd_kernels.hip)
+  size_t plane;          // elements per variable plane (>= nx*ny)
+  int muscl, hlld;       // reconstruction order / Riemann solver
+  double alpha;          // GLM damping parameter
+  double tol;            // HLLD degeneracy tolerance
+  double lowbeta_ratio;  // mixed forms: FP64 recovery where E > ratio * e_int
+  int tile;              // tile-shape id
 };
 
-// Tile shapes available to the stage kernels.
-enum TileShape { TILE_16x16 = 0, TILE_32x8 = 1, TILE_16x8 = 2, TILE_COUNT = 3 };
+// Tile shapes of the stage kernels (cells TX x TY).  The face list of a
+// tile has 2*TX*TY + TX + TY entries; 16x15, 32x7 and 16x7 fit it into whole
+// waves evenly (511 / 487 / 247 faces over 256 / 256 / 128 lanes).
+enum TileShape {
+  TILE_16x16 = 0, TILE_32x8 = 1, TILE_16x8 = 2, TILE_16x15 = 3, TILE_32x7 = 4,
+  TILE_16x7 = 5, TILE_COUNT = 6
+};
+constexpr int kDefaultTile = TILE_16x15;
 
-// FP64 and FP32 entry points.  All launches go to `stream`; none synchronises.
-// State buffers are 9 planes of `plane` elements (SoA), cell (i,j) at j*nx+i.
-template <class R>
-hipError_t launch_speed(const GridDesc& g, const R* state, Ctrl<R>* ctrl,
-                        hipStream_t stream);
-// Moves the speed accumulated by launch_speed into the current-state slot.
-template <class R>
-hipError_t launch_commit_speed(Ctrl<R>* ctrl, hipStream_t stream);
-// Sets a new end time and re-prepares the coming step (no-op when halted).
-template <class R>
-hipError_t launch_set_target(Ctrl<R>* ctrl, double t_end, hipStream_t stream);
-template <class R>
-hipError_t launch_step(const GridDesc& g, R* u, R* u1, Ctrl<R>* ctrl,
-                       hipStream_t stream);
+// Launchers of one arithmetic form.  All launches go to `stream`; none
+// synchronises.  State buffers hold `state_planes` planes of `plane`
+// elements of `state_elem` bytes (SoA), cell (i,j) at j*nx+i.
+struct Launchers {
+  int mode;
+  size_t ctrl_bytes;
+  int time_is_float;
+  size_t state_elem;
+  int state_planes;
+  hipError_t (*speed)(const GridDesc&, const void* state, void* ctrl, hipStream_t);
+  hipError_t (*commit)(void* ctrl, hipStream_t);
+  hipError_t (*set_target)(void* ctrl, double t_end, hipStream_t);
+  // One step; with ev non-null an event is recorded before each of the three
+  // kernels and after the last (ev[0..3]).
+  hipError_t (*step)(const GridDesc&, void* u, void* u1, void* ctrl, hipStream_t,
+                     hipEvent_t* ev);
+};
 
-// One step with an event recorded before each of its three kernels and
-// after the last (ev[0..3]); for kernel timing.
-template <class R>
-hipError_t launch_step_timed(const GridDesc& g, R* u, R* u1, Ctrl<R>* ctrl,
-                             hipStream_t stream, hipEvent_t ev[4]);
+const Launchers* launchers_exact(int mode);  // MODE_FP64, MODE_FP32
+const Launchers* launchers_fma();            // MODE_FP64_FMA
+const Launchers* launchers_mixed(int mode);  // MODE_MIXED, MODE_MIXED_KAHAN
 
-// Operator-level batch kernels (device arrays, AoS 9 per state).
+// Operator-level batch kernels (exact arithmetic; device arrays, AoS 9 per
+// state).  R is double or float.
 template <class R>
-hipError_t launch_glm_flux_batch(int n, int dir, const R* UL, const R* UR,
-                                 R ch, int hlld, double tol, R* F, int* status,
-                                 hipStream_t stream);
+hipError_t launch_glm_flux_batch(int n, int dir, const R* UL, const R* UR, R ch, int hlld,
+                                 double tol, R* F, int* status, hipStream_t stream);
 template <class R>
-hipError_t launch_cons_to_prim_batch(int n, const R* U, R* W,
-                                     hipStream_t stream);
+hipError_t launch_cons_to_prim_batch(int n, const R* U, R* W, hipStream_t stream);
 template <class R>
-hipError_t launch_fast_speed_batch(int n, const R* W, int dir, R* out,
-                                   hipStream_t stream);
+hipError_t launch_fast_speed_batch(int n, const R* W, int dir, R* out, hipStream_t stream);
 
 }  // namespace mhd
 

@@ -1,0 +1,870 @@
+// mhd_step.inc.hip -- CDNA4 (gfx950) kernels of the 2D MHD-GLM SSPRK2 step.
+//
+// Included by three translation units, each compiled with its own floating
+// point flags and its own namespace MHD_NS:
+//   k_exact.hip  contraction off, IEEE div/sqrt  -> FP64 and FP32, bitwise
+//                the reference CPU build
+//   k_fma.hip    FMA contraction on               -> FP64 (relL2 <= 1e-12 bar)
+//   k_mixed.hip  contraction on, fast FP32 div/sqrt -> mixed precision forms
+//
+// One SSPRK2 step (mhd2d_glm_cpu.cpp:513-569) is three launches on one
+// stream, with no host synchronisation:
+//   k_stage<1>: U1 = U + dt L(U)                        + positivity of U1
+//   k_stage<2>: U  = U/2 + (U1 + dt L(U1))/2 (in place) + positivity of U
+//               + the max signal speed of the new U (next step's c_h)
+//   k_finish  : t += dt, step++; next c_h and dt, or halt with the
+//               reference's exit code (3: stage 1, 4: stage 2, 2: rate)
+// Each stage kernel is one fused tile pass: a TX x TY cell tile plus a
+// 2-cell halo is read once from HBM (SoA planes, coalesced rows), primitives
+// are recovered once per cell into LDS, every x and y face of the tile is
+// reconstructed (minmod MUSCL) and gets its GLM-HLLD flux from one merged,
+// direction-generic face list, fluxes go to LDS, and each cell is updated
+// from its four faces.  Only the state itself crosses HBM.
+//
+// A precision policy fixes the types: R (LDS primitives, reconstruction,
+// fluxes), S (stored state element), A (rhs and update arithmetic) and T
+// (time, dt, c_h: the control block).
+//
+// Kernels stop at entry once the device control block says the run is
+// done or halted, so a batch of steps (or a replayed graph) can be launched
+// without knowing in advance where t_end or a failure falls.
+#include "mhd_device.hpp"
+#include "mhd_kernels.h"
+
+#include <climits>
+
+#ifndef MHD_NS
+#error "MHD_NS must name the translation unit's namespace"
+#endif
+
+namespace mhd {
+namespace MHD_NS {
+
+// ---------------------------------------------------------------------------
+// Wave reductions: DPP within each 16-lane row, readlane across the 4 rows.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ unsigned dpp_quad_xor1(unsigned v) {
+  return (unsigned)__builtin_amdgcn_update_dpp(0, (int)v, 0xB1, 0xF, 0xF, false);
+}
+__device__ __forceinline__ unsigned dpp_quad_xor2(unsigned v) {
+  return (unsigned)__builtin_amdgcn_update_dpp(0, (int)v, 0x4E, 0xF, 0xF, false);
+}
+__device__ __forceinline__ unsigned dpp_row_ror4(unsigned v) {
+  return (unsigned)__builtin_amdgcn_update_dpp(0, (int)v, 0x124, 0xF, 0xF, false);
+}
+__device__ __forceinline__ unsigned dpp_row_ror8(unsigned v) {
+  return (unsigned)__builtin_amdgcn_update_dpp(0, (int)v, 0x128, 0xF, 0xF, false);
+}
+
+// Sum of a 32-bit value over the 64 lanes (all lanes active).
+__device__ __forceinline__ unsigned wave_sum_u32(unsigned v) {
+  v += dpp_quad_xor1(v);
+  v += dpp_quad_xor2(v);
+  v += dpp_row_ror4(v);
+  v += dpp_row_ror8(v);
+  return (unsigned)__builtin_amdgcn_readlane((int)v, 0) +
+         (unsigned)__builtin_amdgcn_readlane((int)v, 16) +
+         (unsigned)__builtin_amdgcn_readlane((int)v, 32) +
+         (unsigned)__builtin_amdgcn_readlane((int)v, 48);
+}
+
+__device__ __forceinline__ unsigned long long u64max(unsigned long long a,
+                                                     unsigned long long b) {
+  return a < b ? b : a;
+}
+
+// Max of a 64-bit unsigned value over the 64 lanes (all lanes active).
+__device__ __forceinline__ unsigned long long wave_max_u64(unsigned long long v) {
+#define MHD_DPP_MAX_STEP(OP)                                               \
+  {                                                                        \
+    unsigned lo = OP((unsigned)v), hi = OP((unsigned)(v >> 32));           \
+    v = u64max(v, ((unsigned long long)hi << 32) | lo);                    \
+  }
+  MHD_DPP_MAX_STEP(dpp_quad_xor1)
+  MHD_DPP_MAX_STEP(dpp_quad_xor2)
+  MHD_DPP_MAX_STEP(dpp_row_ror4)
+  MHD_DPP_MAX_STEP(dpp_row_ror8)
+#undef MHD_DPP_MAX_STEP
+  unsigned long long r = 0;
+#pragma unroll
+  for (int row = 0; row < 4; row++) {
+    unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)v, row * 16);
+    unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(v >> 32), row * 16);
+    r = u64max(r, ((unsigned long long)hi << 32) | lo);
+  }
+  return r;
+}
+
+// Bits of a non-negative value, ordered like the value.
+__device__ __forceinline__ unsigned long long speed_bits(double v) {
+  return (unsigned long long)__double_as_longlong(v);
+}
+__device__ __forceinline__ unsigned long long speed_bits(float v) {
+  return (unsigned long long)(unsigned)__float_as_uint(v);
+}
+template <class T> __device__ __forceinline__ T bits_speed(unsigned long long b);
+template <> __device__ __forceinline__ double bits_speed<double>(unsigned long long b) {
+  return __longlong_as_double((long long)b);
+}
+template <> __device__ __forceinline__ float bits_speed<float>(unsigned long long b) {
+  return __uint_as_float((unsigned)b);
+}
+
+template <class R>
+__device__ __forceinline__ Phys<R> make_phys(double tol) {
+  Phys<R> c;
+  c.gamma = (R)(5.0 / 3.0);
+  c.gm1 = c.gamma - (R)1.0;
+  c.rho_floor = (R)1.0e-12;
+  c.p_floor = (R)1.0e-12;
+  c.tol = (R)tol;
+  return c;
+}
+
+// Per-cell contribution to the signal-speed maximum of
+// maxPhysicalSignalSpeed (mhd2d_glm_cpu.cpp:289-305): max(sx, sy) in the
+// std::max form.  A NaN never enters the reference's running maximum and
+// the running maximum starts at +0, so NaN and zero both contribute +0
+// here, and `valid` records whether any cell was not NaN (the reference
+// then reports a non-positive rate).
+template <class T>
+__device__ __forceinline__ T cell_speed(const Phys<T>& c, const T P[9], bool& valid) {
+  const T sx = fabs_(P[1]) + fast_speed(c, P, P[5]);
+  const T sy = fabs_(P[2]) + fast_speed(c, P, P[6]);
+  const T m = smax(sx, sy);
+  valid = (m == m);
+  return (m > (T)0.0) ? m : (T)0.0;
+}
+
+__device__ __forceinline__ int wrap(int i, int n) {
+  i %= n;
+  return i < 0 ? i + n : i;
+}
+
+// Bijective block -> tile remap so that the tiles one XCD receives (blocks
+// are dealt round-robin over the 8 XCDs) form one contiguous band of tile
+// rows, and halo re-reads between neighbouring tiles hit that XCD's L2.
+__device__ __forceinline__ int xcd_tile(int bid, int nblocks) {
+  const int q = nblocks >> 3, r = nblocks & 7, x = bid & 7;
+  return x * q + (x < r ? x : r) + (bid >> 3);
+}
+
+// Publishes a block's signal-speed maximum: one slot per block residue, and
+// the atomic only when the slot does not already hold a larger value (the
+// maximum settles after the first blocks, so most blocks skip the atomic).
+template <class T>
+__device__ __forceinline__ void publish_speed(Ctrl<T>* ctrl, unsigned long long m, bool valid) {
+  unsigned long long* slot = &ctrl->speed_slot[blockIdx.x % Ctrl<T>::kSpeedSlots];
+  if (m > __atomic_load_n(slot, __ATOMIC_RELAXED)) atomicMax(slot, m);
+  if (valid && __atomic_load_n(&ctrl->valid_acc, __ATOMIC_RELAXED) == 0)
+    atomicOr(&ctrl->valid_acc, 1);
+}
+
+// Folds the slots into speed_acc and clears them (single thread).
+template <class T>
+__device__ __forceinline__ void fold_speed_slots(Ctrl<T>* ctrl) {
+  unsigned long long m = ctrl->speed_acc;
+  for (int q = 0; q < Ctrl<T>::kSpeedSlots; q++) {
+    m = u64max(m, ctrl->speed_slot[q]);
+    ctrl->speed_slot[q] = 0;
+  }
+  ctrl->speed_acc = m;
+}
+
+// ---------------------------------------------------------------------------
+// FP64 checks and signal speed of the mixed forms.  These need FP64-level
+// accuracy (c_h sets dt, which every cell sees), not the reference's
+// operation sequence: one reciprocal of rho shared by all quotients and
+// square roots from v_rsq_f64 with one Newton correction (~1 ulp).
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ double sqrt_nr(double x) {
+  const double y = __builtin_amdgcn_rsq(x);
+  const double s0 = x * y;
+  const double s = s0 + (0.5 * y) * (x - s0 * s0);
+  return (x > 0.0 && x < __builtin_inf()) ? s : x;
+}
+
+// Usability (rho > floor, p > floor, all finite) and, when wanted, the
+// cell's signal speed max(|vx|+cf_x, |vy|+cf_y), of an FP64 state.
+__device__ __forceinline__ bool usable_and_speed_fp64(const double U[9], bool want_speed,
+                                                      double& speed, bool& valid) {
+  const double gamma = 5.0 / 3.0, gm1 = gamma - 1.0, floor_ = 1.0e-12;
+  const double rho = smax(U[RHO], floor_);
+  const double inv = 1.0 / rho;
+  const double vx = U[MX] * inv, vy = U[MY] * inv, vz = U[MZ] * inv;
+  const double bx = U[BX], by = U[BY], bz = U[BZ];
+  const double v2 = vx * vx + vy * vy + vz * vz, b2 = bx * bx + by * by + bz * bz;
+  const double p = gm1 * (U[EN] - 0.5 * rho * v2 - 0.5 * b2);
+  bool ok = U[RHO] > floor_ && p > floor_ && finite_(p) && finite_(v2);
+#pragma unroll
+  for (int k = 0; k < 9; k++) ok = ok && finite_(U[k]);
+  speed = 0.0;
+  valid = false;
+  if (want_speed) {
+    const double a2 = gamma * smax(p, floor_) * inv, bb = b2 * inv, s2 = a2 + bb;
+    const double dxn = smax(s2 * s2 - 4.0 * a2 * (bx * bx * inv), 0.0);
+    const double dyn = smax(s2 * s2 - 4.0 * a2 * (by * by * inv), 0.0);
+    const double cfx = sqrt_nr(smax(0.5 * (s2 + sqrt_nr(dxn)), 0.0));
+    const double cfy = sqrt_nr(smax(0.5 * (s2 + sqrt_nr(dyn)), 0.0));
+    const double m = smax(fabs_(vx) + cfx, fabs_(vy) + cfy);
+    valid = (m == m);
+    speed = (m > 0.0) ? m : 0.0;
+  }
+  return ok;
+}
+
+// ---------------------------------------------------------------------------
+// Float-float (unevaluated hi + lo) accumulation for the compensated form.
+// Only additions: unaffected by FMA contraction.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ void two_sum(float a, float b, float& s, float& e) {
+  s = a + b;
+  const float bb = s - a;
+  e = (a - (s - bb)) + (b - bb);
+}
+// (hi, lo) + d
+__device__ __forceinline__ void ff_add_f(float hi, float lo, float d, float& rh, float& rl) {
+  float s, e;
+  two_sum(hi, d, s, e);
+  e += lo;
+  rh = s + e;
+  rl = e - (rh - s);
+}
+// (ah, al) + (bh, bl)
+__device__ __forceinline__ void ff_add(float ah, float al, float bh, float bl, float& rh,
+                                       float& rl) {
+  float s, e;
+  two_sum(ah, bh, s, e);
+  e += al + bl;
+  rh = s + e;
+  rl = e - (rh - s);
+}
+
+// ---------------------------------------------------------------------------
+// Precision policies.
+// ---------------------------------------------------------------------------
+// Plain FP64 / FP32: everything in R, state stored in R.
+template <class R_>
+struct ExactPol {
+  using R = R_;
+  using T = R_;
+  using S = R_;
+  static constexpr int kPlanes = 9;
+  static constexpr int kMode = sizeof(R_) == 8 ? 0 : 1;
+};
+// Mixed: FP32 reconstruction / HLLD / GLM; FP64 state, update accumulation,
+// time; FP64 pressure recovery where FP32 would cancel.
+struct MixedPol {
+  using R = float;
+  using T = double;
+  using S = double;
+  static constexpr int kPlanes = 9;
+};
+// Compensated: as Mixed, but the state is FP32 hi + FP32 lo (planes 0-8 and
+// 9-17), accumulated with float-float additions.
+struct KahanPol {
+  using R = float;
+  using T = double;
+  using S = float;
+  static constexpr int kPlanes = 18;
+};
+
+template <class Pol>
+struct StageArgs {
+  const typename Pol::S* src;   // state L() is evaluated on: U (stage 1) or U1 (stage 2)
+  const typename Pol::S* uold;  // stage 2: U^n (same buffer as dst)
+  typename Pol::S* dst;         // stage 1: U1; stage 2: U
+  size_t plane;
+  int nx, ny, tiles_x, ntiles;
+  int pow2;                 // dx and dy are powers of two: x/dx == x*(1/dx) exactly
+  double inv_dx, inv_dy;
+  double tol;
+  double alpha;
+  float lowbeta_ratio;      // mixed: FP64 recovery where E > ratio * e_internal
+  Ctrl<typename Pol::T>* ctrl;
+};
+
+// Cell state as loaded from HBM, widened to double where the policy stores
+// more than the flux type.
+template <class Pol> struct CellIO;
+
+template <class R>
+struct CellIO<ExactPol<R>> {
+  R v[9];
+  __device__ __forceinline__ void load(const R* p, size_t ps, size_t g) {
+#pragma unroll
+    for (int k = 0; k < 9; k++) v[k] = p[k * ps + g];
+  }
+  __device__ __forceinline__ R psi() const { return v[PSI]; }
+};
+
+template <>
+struct CellIO<MixedPol> {
+  double v[9];
+  __device__ __forceinline__ void load(const double* p, size_t ps, size_t g) {
+#pragma unroll
+    for (int k = 0; k < 9; k++) v[k] = p[k * ps + g];
+  }
+  __device__ __forceinline__ void wide(double out[9]) const {
+#pragma unroll
+    for (int k = 0; k < 9; k++) out[k] = v[k];
+  }
+};
+
+template <>
+struct CellIO<KahanPol> {
+  float hi[9], lo[9];
+  __device__ __forceinline__ void load(const float* p, size_t ps, size_t g) {
+#pragma unroll
+    for (int k = 0; k < 9; k++) {
+      hi[k] = p[k * ps + g];
+      lo[k] = p[(9 + k) * ps + g];
+    }
+  }
+  __device__ __forceinline__ void wide(double out[9]) const {
+#pragma unroll
+    for (int k = 0; k < 9; k++) out[k] = (double)hi[k] + (double)lo[k];
+  }
+};
+
+// Phase A recovery: primitives in the flux type.
+template <bool MUSCL, class R>
+__device__ __forceinline__ void recover(const Phys<R>& c, const CellIO<ExactPol<R>>& u,
+                                        float, R P[9]) {
+  if (MUSCL) {
+    cons_to_prim<R, 9>(c, u.v, P);
+  } else {
+#pragma unroll
+    for (int k = 0; k < 9; k++) P[k] = u.v[k];
+  }
+}
+
+// Mixed recovery: FP32 unless the internal energy is a small remainder of E
+// (low plasma beta or high Mach number: E > ratio * e_int, or e_int <= 0),
+// where p = (gamma-1)(E - rho v^2/2 - B^2/2) is recomputed in FP64.
+template <class Pol, bool MUSCL>
+__device__ __forceinline__ void recover_mixed(const Phys<float>& c, const CellIO<Pol>& u,
+                                              float ratio, float P[9]) {
+  double d[9];
+  u.wide(d);
+  float f[9];
+#pragma unroll
+  for (int k = 0; k < 9; k++) f[k] = (float)d[k];
+  if (!MUSCL) {
+#pragma unroll
+    for (int k = 0; k < 9; k++) P[k] = f[k];
+    return;
+  }
+  cons_to_prim<float, 9>(c, f, P);
+  if (!(P[4] * ratio > c.gm1 * f[EN])) {
+    const Phys<double> cd = make_phys<double>(0.0);
+    double W[9];
+    cons_to_prim<double, 9>(cd, d, W);
+#pragma unroll
+    for (int k = 0; k < 9; k++) P[k] = (float)W[k];
+  }
+}
+
+template <bool MUSCL>
+__device__ __forceinline__ void recover(const Phys<float>& c, const CellIO<MixedPol>& u,
+                                        float ratio, float P[9]) {
+  recover_mixed<MixedPol, MUSCL>(c, u, ratio, P);
+}
+template <bool MUSCL>
+__device__ __forceinline__ void recover(const Phys<float>& c, const CellIO<KahanPol>& u,
+                                        float ratio, float P[9]) {
+  recover_mixed<KahanPol, MUSCL>(c, u, ratio, P);
+}
+
+// Face-flux differences of one cell: -(F_r - F_l)/dx - (G_t - G_b)/dy
+// (mhd2d_glm_cpu.cpp:278-281) in type A.
+template <class A, class R>
+__device__ __forceinline__ A flux_div(R fr, R fl, R gt, R gb, bool pow2, A dx, A dy, A idx,
+                                      A idy) {
+  if (pow2) return -((A)fr - (A)fl) * idx - ((A)gt - (A)gb) * idy;
+  return -((A)fr - (A)fl) / dx - ((A)gt - (A)gb) / dy;
+}
+
+// Phase C update of one cell (exact policy): rhs, stage update, store.
+// Returns the new state in `out` (as T) for the positivity/speed checks.
+template <int STAGE, class R>
+__device__ __forceinline__ void update_cell(const StageArgs<ExactPol<R>>& a,
+                                            const CellIO<ExactPol<R>>& src, size_t g,
+                                            const R rhs_flux[9], R dt, R damping, R out[9]) {
+  CellIO<ExactPol<R>> old;
+  if (STAGE == 2) old.load(a.uold, a.plane, g);
+#pragma unroll
+  for (int k = 0; k < 9; k++) {
+    R rhs = rhs_flux[k];
+    if (k == PSI) rhs -= damping * src.psi();
+    if (STAGE == 1)
+      out[k] = src.v[k] + dt * rhs;
+    else
+      out[k] = (R)0.5 * old.v[k] + (R)0.5 * (src.v[k] + dt * rhs);
+  }
+#pragma unroll
+  for (int k = 0; k < 9; k++) a.dst[k * a.plane + g] = out[k];
+}
+
+template <int STAGE>
+__device__ __forceinline__ void update_cell(const StageArgs<MixedPol>& a,
+                                            const CellIO<MixedPol>& src, size_t g,
+                                            const double rhs_flux[9], double dt,
+                                            double damping, double out[9]) {
+  CellIO<MixedPol> old;
+  if (STAGE == 2) old.load(a.uold, a.plane, g);
+#pragma unroll
+  for (int k = 0; k < 9; k++) {
+    double rhs = rhs_flux[k];
+    if (k == PSI) rhs -= damping * src.v[PSI];
+    if (STAGE == 1)
+      out[k] = src.v[k] + dt * rhs;
+    else
+      out[k] = 0.5 * old.v[k] + 0.5 * (src.v[k] + dt * rhs);
+  }
+#pragma unroll
+  for (int k = 0; k < 9; k++) a.dst[k * a.plane + g] = out[k];
+}
+
+template <int STAGE>
+__device__ __forceinline__ void update_cell(const StageArgs<KahanPol>& a,
+                                            const CellIO<KahanPol>& src, size_t g,
+                                            const float rhs_flux[9], double dt,
+                                            double damping, double out[9]) {
+  CellIO<KahanPol> old;
+  if (STAGE == 2) old.load(a.uold, a.plane, g);
+  const float dtf = (float)dt, dampf = (float)damping;
+#pragma unroll
+  for (int k = 0; k < 9; k++) {
+    float rhs = rhs_flux[k];
+    if (k == PSI) rhs -= dampf * src.hi[PSI];
+    const float d = dtf * rhs;
+    float h, l;
+    ff_add_f(src.hi[k], src.lo[k], d, h, l);
+    if (STAGE == 2) ff_add(0.5f * old.hi[k], 0.5f * old.lo[k], 0.5f * h, 0.5f * l, h, l);
+    a.dst[k * a.plane + g] = h;
+    a.dst[(9 + k) * a.plane + g] = l;
+    out[k] = (double)h + (double)l;
+  }
+}
+
+// rhs type of each policy.
+template <class Pol> struct RhsType { using type = typename Pol::T; };
+template <> struct RhsType<KahanPol> { using type = float; };
+
+template <class Pol, bool MUSCL, bool HLLD, int STAGE, int TX, int TY>
+__global__ __launch_bounds__((TX * TY + 63) / 64 * 64) void k_stage(StageArgs<Pol> a) {
+  using R = typename Pol::R;
+  using T = typename Pol::T;
+  using A = typename RhsType<Pol>::type;
+  constexpr int NT = TX * TY;
+  constexpr int HX = TX + 4, HY = TY + 4, HN = HX * HY;
+  constexpr int NXF = (TX + 1) * TY, NYF = TX * (TY + 1), NF = NXF + NYF;
+  constexpr int NW = (NT + 63) / 64;
+  constexpr int NTHR = NW * 64;  // threads per block: whole waves
+  __shared__ R sP[9 * HN];
+  __shared__ R sF[9 * NF];
+  __shared__ unsigned long long sRed[3 * NW];
+
+  Ctrl<T>* ctrl = a.ctrl;
+  if (ctrl->run != RUN_ACTIVE) return;
+  if (STAGE == 2 && ctrl->bad[0] != INT_MAX) return;
+  const T ch_t = ctrl->ch, dt = ctrl->dt, dx = ctrl->dx, dy = ctrl->dy;
+  const R ch = (R)ch_t;
+  const Phys<R> c = make_phys<R>(a.tol);
+  const int nx = a.nx, ny = a.ny;
+  const size_t ps = a.plane;
+  const int tile = xcd_tile((int)blockIdx.x, a.ntiles);
+  const int ty = tile / a.tiles_x, tx = tile - ty * a.tiles_x;
+  const int i0 = tx * TX, j0 = ty * TY;
+  const int tid = (int)threadIdx.x;
+
+  // Phase A: tile + halo (no corners) -> primitives in LDS.
+  for (int h = tid; h < HN; h += NTHR) {
+    const int r = h / HX, cc = h - r * HX;
+    if ((r < 2 || r >= TY + 2) && (cc < 2 || cc >= TX + 2)) continue;
+    const size_t g = (size_t)wrap(j0 + r - 2, ny) * nx + wrap(i0 + cc - 2, nx);
+    CellIO<Pol> u;
+    u.load(a.src, ps, g);
+    R P[9];
+    recover<MUSCL>(c, u, a.lowbeta_ratio, P);
+#pragma unroll
+    for (int k = 0; k < 9; k++) sP[k * HN + h] = P[k];
+  }
+  __syncthreads();
+
+  // Phase B: faces.  Faces [0, NXF) are x faces (row r, face column fc
+  // between local cells fc-1 and fc); [NXF, NF) are y faces (column cc,
+  // face row fr between local rows fr-1 and fr).  A y face is evaluated in
+  // the x frame by reading the (1,2) and (5,6) slots swapped.
+  unsigned degen = 0, nonphys = 0;
+  for (int f = tid; f < NF; f += NTHR) {
+    const bool ydir = f >= NXF;
+    int base, stride;
+    bool owned;
+    if (!ydir) {
+      const int r = f / (TX + 1), fc = f - r * (TX + 1);
+      base = (r + 2) * HX + fc;
+      stride = 1;
+      owned = (j0 + r < ny) && (fc >= 1 ? (i0 + fc - 1 < nx) : (i0 == 0));
+    } else {
+      const int g = f - NXF, fr = g / TX, cc = g - fr * TX;
+      base = fr * HX + cc + 2;
+      stride = HX;
+      owned = (i0 + cc < nx) && (fr >= 1 ? (j0 + fr - 1 < ny) : (j0 == 0));
+    }
+    const int q1 = (ydir ? 2 : 1) * HN, q2 = (ydir ? 1 : 2) * HN;
+    const int q5 = (ydir ? 6 : 5) * HN, q6 = (ydir ? 5 : 6) * HN;
+    auto load = [&](int cell, R out[9]) {
+      out[0] = sP[cell];
+      out[1] = sP[q1 + cell];
+      out[2] = sP[q2 + cell];
+      out[3] = sP[3 * HN + cell];
+      out[4] = sP[4 * HN + cell];
+      out[5] = sP[q5 + cell];
+      out[6] = sP[q6 + cell];
+      out[7] = sP[7 * HN + cell];
+      out[8] = sP[8 * HN + cell];
+    };
+    R UL[9], UR[9];
+    if (MUSCL) {
+      R Pa[9], Pb[9], Pc[9], Pf[9];
+      load(base, Pa);
+      load(base + stride, Pb);
+      load(base + 2 * stride, Pc);
+      reconstruct_face<R, true>(c, Pa, Pb, Pc, Pf);
+      prim_to_cons(c, Pf, UL);
+      load(base + 3 * stride, Pa);
+      reconstruct_face<R, false>(c, Pb, Pc, Pa, Pf);
+      prim_to_cons(c, Pf, UR);
+    } else {
+      load(base + stride, UL);
+      load(base + 2 * stride, UR);
+    }
+    R F[9];
+    const int st = glm_flux_x<R, HLLD>(c, UL, UR, ch, F);
+    if (HLLD && owned) {
+      degen += (st == HLLD_DEGENERATE);
+      nonphys += (st == HLLD_NONPHYSICAL);
+    }
+    const int p1 = (ydir ? 2 : 1) * NF, p2 = (ydir ? 1 : 2) * NF;
+    const int p5 = (ydir ? 6 : 5) * NF, p6 = (ydir ? 5 : 6) * NF;
+    sF[f] = F[0];
+    sF[p1 + f] = F[1];
+    sF[p2 + f] = F[2];
+    sF[3 * NF + f] = F[3];
+    sF[4 * NF + f] = F[4];
+    sF[p5 + f] = F[5];
+    sF[p6 + f] = F[6];
+    sF[7 * NF + f] = F[7];
+    sF[8 * NF + f] = F[8];
+  }
+  __syncthreads();
+
+  // Phase C: one cell per thread.  rhs (mhd2d_glm_cpu.cpp:273-284), the
+  // stage update (528-535 / 549-557) and the positivity check (328-345).
+  const int r = tid / TX, cc = tid - r * TX;
+  const int gi = i0 + cc, gj = j0 + r;
+  unsigned long long sbits = 0;
+  unsigned svalid = 0;
+  if (tid < NT && gi < nx && gj < ny) {
+    const int xl = r * (TX + 1) + cc, xr = xl + 1;
+    const int yb = NXF + r * TX + cc, yt = yb + TX;
+    const size_t g = (size_t)gj * nx + gi;
+    const A adx = (A)dx, ady = (A)dy, aidx = (A)a.inv_dx, aidy = (A)a.inv_dy;
+    const bool pow2 = a.pow2 != 0;
+    A rhs[9];
+#pragma unroll
+    for (int k = 0; k < 9; k++)
+      rhs[k] = flux_div<A, R>(sF[k * NF + xr], sF[k * NF + xl], sF[k * NF + yt],
+                              sF[k * NF + yb], pow2, adx, ady, aidx, aidy);
+    CellIO<Pol> src;
+    src.load(a.src, ps, g);
+    const T damping = ch_t / smax((T)a.alpha, (T)1.0e-12);
+    T out[9];
+    update_cell<STAGE>(a, src, g, rhs, dt, damping, out);
+    if constexpr (sizeof(R) == sizeof(T)) {
+      // Exact forms: the reference's check (mhd2d_glm_cpu.cpp:328-345) and
+      // wave speeds (289-305), bitwise.
+      const Phys<T> ct = make_phys<T>(a.tol);
+      T P[9];
+      if (!state_usable(ct, out, P)) atomicMin(&ctrl->bad[STAGE - 1], (int)g);
+      if (STAGE == 2) {
+        bool valid;
+        const T s = cell_speed(ct, P, valid);
+        sbits = speed_bits(s);
+        svalid = valid ? 1u : 0u;
+      }
+    } else {
+      double speed;
+      bool valid;
+      if (!usable_and_speed_fp64(out, STAGE == 2, speed, valid))
+        atomicMin(&ctrl->bad[STAGE - 1], (int)g);
+      if (STAGE == 2) {
+        sbits = speed_bits(speed);
+        svalid = valid ? 1u : 0u;
+      }
+    }
+  }
+
+  // Block reductions: one atomic per block and counter.
+  const int lane = tid & 63, wid = tid >> 6;
+  const unsigned wd = wave_sum_u32(degen), wn = wave_sum_u32(nonphys);
+  unsigned long long wm = 0;
+  unsigned wv = 0;
+  if (STAGE == 2) {
+    wm = wave_max_u64(sbits);
+    wv = wave_sum_u32(svalid);
+  }
+  if (lane == 0) {
+    sRed[wid] = ((unsigned long long)wn << 32) | wd;
+    sRed[NW + wid] = wm;
+    sRed[2 * NW + wid] = wv;
+  }
+  __syncthreads();
+  if (tid == 0) {
+    unsigned long long d = 0, n = 0, m = 0, v = 0;
+    for (int w = 0; w < NW; w++) {
+      d += sRed[w] & 0xffffffffull;
+      n += sRed[w] >> 32;
+      m = u64max(m, sRed[NW + w]);
+      v += sRed[2 * NW + w];
+    }
+    if (d) atomicAdd(&ctrl->degenerate, d);
+    if (n) atomicAdd(&ctrl->nonphysical, n);
+    if (STAGE == 2) publish_speed(ctrl, m, v != 0);
+  }
+}
+
+// Signal-speed maximum of a state (initial state / after set_state), in T.
+template <class Pol>
+__device__ __forceinline__ void load_wide(const typename Pol::S* src, size_t ps, size_t g,
+                                          typename Pol::T out[9]) {
+  CellIO<Pol> u;
+  u.load(src, ps, g);
+  if constexpr (sizeof(typename Pol::S) == sizeof(typename Pol::T) && Pol::kPlanes == 9) {
+#pragma unroll
+    for (int k = 0; k < 9; k++) out[k] = u.v[k];
+  } else {
+    u.wide(out);
+  }
+}
+
+template <class Pol>
+__global__ __launch_bounds__(256) void k_speed(const typename Pol::S* src, size_t plane,
+                                               int nx, int ny, double tol,
+                                               Ctrl<typename Pol::T>* ctrl) {
+  using T = typename Pol::T;
+  __shared__ unsigned long long sRed[2 * 4];
+  const Phys<T> c = make_phys<T>(tol);
+  const long long n = (long long)nx * ny;
+  unsigned long long best = 0;
+  unsigned valid_any = 0;
+  for (long long q = (long long)blockIdx.x * 256 + threadIdx.x; q < n;
+       q += (long long)gridDim.x * 256) {
+    T U[9], P[9];
+    load_wide<Pol>(src, plane, (size_t)q, U);
+    cons_to_prim<T, 9>(c, U, P);
+    bool valid;
+    const T s = cell_speed(c, P, valid);
+    best = u64max(best, speed_bits(s));
+    valid_any |= valid ? 1u : 0u;
+  }
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  const unsigned long long wm = wave_max_u64(best);
+  const unsigned wv = wave_sum_u32(valid_any);
+  if (lane == 0) { sRed[wid] = wm; sRed[4 + wid] = wv; }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    unsigned long long m = 0, v = 0;
+    for (int w = 0; w < 4; w++) { m = u64max(m, sRed[w]); v += sRed[4 + w]; }
+    publish_speed(ctrl, m, v != 0);
+  }
+}
+
+// c_h and dt of the coming step (mhd2d_glm_cpu.cpp:514-524).  The rate
+// maximum of maxSignalRate (307-326) is exact in closed form: every cell's
+// speed is <= c_h, so max(s, c_h) == c_h bitwise and each cell's rate is
+// c_h/dx + c_h/dy; NaN cells never enter the running maximum, so with no
+// non-NaN cell the maximum stays 0.
+template <class T>
+__device__ void prepare_step(Ctrl<T>* ctrl) {
+  ctrl->bad[0] = INT_MAX;
+  ctrl->bad[1] = INT_MAX;
+  if (!(ctrl->t < ctrl->t_end)) {
+    ctrl->run = RUN_DONE;
+    return;
+  }
+  const T speed = bits_speed<T>(ctrl->speed_cur);
+  const T ch = smax(speed, (T)1.0e-12);
+  const T r = ch / ctrl->dx + ch / ctrl->dy;
+  const T rate = ctrl->valid_cur ? smax((T)0.0, r) : (T)0.0;
+  ctrl->ch = ch;
+  if (rate <= (T)0.0) {
+    ctrl->run = RUN_HALTED;
+    ctrl->code = 2;
+    ctrl->fail_step = ctrl->step;
+    ctrl->fail_t = ctrl->t;
+    ctrl->fail_ch = ch;
+    return;
+  }
+  T dt = ctrl->cfl / rate;
+  if (ctrl->t + dt > ctrl->t_end) dt = ctrl->t_end - ctrl->t;
+  ctrl->dt = dt;
+}
+
+template <class T>
+__global__ void k_commit_speed(Ctrl<T>* ctrl) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  fold_speed_slots(ctrl);
+  ctrl->speed_cur = ctrl->speed_acc;
+  ctrl->valid_cur = ctrl->valid_acc;
+  ctrl->speed_acc = 0;
+  ctrl->valid_acc = 0;
+}
+
+template <class T>
+__global__ void k_set_target(Ctrl<T>* ctrl, double t_end) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  if (ctrl->run == RUN_HALTED) return;
+  ctrl->t_end = (T)t_end;
+  ctrl->run = RUN_ACTIVE;
+  prepare_step(ctrl);
+}
+
+template <class T>
+__global__ void k_finish(Ctrl<T>* ctrl) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  if (ctrl->run != RUN_ACTIVE) return;
+  if (ctrl->bad[0] != INT_MAX || ctrl->bad[1] != INT_MAX) {
+    const bool first = ctrl->bad[0] != INT_MAX;
+    ctrl->run = RUN_HALTED;
+    ctrl->code = first ? 3 : 4;
+    ctrl->fail_cell = first ? ctrl->bad[0] : ctrl->bad[1];
+    ctrl->fail_step = ctrl->step;
+    ctrl->fail_t = ctrl->t;
+    ctrl->fail_dt = ctrl->dt;
+    ctrl->fail_ch = ctrl->ch;
+    return;
+  }
+  ctrl->t = ctrl->t + ctrl->dt;
+  ctrl->step += 1;
+  fold_speed_slots(ctrl);
+  ctrl->speed_cur = ctrl->speed_acc;
+  ctrl->valid_cur = ctrl->valid_acc;
+  ctrl->speed_acc = 0;
+  ctrl->valid_acc = 0;
+  prepare_step(ctrl);
+}
+
+// ---------------------------------------------------------------------------
+// Host launchers.
+// ---------------------------------------------------------------------------
+template <class Pol, bool MUSCL, bool HLLD, int TX, int TY>
+static hipError_t launch_step_t(const GridDesc& g, void* u, void* u1, void* ctrl,
+                                hipStream_t s, hipEvent_t* ev) {
+  using S = typename Pol::S;
+  using T = typename Pol::T;
+  const int tiles_x = (g.nx + TX - 1) / TX, tiles_y = (g.ny + TY - 1) / TY;
+  StageArgs<Pol> a;
+  a.plane = g.plane;
+  a.nx = g.nx;
+  a.ny = g.ny;
+  a.tiles_x = tiles_x;
+  a.ntiles = tiles_x * tiles_y;
+  a.tol = g.tol;
+  a.alpha = g.alpha;
+  a.lowbeta_ratio = (float)g.lowbeta_ratio;
+  a.ctrl = (Ctrl<T>*)ctrl;
+  // dx = (1-0)/nx in the control type, as the reference computes it.  When
+  // nx and ny are powers of two, dx and dy are exact powers of two and
+  // dividing by them equals multiplying by their exact inverses.
+  const T dx = ((T)1.0 - (T)0.0) / (T)g.nx, dy = ((T)1.0 - (T)0.0) / (T)g.ny;
+  a.pow2 = ((g.nx & (g.nx - 1)) == 0 && (g.ny & (g.ny - 1)) == 0) ? 1 : 0;
+  a.inv_dx = (double)((T)1.0 / dx);
+  a.inv_dy = (double)((T)1.0 / dy);
+  a.src = (const S*)u;
+  a.uold = (const S*)u;
+  a.dst = (S*)u1;
+  constexpr int nthr = (TX * TY + 63) / 64 * 64;
+  if (ev) (void)hipEventRecord(ev[0], s);
+  hipLaunchKernelGGL((k_stage<Pol, MUSCL, HLLD, 1, TX, TY>), dim3(a.ntiles), dim3(nthr), 0, s,
+                     a);
+  if (ev) (void)hipEventRecord(ev[1], s);
+  a.src = (const S*)u1;
+  a.uold = (const S*)u;
+  a.dst = (S*)u;
+  hipLaunchKernelGGL((k_stage<Pol, MUSCL, HLLD, 2, TX, TY>), dim3(a.ntiles), dim3(nthr), 0, s,
+                     a);
+  if (ev) (void)hipEventRecord(ev[2], s);
+  hipLaunchKernelGGL(k_finish<T>, dim3(1), dim3(64), 0, s, (Ctrl<T>*)ctrl);
+  if (ev) (void)hipEventRecord(ev[3], s);
+  return hipGetLastError();
+}
+
+template <class Pol, int TX, int TY>
+static hipError_t launch_step_tile(const GridDesc& g, void* u, void* u1, void* ctrl,
+                                   hipStream_t s, hipEvent_t* ev) {
+  if (g.muscl && g.hlld) return launch_step_t<Pol, true, true, TX, TY>(g, u, u1, ctrl, s, ev);
+  if (g.muscl) return launch_step_t<Pol, true, false, TX, TY>(g, u, u1, ctrl, s, ev);
+  if (g.hlld) return launch_step_t<Pol, false, true, TX, TY>(g, u, u1, ctrl, s, ev);
+  return launch_step_t<Pol, false, false, TX, TY>(g, u, u1, ctrl, s, ev);
+}
+
+// Every tile shape is instantiated for the production scheme (MUSCL +
+// HLLD); the other order/solver combinations use the default tile.
+template <class Pol>
+static hipError_t launch_step(const GridDesc& g, void* u, void* u1, void* ctrl,
+                              hipStream_t s, hipEvent_t* ev) {
+  if (!(g.muscl && g.hlld)) return launch_step_tile<Pol, 16, 15>(g, u, u1, ctrl, s, ev);
+  switch (g.tile) {
+    case TILE_16x16: return launch_step_t<Pol, true, true, 16, 16>(g, u, u1, ctrl, s, ev);
+    case TILE_32x8: return launch_step_t<Pol, true, true, 32, 8>(g, u, u1, ctrl, s, ev);
+    case TILE_16x8: return launch_step_t<Pol, true, true, 16, 8>(g, u, u1, ctrl, s, ev);
+    case TILE_32x7: return launch_step_t<Pol, true, true, 32, 7>(g, u, u1, ctrl, s, ev);
+    case TILE_16x7: return launch_step_t<Pol, true, true, 16, 7>(g, u, u1, ctrl, s, ev);
+    default: return launch_step_t<Pol, true, true, 16, 15>(g, u, u1, ctrl, s, ev);
+  }
+}
+
+template <class Pol>
+static hipError_t launch_speed(const GridDesc& g, const void* state, void* ctrl,
+                               hipStream_t s) {
+  const long long n = (long long)g.nx * g.ny;
+  long long blocks = (n + 255) / 256;
+  if (blocks > 8192) blocks = 8192;
+  hipLaunchKernelGGL(k_speed<Pol>, dim3((unsigned)blocks), dim3(256), 0, s,
+                     (const typename Pol::S*)state, g.plane, g.nx, g.ny, g.tol,
+                     (Ctrl<typename Pol::T>*)ctrl);
+  return hipGetLastError();
+}
+
+template <class T>
+static hipError_t launch_commit(void* ctrl, hipStream_t s) {
+  hipLaunchKernelGGL(k_commit_speed<T>, dim3(1), dim3(64), 0, s, (Ctrl<T>*)ctrl);
+  return hipGetLastError();
+}
+
+template <class T>
+static hipError_t launch_target(void* ctrl, double t_end, hipStream_t s) {
+  hipLaunchKernelGGL(k_set_target<T>, dim3(1), dim3(64), 0, s, (Ctrl<T>*)ctrl, t_end);
+  return hipGetLastError();
+}
+
+template <class Pol>
+static Launchers make_launchers(int mode) {
+  Launchers L;
+  L.mode = mode;
+  L.ctrl_bytes = sizeof(Ctrl<typename Pol::T>);
+  L.time_is_float = sizeof(typename Pol::T) == 4 ? 1 : 0;
+  L.state_elem = sizeof(typename Pol::S);
+  L.state_planes = Pol::kPlanes;
+  L.speed = &launch_speed<Pol>;
+  L.commit = &launch_commit<typename Pol::T>;
+  L.set_target = &launch_target<typename Pol::T>;
+  L.step = &launch_step<Pol>;
+  return L;
+}
+
+}  // namespace MHD_NS
+}  // namespace mhd

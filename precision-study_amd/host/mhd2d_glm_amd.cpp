@@ -85,7 +85,7 @@ void print_failure(const char* label, const Field<R>& f, int bi, int bj, R dx, R
 }
 
 template <class R>
-int run(const Args& a, int precision, double tol) {
+int run(const Args& a, int precision, const char* prec_name, double tol) {
   const R t_end = (R)a.t_end_raw, cfl = (R)a.cfl_raw, alpha = (R)a.alpha_raw;
   if (a.nx <= 0 || a.ny <= 0 || t_end <= (R)0.0) {
     std::cerr << "Error: invalid grid or final time." << std::endl;
@@ -125,8 +125,11 @@ int run(const Args& a, int precision, double tol) {
   std::cout << "Solver: " << (use_hlld ? "HLLD" : "HLL") << std::endl;
   std::cout << "Reconstruction/time: "
             << (use_muscl ? "MUSCL + SSPRK2" : "first-order + SSPRK2") << std::endl;
-  std::cout << "Precision: " << (int)(sizeof(R) * 8) << " bit"
-            << (sizeof(R) == 8 ? " (fp64)" : " (fp32)") << std::endl;
+  if (precision == MHD_PREC_FP64 || precision == MHD_PREC_FP32)
+    std::cout << "Precision: " << (int)(sizeof(R) * 8) << " bit"
+              << (sizeof(R) == 8 ? " (fp64)" : " (fp32)") << std::endl;
+  else
+    std::cout << "Precision: " << prec_name << std::endl;
   std::cout << "gamma = " << gamma << ", alpha = " << alpha << std::endl;
   std::cout << "Nx x Ny = " << nx << " x " << ny << ", CFL = " << cfl
             << ", t_end = " << t_end << std::endl;
@@ -143,6 +146,8 @@ int run(const Args& a, int precision, double tol) {
   cfg.hlld_tol = tol;
   cfg.tile = -1;
   cfg.batch_steps = 0;
+  cfg.lowbeta_ratio = 0.0;
+  if (const char* s = std::getenv("MHD_LOWBETA_RATIO")) cfg.lowbeta_ratio = std::atof(s);
   if (const char* s = std::getenv("MHD_TILE")) cfg.tile = std::atoi(s);
   mhd_ctx* ctx = nullptr;
   int rc = mhd_create(&cfg, &ctx);
@@ -235,7 +240,7 @@ int run(const Args& a, int precision, double tol) {
   if (std::system(("mkdir -p " + out_dir).c_str()) != 0) { /* ignore */ }
   std::ostringstream fname;
   fname << out_dir << "/" << base << "_glm_" << a.order << "_" << a.solver << "_"
-        << (sizeof(R) == 8 ? "fp64" : "fp32") << "_N" << nx << "x" << ny << ".dat";
+        << prec_name << "_N" << nx << "x" << ny << ".dat";
   std::ofstream out(fname.str());
   if (!out) {
     std::cerr << "Error: failed to open output file: " << fname.str() << std::endl;
@@ -287,19 +292,25 @@ int main(int argc, char* argv[]) {
   if (argc > 6) a.solver = argv[6];
   if (argc > 7) a.alpha_raw = std::atof(argv[7]);
   if (argc > 8) a.problem = argv[8];
-  int precision = MHD_PREC_FP64;
+  static const struct { const char* name; int prec; } kForms[] = {
+      {"fp64", MHD_PREC_FP64}, {"fp32", MHD_PREC_FP32}, {"mixed", MHD_PREC_MIXED},
+      {"mixed_kahan", MHD_PREC_MIXED_KAHAN}, {"fp64_fma", MHD_PREC_FP64_FMA}};
+  int form = 0;
   const std::string prog = argv[0];
-  if (ends_with(prog, "fp32")) precision = MHD_PREC_FP32;
+  for (int q = 0; q < 5; q++)
+    if (ends_with(prog, std::string("_") + kForms[q].name)) form = q;
   if (const char* s = std::getenv("MHD_PRECISION")) {
-    if (!std::strcmp(s, "fp32")) precision = MHD_PREC_FP32;
-    else if (!std::strcmp(s, "fp64")) precision = MHD_PREC_FP64;
-    else {
-      std::cerr << "Error: MHD_PRECISION must be fp64 or fp32." << std::endl;
+    form = -1;
+    for (int q = 0; q < 5; q++)
+      if (!std::strcmp(s, kForms[q].name)) form = q;
+    if (form < 0) {
+      std::cerr << "Error: MHD_PRECISION must be fp64, fp32, mixed, mixed_kahan or fp64_fma."
+                << std::endl;
       return 1;
     }
   }
   double tol = 1.0e-4;
   if (const char* s = std::getenv("MHD_HLLD_TOLERANCE")) tol = std::atof(s);
-  if (precision == MHD_PREC_FP32) return run<float>(a, precision, tol);
-  return run<double>(a, precision, tol);
+  if (kForms[form].prec == MHD_PREC_FP32) return run<float>(a, kForms[form].prec, "fp32", tol);
+  return run<double>(a, kForms[form].prec, kForms[form].name, tol);
 }

@@ -27,14 +27,21 @@ CLI_PATH = os.path.join(PKG_ROOT, "bin", "mhd2d_glm_amd")
 
 PREC_FP64 = 0
 PREC_FP32 = 1
-DTYPES = {PREC_FP64: np.float64, PREC_FP32: np.float32}
+PREC_MIXED = 2
+PREC_MIXED_KAHAN = 3
+PREC_FP64_FMA = 4
+PRECISIONS = {"fp64": PREC_FP64, "fp32": PREC_FP32, "mixed": PREC_MIXED,
+              "mixed_kahan": PREC_MIXED_KAHAN, "fp64_fma": PREC_FP64_FMA}
+# Element type of host state buffers.
+DTYPES = {PREC_FP64: np.float64, PREC_FP32: np.float32, PREC_MIXED: np.float64,
+          PREC_MIXED_KAHAN: np.float64, PREC_FP64_FMA: np.float64}
 
 # Public symbols of include/mhd_glm.h.
 EXPORTED = (
     "mhd_create", "mhd_destroy", "mhd_last_error", "mhd_problem_state",
     "mhd_set_state", "mhd_init_problem", "mhd_advance", "mhd_get_status",
     "mhd_get_state", "mhd_device_state", "mhd_stream", "mhd_enqueue_steps",
-    "mhd_time_kernels", "mhd_glm_flux_batch", "mhd_cons_to_prim_batch",
+    "mhd_time_kernels", "mhd_checkpoint_save", "mhd_checkpoint_load", "mhd_glm_flux_batch", "mhd_cons_to_prim_batch",
     "mhd_fast_speed_batch",
 )
 
@@ -54,6 +61,7 @@ class Cfg(ctypes.Structure):
         ("muscl", ctypes.c_int), ("hlld", ctypes.c_int),
         ("precision", ctypes.c_int), ("hlld_tol", ctypes.c_double),
         ("tile", ctypes.c_int), ("batch_steps", ctypes.c_int),
+        ("lowbeta_ratio", ctypes.c_double),
     ]
 
 
@@ -108,6 +116,10 @@ def load(path: str = LIB_PATH) -> ctypes.CDLL:
     lib.mhd_stream.restype = P
     lib.mhd_enqueue_steps.argtypes = [P, D, I]
     lib.mhd_enqueue_steps.restype = I
+    lib.mhd_checkpoint_save.argtypes = [P, ctypes.c_char_p]
+    lib.mhd_checkpoint_save.restype = I
+    lib.mhd_checkpoint_load.argtypes = [P, ctypes.c_char_p]
+    lib.mhd_checkpoint_load.restype = I
     lib.mhd_time_kernels.argtypes = [P, D, I, ctypes.POINTER(ctypes.c_double)]
     lib.mhd_time_kernels.restype = I
     lib.mhd_glm_flux_batch.argtypes = [I, I, I, P, P, D, I, D, P, P]
@@ -146,6 +158,7 @@ class SolverConfig:
     hlld_tol: float = 1.0e-4
     tile: int = -1
     batch_steps: int = 0
+    lowbeta_ratio: float = 0.0
 
 
 class Solver:
@@ -155,7 +168,7 @@ class Solver:
         self.lib = load()
         self.cfg = cfg
         c = Cfg(cfg.nx, cfg.ny, cfg.cfl, cfg.alpha, int(cfg.muscl), int(cfg.hlld),
-                cfg.precision, cfg.hlld_tol, cfg.tile, cfg.batch_steps)
+                cfg.precision, cfg.hlld_tol, cfg.tile, cfg.batch_steps, cfg.lowbeta_ratio)
         h = ctypes.c_void_p()
         rc = self.lib.mhd_create(ctypes.byref(c), ctypes.byref(h))
         if rc:
@@ -202,6 +215,14 @@ class Solver:
 
     def enqueue_steps(self, t_end: float, nsteps: int):
         self._check(self.lib.mhd_enqueue_steps(self.h, t_end, nsteps))
+
+    def save(self, path: str):
+        """Binary checkpoint (control block + raw state planes)."""
+        self._check(self.lib.mhd_checkpoint_save(self.h, os.fsencode(path)))
+
+    def load(self, path: str):
+        """Restores a checkpoint written by a context of the same shape/form."""
+        self._check(self.lib.mhd_checkpoint_load(self.h, os.fsencode(path)))
 
     def time_kernels(self, t_end: float, nsteps: int):
         """Average ms of (stage-1, stage-2, finish) kernels over nsteps steps."""

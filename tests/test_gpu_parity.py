@@ -124,7 +124,7 @@ def test_steps_ot_all_schemes(mhd, oracle, prec, muscl, hlld):
     check_steps(mhd, oracle, u, 25, prec, muscl=muscl, hlld=hlld)
 
 
-@pytest.mark.parametrize("tile", [0, 1, 2])
+@pytest.mark.parametrize("tile", [0, 1, 2, 3, 4, 5])
 @pytest.mark.parametrize("nx,ny", [(64, 64), (50, 37), (33, 70), (2, 5)])
 def test_steps_tiles_and_odd_sizes(mhd, oracle, tile, nx, ny):
     say(f"steps OT {nx}x{ny} tile={tile}")
