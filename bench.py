@@ -110,7 +110,11 @@ def accuracy_check(mhdglm, n, forms):
         t0 = time.perf_counter()
         s = mhdglm.Solver(mhdglm.SolverConfig(nx=n, ny=n, precision=mhdglm.PRECISIONS[form]))
         s.init_problem("ot")
-        st = s.advance(0.5)
+        while True:  # chunks, so that long runs report progress
+            st = s.advance(0.5, 2000)
+            if st.code != 0 or st.done:
+                break
+            log(f"bench: accuracy run {form} {n}^2: step {st.steps}, t={st.t:.4f}")
         u = s.get_state(0).astype("float64")
         s.close()
         log(f"bench: accuracy run {form} {n}^2: {st.steps} steps, {time.perf_counter() - t0:.1f} s")

@@ -56,7 +56,9 @@ typedef struct mhd_cfg {
   int hlld;          /* 1: HLLD with HLL fallback ("hlld"), 0: HLL ("hll") */
   int precision;     /* enum mhd_precision */
   double hlld_tol;   /* HLLD degeneracy tolerance (protocol value 1e-4) */
-  int tile;          /* stage-kernel tile shape id; -1 = default */
+  int tile;          /* stage-kernel tile shape id (0 16x16, 1 32x8, 2 16x8,
+                        3 16x15, 4 32x7, 5 16x7); -1 = the autotuned choice
+                        for this form and size (autotune.txt), else 16x15 */
   int batch_steps;   /* steps launched per host check; <= 0 = default */
   double lowbeta_ratio; /* MIXED*: FP64 recovery where E > ratio * e_int;
                            <= 0 = default (16) */
@@ -91,6 +93,12 @@ const char* mhd_last_error(const mhd_ctx* ctx);
  * mhd2d_glm_gpu.cu:415-457) into a caller SoA buffer (no device needed). */
 int mhd_problem_state(const char* problem, int nx, int ny, int precision,
                       void* soa9, size_t ld);
+
+/* Same for problems with parameters: "otv" takes 5 (B amplitude, and the
+ * phases added to the sine arguments of vx, vy, Bx, By); the others take 0.
+ * No reference counterpart (the ensemble workload's initial conditions). */
+int mhd_problem_state_ex(const char* problem, const double* params, int nparams,
+                         int nx, int ny, int precision, void* soa9, size_t ld);
 
 /* Uploads a state and restarts time at t=0 with cleared counters; the
  * first step's c_h and dt are computed on the device. */
@@ -128,6 +136,10 @@ int mhd_enqueue_steps(mhd_ctx* ctx, double t_end, int nsteps);
 int mhd_checkpoint_save(mhd_ctx* ctx, const char* path);
 int mhd_checkpoint_load(mhd_ctx* ctx, const char* path);
 
+/* Tile shape a context of this form and size gets when cfg.tile is -1
+ * (the autotuner's table, else the default).  No device needed. */
+int mhd_autotuned_tile(int precision, int nx, int ny);
+
 /* Measurement (no reference counterpart; the reference times its loop with
  * a host wall clock, mhd2d_glm_cpu.cpp:512,571-572): runs nsteps steps with
  * HIP events recorded on the context stream around each kernel and writes
@@ -148,6 +160,32 @@ int mhd_glm_flux_batch(int precision, int dir, int n, const void* UL,
 int mhd_cons_to_prim_batch(int precision, int n, const void* U, void* W);
 int mhd_fast_speed_batch(int precision, int dir, int n, const void* W,
                          void* out);
+
+/* Ensembles (no reference counterpart: the reference runs one grid per
+ * process).  n instances of sizes nx[i] x ny[i] share the configuration in
+ * cfg (scheme, precision, tile) and advance together: each step is one
+ * persistent kernel per SSPRK2 stage pulling (instance, tile) items from a
+ * device work queue, then one control kernel; every instance keeps its own
+ * dt, t, c_h, counters and halt state, exactly as a single context would
+ * (an instance's result is bitwise that of mhd_advance on its own).  Calls
+ * mirror the single-context ones with an instance index. */
+typedef struct mhd_ensemble mhd_ensemble;
+int mhd_ensemble_create(const mhd_cfg* cfg, int n, const int* nx, const int* ny,
+                        mhd_ensemble** out);
+void mhd_ensemble_destroy(mhd_ensemble* e);
+const char* mhd_ensemble_last_error(const mhd_ensemble* e);
+int mhd_ensemble_set_state(mhd_ensemble* e, int i, const void* soa9, size_t ld);
+int mhd_ensemble_init_problem(mhd_ensemble* e, int i, const char* problem,
+                              const double* params, int nparams);
+/* Advances every instance until each reaches t_end or halts, or max_steps
+ * steps (< 0: no limit); st (may be NULL) receives n statuses.  Returns the
+ * largest instance status code. */
+int mhd_ensemble_advance(mhd_ensemble* e, double t_end, long long max_steps,
+                         mhd_status* st);
+int mhd_ensemble_get_status(mhd_ensemble* e, int i, mhd_status* st);
+int mhd_ensemble_get_state(mhd_ensemble* e, int i, int which, void* soa9, size_t ld);
+int mhd_ensemble_enqueue_steps(mhd_ensemble* e, double t_end, int nsteps);
+int mhd_ensemble_time_kernels(mhd_ensemble* e, double t_end, int nsteps, double* avg_ms);
 
 #ifdef __cplusplus
 }

@@ -1,0 +1,498 @@
+// k_select.hip -- kernels of the learned precision selector.
+//
+// The selector decides, per stage-kernel tile and step, whether the FP32
+// flux stage (mixed form) stays inside the error budget or the tile must be
+// evaluated by the FP64 kernel.  It is a small MLP 12 -> 64 -> 64 -> 1
+// (ReLU, sigmoid) with BF16 operands and FP32 accumulation on MFMA
+// (v_mfma_f32_16x16x16_bf16), trained on the GPU by the kernels below on
+// labels the solver itself produces: the per-tile difference between an
+// FP32-flux and an FP64-flux stage update of the same state.
+//
+//   k_tile_features  12 features per tile from the FP64 state
+//   k_sumsq / k_tile_errors   label source (scaled max difference per tile)
+//   k_mlp_infer      probabilities + FP64 flags (with sticky audit flags)
+//   k_mlp_grad       forward (MFMA) + backward (FP32) of a minibatch,
+//                    gradients accumulated into a global buffer
+//   k_adam           Adam update of the FP32 master weights
+//   k_audit_compare  audit of FP32 tiles against an FP64 recomputation
+#include <hip/hip_runtime.h>
+
+#include "mhd_select.h"
+
+namespace mhd {
+namespace sel {
+
+typedef short bf16x4 __attribute__((ext_vector_type(4)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+// Parameter layout (FP32 masters): W1[16][64], b1[64], W2[64][64], b2[64],
+// w3[64], b3.
+constexpr int kW1 = 0, kB1 = kW1 + NIN * H, kW2 = kB1 + H, kB2 = kW2 + H * H, kW3 = kB2 + H,
+              kB3 = kW3 + H;
+static_assert(kB3 + 1 == kParams, "parameter layout");
+
+__device__ __forceinline__ short to_bf16(float f) {
+  unsigned u = __float_as_uint(f);
+  u += 0x7FFFu + ((u >> 16) & 1u);
+  return (short)(u >> 16);
+}
+__device__ __forceinline__ float bf16_val(float f) {
+  return __uint_as_float(((unsigned)(unsigned short)to_bf16(f)) << 16);
+}
+
+// ---------------------------------------------------------------------------
+// Block reductions (256 threads = 4 waves).
+// ---------------------------------------------------------------------------
+template <class Op>
+__device__ __forceinline__ double block_reduce(double v, double* red, Op op) {
+  for (int off = 32; off > 0; off >>= 1) v = op(v, __shfl_xor(v, off, 64));
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  __syncthreads();
+  if (lane == 0) red[w] = v;
+  __syncthreads();
+  return op(op(red[0], red[1]), op(red[2], red[3]));
+}
+struct Max { __device__ double operator()(double a, double b) const { return a < b ? b : a; } };
+struct Min { __device__ double operator()(double a, double b) const { return b < a ? b : a; } };
+struct Sum { __device__ double operator()(double a, double b) const { return a + b; } };
+
+__device__ __forceinline__ int wrapi(int i, int n) { return i < 0 ? i + n : (i >= n ? i - n : i); }
+
+struct CellPrim {
+  double rho, vx, vy, vz, p, bx, by, bz, b2;
+};
+__device__ __forceinline__ CellPrim prim_at(const double* u, size_t ps, size_t g) {
+  const double gm1 = 5.0 / 3.0 - 1.0;
+  CellPrim c;
+  c.rho = fmax(u[g], 1e-12);
+  c.vx = u[ps + g] / c.rho;
+  c.vy = u[2 * ps + g] / c.rho;
+  c.vz = u[3 * ps + g] / c.rho;
+  c.bx = u[5 * ps + g];
+  c.by = u[6 * ps + g];
+  c.bz = u[7 * ps + g];
+  c.b2 = c.bx * c.bx + c.by * c.by + c.bz * c.bz;
+  c.p = gm1 * (u[4 * ps + g] - 0.5 * c.rho * (c.vx * c.vx + c.vy * c.vy + c.vz * c.vz) - 0.5 * c.b2);
+  return c;
+}
+
+// One block (256 threads) per tile; thread t handles cell t of the tile.
+__global__ __launch_bounds__(256) void k_tile_features(const double* u, size_t ps, int nx, int ny,
+                                                       int tx, int ty, int tiles_x, int ntiles,
+                                                       const void* ctrl_ch, float* feats) {
+  __shared__ double red[4];
+  const int tile = blockIdx.x;
+  if (tile >= ntiles) return;
+  const double ch = *(const double*)ctrl_ch;
+  const int j0 = (tile / tiles_x) * ty, i0 = (tile % tiles_x) * tx;
+  const int t = threadIdx.x, li = t % tx, lj = t / tx;
+  const int i = i0 + li, j = j0 + lj;
+  const bool live = t < tx * ty && i < nx && j < ny;
+  const double gamma = 5.0 / 3.0;
+  double beta_mn = 1e300, beta_mx = -1e300, drho = 0, dp = 0, db = 0, mach = 0, psi = 0, bmax = 0;
+  double divn = 0, bnmin = 1e300, vch = 0, cf = 0, ke = 0, me = 0, n = 0;
+  if (live) {
+    const size_t g = (size_t)j * nx + i;
+    const CellPrim c = prim_at(u, ps, g);
+    const CellPrim r = prim_at(u, ps, (size_t)j * nx + wrapi(i + 1, nx));
+    const CellPrim tp = prim_at(u, ps, (size_t)wrapi(j + 1, ny) * nx + i);
+    const double bxl = u[5 * ps + (size_t)j * nx + wrapi(i - 1, nx)];
+    const double byd = u[6 * ps + (size_t)wrapi(j - 1, ny) * nx + i];
+    const double pp = fmax(c.p, 1e-12), bmag = sqrt(c.b2);
+    const double beta = fmin(fmax(2.0 * pp / fmax(c.b2, 1e-30), 1e-6), 1e6);
+    beta_mn = beta_mx = beta;
+    drho = fmax(fabs(r.rho - c.rho), fabs(tp.rho - c.rho)) / c.rho;
+    dp = fmax(fabs(fmax(r.p, 1e-12) - pp), fabs(fmax(tp.p, 1e-12) - pp)) / pp;
+    const double dbr = sqrt((r.bx - c.bx) * (r.bx - c.bx) + (r.by - c.by) * (r.by - c.by) +
+                            (r.bz - c.bz) * (r.bz - c.bz));
+    const double dbt = sqrt((tp.bx - c.bx) * (tp.bx - c.bx) + (tp.by - c.by) * (tp.by - c.by) +
+                            (tp.bz - c.bz) * (tp.bz - c.bz));
+    db = fmax(dbr, dbt) / fmax(bmag, 1e-12);
+    const double v2 = c.vx * c.vx + c.vy * c.vy + c.vz * c.vz, cs2 = gamma * pp / c.rho;
+    mach = fmin(sqrt(v2 / cs2), 100.0);
+    psi = fabs(u[8 * ps + g]);
+    bmax = bmag;
+    const double h = 1.0 / (double)(nx > ny ? nx : ny);
+    const double divb = 0.5 * (r.bx - bxl) * nx + 0.5 * (tp.by - byd) * ny;
+    divn = fabs(divb) * h / (bmag + 1e-12);
+    bnmin = fmin(fabs(c.bx), fabs(c.by)) / (bmag + 1e-12);
+    vch = sqrt(v2) / ch;
+    cf = sqrt(cs2 + c.b2 / c.rho) / ch;
+    ke = 0.5 * c.rho * v2;
+    me = 0.5 * c.b2;
+    n = 1.0;
+  }
+  const Max mx;
+  const Min mn;
+  const Sum sm;
+  const double f0 = block_reduce(beta_mn, red, mn), f1 = block_reduce(beta_mx, red, mx);
+  const double f2 = block_reduce(drho, red, mx), f3 = block_reduce(dp, red, mx);
+  const double f4 = block_reduce(db, red, mx), f5 = block_reduce(mach, red, mx);
+  const double psimax = block_reduce(psi, red, mx), bm = block_reduce(bmax, red, mx);
+  const double f7 = block_reduce(divn, red, mx), f8 = block_reduce(bnmin, red, mn);
+  const double f9 = block_reduce(vch, red, mx), f10 = block_reduce(cf, red, mx);
+  const double kes = block_reduce(ke, red, sm), mes = block_reduce(me, red, sm);
+  const double cnt = block_reduce(n, red, sm);
+  if (threadIdx.x == 0) {
+    float* o = feats + (size_t)tile * NIN;
+    o[0] = (float)log10(f0);
+    o[1] = (float)log10(f1);
+    o[2] = (float)log10(f2 + 1e-12);
+    o[3] = (float)log10(f3 + 1e-12);
+    o[4] = (float)log10(f4 + 1e-12);
+    o[5] = (float)f5;
+    o[6] = (float)log10(psimax / (bm + 1e-12) + 1e-12);
+    o[7] = (float)log10(f7 + 1e-12);
+    o[8] = (float)log10(f8 + 1e-12);
+    o[9] = (float)f9;
+    o[10] = (float)f10;
+    o[11] = (float)log10(kes / cnt / (mes / cnt + 1e-12) + 1e-12);
+    for (int k = NFEAT; k < NIN; k++) o[k] = 0.f;
+  }
+}
+
+// Sum of squares of each of the 9 planes (atomic double adds; out zeroed by
+// the caller).
+__global__ __launch_bounds__(256) void k_sumsq(const double* u, size_t ps, long long cells,
+                                               double* out9) {
+  __shared__ double red[4];
+  for (int k = 0; k < 9; k++) {
+    double s = 0;
+    for (long long q = (long long)blockIdx.x * 256 + threadIdx.x; q < cells;
+         q += (long long)gridDim.x * 256) {
+      const double v = u[k * ps + q];
+      s += v * v;
+    }
+    const double b = block_reduce(s, red, Sum());
+    if (threadIdx.x == 0) atomicAdd(&out9[k], b);
+    __syncthreads();
+  }
+}
+
+// Per-tile max over cells and variables of |a-b| / (|b| + rms_k(b)).
+__global__ __launch_bounds__(256) void k_tile_errors(const double* a, const double* b, size_t ps,
+                                                     int nx, int ny, int tx, int ty, int tiles_x,
+                                                     int ntiles, const double* sumsq9,
+                                                     float* err) {
+  __shared__ double red[4];
+  const int tile = blockIdx.x;
+  if (tile >= ntiles) return;
+  const int j0 = (tile / tiles_x) * ty, i0 = (tile % tiles_x) * tx;
+  const int t = threadIdx.x, i = i0 + t % tx, j = j0 + t / tx;
+  double e = 0.0;
+  if (t < tx * ty && i < nx && j < ny) {
+    const size_t g = (size_t)j * nx + i;
+    const double cells = (double)nx * (double)ny;
+    for (int k = 0; k < 9; k++) {
+      const double rms = sqrt(sumsq9[k] / cells) + 1e-30;
+      const double bv = b[k * ps + g];
+      e = fmax(e, fabs(a[k * ps + g] - bv) / (fabs(bv) + rms));
+    }
+  }
+  const double m = block_reduce(e, red, Max());
+  if (threadIdx.x == 0) err[tile] = (float)m;
+}
+
+// ---------------------------------------------------------------------------
+// MLP forward on MFMA.  A wave evaluates 16 samples: layer 1 is one K=16
+// step over 4 N-blocks, layer 2 four K=16 steps over 4 N-blocks, layer 3 an
+// FP32 dot product.  Operand layouts of v_mfma_f32_16x16x16_bf16: lane l
+// holds A[l%16][4*(l/16)+0..3], B[4*(l/16)+0..3][l%16], and receives
+// D[4*(l/16)+i][l%16], i = 0..3.
+// ---------------------------------------------------------------------------
+struct WaveScratch {
+  float z1[16][H];
+  float z2[16][H];
+  short a1[16][H];  // bf16 of relu(z1)
+};
+
+__device__ __forceinline__ void mlp_forward_wave(const float* x16, int nvalid, const float* P,
+                                                 WaveScratch& ws, float* z3_out) {
+  const int l = threadIdx.x & 63, row = l & 15, kq = l >> 4;
+  // Layer 1: A = x (16 x 16), B = W1 (16 x 64).
+  bf16x4 a;
+  for (int q = 0; q < 4; q++) {
+    const int k = 4 * kq + q;
+    a[q] = to_bf16(row < nvalid ? x16[row * NIN + k] : 0.f);
+  }
+  for (int nb = 0; nb < 4; nb++) {
+    bf16x4 b;
+    for (int q = 0; q < 4; q++) b[q] = to_bf16(P[kW1 + (4 * kq + q) * H + nb * 16 + row]);
+    f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+    acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc, 0, 0, 0);
+    for (int q = 0; q < 4; q++) {
+      const int m = 4 * kq + q, n = nb * 16 + row;
+      const float z = acc[q] + P[kB1 + n];
+      ws.z1[m][n] = z;
+      ws.a1[m][n] = to_bf16(z > 0.f ? z : 0.f);
+    }
+  }
+  __syncthreads();
+  // Layer 2: A = a1 (16 x 64) from LDS, B = W2 (64 x 64).
+  for (int nb = 0; nb < 4; nb++) {
+    f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+    for (int ks = 0; ks < 4; ks++) {
+      bf16x4 aa, b;
+      for (int q = 0; q < 4; q++) {
+        const int k = ks * 16 + 4 * kq + q;
+        aa[q] = ws.a1[row][k];
+        b[q] = to_bf16(P[kW2 + k * H + nb * 16 + row]);
+      }
+      acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(aa, b, acc, 0, 0, 0);
+    }
+    for (int q = 0; q < 4; q++) {
+      const int m = 4 * kq + q, n = nb * 16 + row;
+      ws.z2[m][n] = acc[q] + P[kB2 + n];
+    }
+  }
+  __syncthreads();
+  // Layer 3 (FP32): lanes 0..15 each finish one sample.
+  if (l < 16) {
+    float z = P[kB3];
+    for (int n = 0; n < H; n++) {
+      const float a2 = ws.z2[l][n] > 0.f ? ws.z2[l][n] : 0.f;
+      z += a2 * P[kW3 + n];
+    }
+    z3_out[l] = z;
+  }
+  __syncthreads();
+}
+
+// Standardised features of sample s into x16 (16 floats).
+__device__ __forceinline__ void standardise(const float* raw, const float* mean, const float* istd,
+                                            float* x16) {
+  for (int k = 0; k < NIN; k++) x16[k] = k < NFEAT ? (raw[k] - mean[k]) * istd[k] : 0.f;
+}
+
+// Block of 256 threads = 4 waves x 16 samples.  raw: n x NIN features
+// (standardised when mean is null); prob[s]; flags[s] = prob > threshold or
+// sticky[s].
+__global__ __launch_bounds__(256) void k_mlp_infer(const float* raw, int n, const float* P,
+                                                   const float* mean, const float* istd,
+                                                   float threshold, const unsigned char* sticky,
+                                                   float* prob, unsigned char* flags) {
+  __shared__ WaveScratch ws[4];
+  __shared__ float xs[4][16 * NIN];
+  __shared__ float z3[4][16];
+  const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+  const int base = blockIdx.x * 64 + w * 16;
+  const int nvalid = n - base < 0 ? 0 : (n - base > 16 ? 16 : n - base);
+  if (l < 16 && l < nvalid) {
+    if (mean)
+      standardise(raw + (size_t)(base + l) * NIN, mean, istd, &xs[w][l * NIN]);
+    else
+      for (int k = 0; k < NIN; k++) xs[w][l * NIN + k] = raw[(size_t)(base + l) * NIN + k];
+  }
+  __syncthreads();
+  mlp_forward_wave(xs[w], nvalid, P, ws[w], z3[w]);
+  if (l < nvalid) {
+    const float p = 1.f / (1.f + __expf(-z3[w][l]));
+    if (prob) prob[base + l] = p;
+    if (flags) flags[base + l] = (p > threshold || (sticky && sticky[base + l])) ? 1 : 0;
+  }
+}
+
+// Forward + backward of up to 64 samples per block; gradients of
+// sum_s w_s BCE(p_s, y_s) / wsum added into grad (FP32 atomics).
+// Samples are idx[offset + s] of the training arrays.
+__global__ __launch_bounds__(256) void k_mlp_grad(const float* X, const float* Y, const float* Wt,
+                                                  const int* idx, int offset, int count,
+                                                  float wsum, const float* P, float* grad,
+                                                  float* loss) {
+  __shared__ WaveScratch ws[4];
+  __shared__ float xs[4][16 * NIN];
+  __shared__ float z3[4][16];
+  __shared__ float dz3[64];
+  __shared__ float dz2[64][H];
+  __shared__ float dz1[64][H];
+  const int w = threadIdx.x >> 6, l = threadIdx.x & 63, tid = threadIdx.x;
+  const int b0 = blockIdx.x * 64;
+  const int nblk = count - b0 < 0 ? 0 : (count - b0 > 64 ? 64 : count - b0);
+  const int nvalid = nblk - w * 16 < 0 ? 0 : (nblk - w * 16 > 16 ? 16 : nblk - w * 16);
+  if (l < nvalid) {
+    const int s = idx[offset + b0 + w * 16 + l];
+    for (int k = 0; k < NIN; k++) xs[w][l * NIN + k] = X[(size_t)s * NIN + k];
+  }
+  __syncthreads();
+  mlp_forward_wave(xs[w], nvalid, P, ws[w], z3[w]);
+  // dL/dz3 and the loss.
+  if (tid < 64) {
+    float d = 0.f;
+    if (tid < nblk) {
+      const int s = idx[offset + b0 + tid];
+      const float z = z3[tid >> 4][tid & 15];
+      const float p = 1.f / (1.f + __expf(-z));
+      const float y = Y[s], wt = Wt[s];
+      d = (p - y) * wt / wsum;
+      const float lp = -wt * (y * __logf(fmaxf(p, 1e-12f)) + (1.f - y) * __logf(fmaxf(1.f - p, 1e-12f)));
+      atomicAdd(loss, lp / wsum);
+    }
+    dz3[tid] = d;
+  }
+  __syncthreads();
+  // dz2 = dz3 w3 (z2 > 0); gw3 and gb3.
+  for (int e = tid; e < 64 * H; e += 256) {
+    const int s = e / H, nn = e % H;
+    const float z2 = ws[s >> 4].z2[s & 15][nn];
+    dz2[s][nn] = (s < nblk && z2 > 0.f) ? dz3[s] * P[kW3 + nn] : 0.f;
+  }
+  if (tid < H) {
+    float g = 0.f;
+    for (int s = 0; s < nblk; s++) {
+      const float z2 = ws[s >> 4].z2[s & 15][tid];
+      g += (z2 > 0.f ? z2 : 0.f) * dz3[s];
+    }
+    atomicAdd(&grad[kW3 + tid], g);
+  }
+  if (tid == 0) {
+    float g = 0.f;
+    for (int s = 0; s < nblk; s++) g += dz3[s];
+    atomicAdd(&grad[kB3], g);
+  }
+  __syncthreads();
+  // gW2[k][n] = sum_s a1_bf16[s][k] dz2[s][n]; gb2; dz1 = (dz2 W2_bf16^T) (z1 > 0).
+  for (int e = tid; e < H * H; e += 256) {
+    const int k = e / H, nn = e % H;
+    float g = 0.f;
+    for (int s = 0; s < nblk; s++) {
+      const float a1 = __uint_as_float(((unsigned)(unsigned short)ws[s >> 4].a1[s & 15][k]) << 16);
+      g += a1 * dz2[s][nn];
+    }
+    atomicAdd(&grad[kW2 + e], g);
+  }
+  if (tid < H) {
+    float g = 0.f;
+    for (int s = 0; s < nblk; s++) g += dz2[s][tid];
+    atomicAdd(&grad[kB2 + tid], g);
+  }
+  for (int e = tid; e < 64 * H; e += 256) {
+    const int s = e / H, k = e % H;
+    float d = 0.f;
+    if (s < nblk && ws[s >> 4].z1[s & 15][k] > 0.f)
+      for (int nn = 0; nn < H; nn++) d += dz2[s][nn] * bf16_val(P[kW2 + k * H + nn]);
+    dz1[s][k] = d;
+  }
+  __syncthreads();
+  // gW1[k][n] = sum_s x_bf16[s][k] dz1[s][n]; gb1.
+  for (int e = tid; e < NIN * H; e += 256) {
+    const int k = e / H, nn = e % H;
+    float g = 0.f;
+    for (int s = 0; s < nblk; s++) g += bf16_val(xs[s >> 4][(s & 15) * NIN + k]) * dz1[s][nn];
+    atomicAdd(&grad[kW1 + e], g);
+  }
+  if (tid < H) {
+    float g = 0.f;
+    for (int s = 0; s < nblk; s++) g += dz1[s][tid];
+    atomicAdd(&grad[kB1 + tid], g);
+  }
+}
+
+// Adam on the FP32 masters; clears the gradient.
+__global__ void k_adam(float* P, float* grad, float* m, float* v, int step, float lr) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= kParams) return;
+  const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
+  const float g = grad[i];
+  const float mi = b1 * m[i] + (1.f - b1) * g, vi = b2 * v[i] + (1.f - b2) * g * g;
+  m[i] = mi;
+  v[i] = vi;
+  const float mh = mi / (1.f - __powf(b1, (float)step)), vh = vi / (1.f - __powf(b2, (float)step));
+  float p = P[i] - lr * mh / (sqrtf(vh) + eps);
+  if (i >= kW1 + NFEAT * H && i < kB1) p = 0.f;  // padding rows of W1 stay zero
+  P[i] = p;
+  grad[i] = 0.f;
+}
+
+// Audit: tiles flagged for audit (audit[t] = 1) were evaluated by both the
+// FP32 and the FP64 kernels; a scaled difference above `budget` marks the
+// tile sticky (FP64 from then on) and counts a miss.
+__global__ void k_audit_compare(const float* err, const unsigned char* audit, int ntiles,
+                                float budget, unsigned char* sticky, unsigned long long* counters) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= ntiles || !audit[t]) return;
+  atomicAdd(&counters[0], 1ull);
+  if (err[t] > budget) {
+    sticky[t] = 1;
+    atomicAdd(&counters[1], 1ull);
+  }
+}
+
+// Audit sample of a step: unflagged tiles whose hash falls under `frac`.
+__global__ void k_audit_pick(const unsigned char* flags, int ntiles, unsigned step, float frac,
+                             unsigned char* audit) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= ntiles) return;
+  unsigned h = (unsigned)t * 2654435761u ^ (step * 2246822519u + 0x9E3779B9u);
+  h ^= h >> 15;
+  h *= 2246822519u;
+  h ^= h >> 13;
+  audit[t] = (!flags[t] && (float)(h & 0xFFFFFF) < frac * 16777216.f) ? 1 : 0;
+}
+
+}  // namespace sel
+
+// ---------------------------------------------------------------------------
+// Launchers.
+// ---------------------------------------------------------------------------
+hipError_t sel_tile_features(const double* u, size_t plane, int nx, int ny, int tx, int ty,
+                             const void* ctrl_ch, float* feats, hipStream_t s) {
+  const int tiles_x = (nx + tx - 1) / tx, ntiles = tiles_x * ((ny + ty - 1) / ty);
+  hipLaunchKernelGGL(sel::k_tile_features, dim3(ntiles), dim3(256), 0, s, u, plane, nx, ny, tx, ty,
+                     tiles_x, ntiles, ctrl_ch, feats);
+  return hipGetLastError();
+}
+
+hipError_t sel_tile_errors(const double* a, const double* b, size_t plane, int nx, int ny, int tx,
+                           int ty, double* sumsq9, float* err, hipStream_t s) {
+  const int tiles_x = (nx + tx - 1) / tx, ntiles = tiles_x * ((ny + ty - 1) / ty);
+  const long long cells = (long long)nx * ny;
+  hipError_t e = hipMemsetAsync(sumsq9, 0, 9 * sizeof(double), s);
+  if (e != hipSuccess) return e;
+  long long blocks = (cells + 255) / 256;
+  if (blocks > 4096) blocks = 4096;
+  hipLaunchKernelGGL(sel::k_sumsq, dim3((unsigned)blocks), dim3(256), 0, s, b, plane, cells, sumsq9);
+  hipLaunchKernelGGL(sel::k_tile_errors, dim3(ntiles), dim3(256), 0, s, a, b, plane, nx, ny, tx, ty,
+                     tiles_x, ntiles, sumsq9, err);
+  return hipGetLastError();
+}
+
+hipError_t sel_mlp_infer(const float* raw, int n, const float* params, const float* mean,
+                         const float* istd, float threshold, const unsigned char* sticky,
+                         float* prob, unsigned char* flags, hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(sel::k_mlp_infer, dim3((n + 63) / 64), dim3(256), 0, s, raw, n, params, mean,
+                     istd, threshold, sticky, prob, flags);
+  return hipGetLastError();
+}
+
+hipError_t sel_mlp_grad(const float* X, const float* Y, const float* W, const int* idx, int offset,
+                        int count, float wsum, const float* params, float* grad, float* loss,
+                        hipStream_t s) {
+  if (count <= 0) return hipSuccess;
+  hipLaunchKernelGGL(sel::k_mlp_grad, dim3((count + 63) / 64), dim3(256), 0, s, X, Y, W, idx,
+                     offset, count, wsum, params, grad, loss);
+  return hipGetLastError();
+}
+
+hipError_t sel_adam(float* params, float* grad, float* m, float* v, int step, float lr,
+                    hipStream_t s) {
+  hipLaunchKernelGGL(sel::k_adam, dim3((sel::kParams + 255) / 256), dim3(256), 0, s, params, grad, m, v,
+                     step, lr);
+  return hipGetLastError();
+}
+
+hipError_t sel_audit_pick(const unsigned char* flags, int ntiles, unsigned step, float frac,
+                          unsigned char* audit, hipStream_t s) {
+  hipLaunchKernelGGL(sel::k_audit_pick, dim3((ntiles + 255) / 256), dim3(256), 0, s, flags, ntiles,
+                     step, frac, audit);
+  return hipGetLastError();
+}
+
+hipError_t sel_audit_compare(const float* err, const unsigned char* audit, int ntiles, float budget,
+                             unsigned char* sticky, unsigned long long* counters, hipStream_t s) {
+  hipLaunchKernelGGL(sel::k_audit_compare, dim3((ntiles + 255) / 256), dim3(256), 0, s, err, audit,
+                     ntiles, budget, sticky, counters);
+  return hipGetLastError();
+}
+
+}  // namespace mhd

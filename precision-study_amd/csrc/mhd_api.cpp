@@ -16,10 +16,13 @@
 
 #include <hip/hip_runtime_api.h>
 
+#include <dlfcn.h>
+
 #include <climits>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <algorithm>
 #include <string>
 #include <vector>
 
@@ -92,7 +95,7 @@ void init_ctrl(Ctrl<R>* h, const mhd_cfg& cfg) {
 }
 
 template <class R>
-void fill_status(const mhd_ctx* c, const Ctrl<R>* h, mhd_status* st) {
+void fill_status_of(int hlld, int nx, int ny, const Ctrl<R>* h, mhd_status* st) {
   std::memset(st, 0, sizeof(*st));
   st->code = (h->run == mhd::RUN_HALTED) ? h->code : 0;
   st->done = (h->run == mhd::RUN_DONE) ? 1 : 0;
@@ -101,20 +104,77 @@ void fill_status(const mhd_ctx* c, const Ctrl<R>* h, mhd_status* st) {
   st->dt = (double)(h->run == mhd::RUN_HALTED ? h->fail_dt : h->dt);
   st->ch_last = (double)(h->run == mhd::RUN_HALTED ? h->fail_ch : h->ch);
   const unsigned long long steps = (unsigned long long)h->step;
-  if (c->cfg.hlld) {
+  if (hlld) {
     // The reference evaluates (nx+1)*ny x faces and nx*(ny+1) y faces per
     // stage (mhd2d_glm_cpu.cpp:253-271), two stages per step.
-    st->x_evaluations = 2ull * steps * (unsigned long long)(c->cfg.nx + 1) * c->cfg.ny;
-    st->y_evaluations = 2ull * steps * (unsigned long long)c->cfg.nx * (c->cfg.ny + 1);
+    st->x_evaluations = 2ull * steps * (unsigned long long)(nx + 1) * ny;
+    st->y_evaluations = 2ull * steps * (unsigned long long)nx * (ny + 1);
     st->degenerate = h->degenerate;
     st->nonphysical = h->nonphysical;
   }
   st->bad_i = st->bad_j = -1;
   if (h->run == mhd::RUN_HALTED && (h->code == 3 || h->code == 4) && h->fail_cell >= 0) {
-    st->bad_i = h->fail_cell % c->cfg.nx;
-    st->bad_j = h->fail_cell / c->cfg.nx;
+    st->bad_i = h->fail_cell % nx;
+    st->bad_j = h->fail_cell / nx;
   }
   st->fail_step = h->fail_step;
+}
+
+template <class R>
+void fill_status(const mhd_ctx* c, const Ctrl<R>* h, mhd_status* st) {
+  fill_status_of(c->cfg.hlld, c->cfg.nx, c->cfg.ny, h, st);
+}
+
+// Host <-> device state transfer in the caller's SoA layout (element type
+// float for FP32, double otherwise); the compensated form stores hi and lo
+// FP32 planes on the device.
+hipError_t upload_state(const mhd::Launchers* L, void* dev, size_t plane, size_t cells,
+                        const void* soa9, size_t ld, hipStream_t s) {
+  const size_t es = L->state_elem;
+  hipError_t e = hipSuccess;
+  if (L->state_planes == 18) {
+    std::vector<float> hl(18 * cells);
+    const double* src = (const double*)soa9;
+    for (int k = 0; k < 9; k++)
+      for (size_t q = 0; q < cells; q++) {
+        const double x = src[k * ld + q];
+        const float hi = (float)x;
+        hl[k * cells + q] = hi;
+        hl[(9 + k) * cells + q] = (float)(x - (double)hi);
+      }
+    for (int k = 0; k < 18 && e == hipSuccess; k++)
+      e = hipMemcpy((char*)dev + k * plane * es, hl.data() + k * cells, cells * es,
+                    hipMemcpyHostToDevice);
+    return e;
+  }
+  for (int k = 0; k < 9 && e == hipSuccess; k++)
+    e = hipMemcpyAsync((char*)dev + k * plane * es, (const char*)soa9 + k * ld * es, cells * es,
+                       hipMemcpyHostToDevice, s);
+  return e;
+}
+
+hipError_t download_state(const mhd::Launchers* L, const void* dev, size_t plane, size_t cells,
+                          void* soa9, size_t ld, hipStream_t s) {
+  const size_t es = L->state_elem;
+  hipError_t e = hipSuccess;
+  if (L->state_planes == 18) {
+    std::vector<float> hl(18 * cells);
+    for (int k = 0; k < 18 && e == hipSuccess; k++)
+      e = hipMemcpyAsync(hl.data() + k * cells, (const char*)dev + k * plane * es, cells * es,
+                         hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    if (e != hipSuccess) return e;
+    double* out = (double*)soa9;
+    for (int k = 0; k < 9; k++)
+      for (size_t q = 0; q < cells; q++)
+        out[k * ld + q] = (double)hl[k * cells + q] + (double)hl[(9 + k) * cells + q];
+    return hipSuccess;
+  }
+  for (int k = 0; k < 9 && e == hipSuccess; k++)
+    e = hipMemcpyAsync((char*)soa9 + k * ld * es, (const char*)dev + k * plane * es, cells * es,
+                       hipMemcpyDeviceToHost, s);
+  if (e == hipSuccess) e = hipStreamSynchronize(s);
+  return e;
 }
 
 int read_ctrl(mhd_ctx* c) {
@@ -166,18 +226,67 @@ int ensure_graph(mhd_ctx* c, int batch) {
   return 0;
 }
 
+// Tile shape chosen by the autotuner (mhdglm.autotune) for a form and grid
+// size.  The table is the text file `autotune.txt` next to the library's
+// lib/ directory (or $MHD_AUTOTUNE_FILE): lines "form max_cells tile", the
+// first line with the form and max_cells >= cells wins; '#' starts a
+// comment.  Without a match the built-in default tile is used.
+struct TuneRow {
+  int form;
+  long long max_cells;
+  int tile;
+};
+
+const std::vector<TuneRow>& tune_table() {
+  static const std::vector<TuneRow> rows = [] {
+    std::vector<TuneRow> out;
+    std::string path;
+    if (const char* env = std::getenv("MHD_AUTOTUNE_FILE")) {
+      path = env;
+    } else {
+      Dl_info info;
+      if (dladdr((void*)&tune_table, &info) && info.dli_fname) {
+        std::string lib = info.dli_fname;
+        const size_t slash = lib.rfind('/');
+        const std::string dir = slash == std::string::npos ? "." : lib.substr(0, slash);
+        path = dir + "/../autotune.txt";
+      }
+    }
+    FILE* f = path.empty() ? nullptr : std::fopen(path.c_str(), "r");
+    if (!f) return out;
+    char line[256];
+    while (std::fgets(line, sizeof(line), f)) {
+      if (line[0] == '#') continue;
+      TuneRow r;
+      if (std::sscanf(line, "%d %lld %d", &r.form, &r.max_cells, &r.tile) == 3 && r.tile >= 0 &&
+          r.tile < mhd::TILE_COUNT)
+        out.push_back(r);
+    }
+    std::fclose(f);
+    return out;
+  }();
+  return rows;
+}
+
+int tuned_tile(int form, long long cells) {
+  for (const TuneRow& r : tune_table())
+    if (r.form == form && cells <= r.max_cells) return r.tile;
+  return mhd::kDefaultTile;
+}
+
 bool use_graphs() {
   const char* s = std::getenv("MHD_GRAPHS");
   return !(s && s[0] == '0');
 }
 
 template <class R>
-void problem_soa(const char* problem, int nx, int ny, R* out, size_t ld) {
+void problem_soa(const char* problem, int nx, int ny, R* out, size_t ld,
+                 const double* params = nullptr) {
   const R dx = ((R)1.0 - (R)0.0) / (R)nx, dy = ((R)1.0 - (R)0.0) / (R)ny;
   for (int j = 0; j < ny; j++)
     for (int i = 0; i < nx; i++) {
       R P[9], U[9];
-      mhd::problem_primitive<R>(problem, i, j, dx, dy, P);
+      mhd::problem_primitive<R>(problem, i, j, dx, dy, P, params);
       mhd::host_prim_to_cons<R>(P, U);
       const size_t q = (size_t)j * nx + i;
       for (int k = 0; k < 9; k++) out[k * ld + q] = U[k];
@@ -187,6 +296,10 @@ void problem_soa(const char* problem, int nx, int ny, R* out, size_t ld) {
 }  // namespace
 
 extern "C" {
+
+int mhd_autotuned_tile(int precision, int nx, int ny) {
+  return tuned_tile(precision, (long long)nx * ny);
+}
 
 const char* mhd_last_error(const mhd_ctx* ctx) {
   return ctx ? ctx->err.c_str() : kNoCtx;
@@ -207,7 +320,8 @@ int mhd_create(const mhd_cfg* cfg, mhd_ctx** out) {
   if (!(c->cfg.lowbeta_ratio > 0.0)) c->cfg.lowbeta_ratio = 16.0;
   if (c->cfg.batch_steps <= 0) c->cfg.batch_steps = 32;
   if (c->cfg.hlld_tol <= 0.0) c->cfg.hlld_tol = 1.0e-4;
-  if (c->cfg.tile < 0 || c->cfg.tile >= mhd::TILE_COUNT) c->cfg.tile = mhd::kDefaultTile;
+  if (c->cfg.tile < 0 || c->cfg.tile >= mhd::TILE_COUNT)
+    c->cfg.tile = tuned_tile(cfg->precision, (long long)cfg->nx * cfg->ny);
   // Planes start on 512-byte boundaries.
   const size_t cells = (size_t)cfg->nx * cfg->ny;
   c->plane = (cells + 63) / 64 * 64;
@@ -248,17 +362,23 @@ void mhd_destroy(mhd_ctx* c) {
   delete c;
 }
 
-int mhd_problem_state(const char* problem, int nx, int ny, int precision,
-                      void* soa9, size_t ld) {
+int mhd_problem_state_ex(const char* problem, const double* params, int nparams, int nx,
+                         int ny, int precision, void* soa9, size_t ld) {
   if (!mhd::problem_known(problem) || nx < 1 || ny < 1 || !soa9) return 1;
   if (ld < (size_t)nx * ny) return 1;
+  if (nparams != mhd::problem_nparams(problem) || (nparams > 0 && !params)) return 1;
   if (precision == MHD_PREC_FP32)
-    problem_soa<float>(problem, nx, ny, (float*)soa9, ld);
+    problem_soa<float>(problem, nx, ny, (float*)soa9, ld, params);
   else if (launchers_for(precision))
-    problem_soa<double>(problem, nx, ny, (double*)soa9, ld);
+    problem_soa<double>(problem, nx, ny, (double*)soa9, ld, params);
   else
     return 1;
   return 0;
+}
+
+int mhd_problem_state(const char* problem, int nx, int ny, int precision,
+                      void* soa9, size_t ld) {
+  return mhd_problem_state_ex(problem, nullptr, 0, nx, ny, precision, soa9, ld);
 }
 
 int mhd_set_state(mhd_ctx* c, const void* soa9, size_t ld) {
@@ -586,6 +706,354 @@ int mhd_fast_speed_batch(int precision, int dir, int n, const void* W, void* out
   if (precision == MHD_PREC_FP32) return speed_batch<float>(dir, n, W, out);
   if (precision == MHD_PREC_FP64) return speed_batch<double>(dir, n, W, out);
   return 1;
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Ensembles.
+// ---------------------------------------------------------------------------
+struct mhd_ensemble {
+  mhd_cfg cfg;
+  const mhd::Launchers* L = nullptr;
+  int n = 0;
+  std::vector<int> nx, ny;
+  std::vector<size_t> plane;
+  std::vector<void*> u, u1;
+  std::vector<char> have;
+  void* ctrls = nullptr;       // device, n control blocks
+  void* ctrls_host = nullptr;  // pinned copy
+  void* desc = nullptr;        // device descriptors
+  int* prefix = nullptr;       // device prefix of tile counts
+  unsigned* queue = nullptr;   // device work-queue counters
+  mhd::EnsLaunch el;
+  hipStream_t stream = nullptr;
+  hipGraphExec_t graph = nullptr;
+  int graph_steps = 0;
+  std::string err;
+};
+
+namespace {
+
+int efail(mhd_ensemble* e, int code, const std::string& msg) {
+  if (e) e->err = msg;
+  return code;
+}
+
+#define EHIP(e, call)                                                                   \
+  do {                                                                                  \
+    hipError_t e_ = (call);                                                             \
+    if (e_ != hipSuccess)                                                               \
+      return efail(e, 9, std::string("HIP error in ") + #call + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+mhd::GridDesc inst_grid(const mhd_ensemble* e, int i) {
+  mhd::GridDesc g;
+  g.nx = e->nx[i];
+  g.ny = e->ny[i];
+  g.plane = e->plane[i];
+  g.muscl = e->cfg.muscl ? 1 : 0;
+  g.hlld = e->cfg.hlld ? 1 : 0;
+  g.alpha = e->cfg.alpha;
+  g.tol = e->cfg.hlld_tol;
+  g.lowbeta_ratio = e->cfg.lowbeta_ratio;
+  g.tile = e->cfg.tile;
+  return g;
+}
+
+void* ctrl_at(void* base, const mhd_ensemble* e, int i) {
+  return (char*)base + (size_t)i * e->L->ctrl_bytes;
+}
+
+int ens_read_ctrls(mhd_ensemble* e) {
+  EHIP(e, hipMemcpyAsync(e->ctrls_host, e->ctrls, (size_t)e->n * e->L->ctrl_bytes,
+                         hipMemcpyDeviceToHost, e->stream));
+  EHIP(e, hipStreamSynchronize(e->stream));
+  return 0;
+}
+
+void ens_status(const mhd_ensemble* e, int i, mhd_status* st) {
+  const void* h = ctrl_at(e->ctrls_host, e, i);
+  if (e->L->time_is_float)
+    fill_status_of(e->cfg.hlld, e->nx[i], e->ny[i], (const Ctrl<float>*)h, st);
+  else
+    fill_status_of(e->cfg.hlld, e->nx[i], e->ny[i], (const Ctrl<double>*)h, st);
+}
+
+hipError_t ens_enqueue_step(mhd_ensemble* e, hipEvent_t* ev = nullptr) {
+  return e->L->ens_step(e->el, e->stream, ev);
+}
+
+int ens_ensure_graph(mhd_ensemble* e, int batch) {
+  if (e->graph && e->graph_steps == batch) return 0;
+  if (e->graph) {
+    (void)hipGraphExecDestroy(e->graph);
+    e->graph = nullptr;
+  }
+  hipGraph_t g = nullptr;
+  EHIP(e, hipStreamBeginCapture(e->stream, hipStreamCaptureModeThreadLocal));
+  hipError_t le = hipSuccess;
+  for (int q = 0; q < batch && le == hipSuccess; q++) le = ens_enqueue_step(e);
+  hipError_t ee = hipStreamEndCapture(e->stream, &g);
+  if (le != hipSuccess) return efail(e, 9, "ensemble step launch during capture");
+  if (ee != hipSuccess) return efail(e, 9, "hipStreamEndCapture");
+  hipError_t ie = hipGraphInstantiate(&e->graph, g, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(g);
+  if (ie != hipSuccess) {
+    e->graph = nullptr;
+    return efail(e, 9, "hipGraphInstantiate");
+  }
+  e->graph_steps = batch;
+  return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* mhd_ensemble_last_error(const mhd_ensemble* e) {
+  return e ? e->err.c_str() : kNoCtx;
+}
+
+void mhd_ensemble_destroy(mhd_ensemble* e) {
+  if (!e) return;
+  if (e->graph) (void)hipGraphExecDestroy(e->graph);
+  if (e->stream) (void)hipStreamSynchronize(e->stream);
+  for (void* p : e->u)
+    if (p) (void)hipFree(p);
+  for (void* p : e->u1)
+    if (p) (void)hipFree(p);
+  if (e->ctrls) (void)hipFree(e->ctrls);
+  if (e->ctrls_host) (void)hipHostFree(e->ctrls_host);
+  if (e->desc) (void)hipFree(e->desc);
+  if (e->prefix) (void)hipFree(e->prefix);
+  if (e->queue) (void)hipFree(e->queue);
+  if (e->stream) (void)hipStreamDestroy(e->stream);
+  delete e;
+}
+
+int mhd_ensemble_create(const mhd_cfg* cfg, int n, const int* nx, const int* ny,
+                        mhd_ensemble** out) {
+  if (!out) return 1;
+  *out = nullptr;
+  if (!cfg || n < 1 || !nx || !ny) return 1;
+  if (!(cfg->cfl > 0.0 && cfg->cfl < 1.0) || !(cfg->alpha > 0.0)) return 1;
+  if (!launchers_for(cfg->precision)) return 1;
+  for (int i = 0; i < n; i++)
+    if (nx[i] < 2 || ny[i] < 2 || (long long)nx[i] * ny[i] > (long long)INT_MAX) return 1;
+  mhd_ensemble* e = new mhd_ensemble();
+  e->cfg = *cfg;
+  e->L = launchers_for(cfg->precision);
+  e->n = n;
+  if (e->cfg.batch_steps <= 0) e->cfg.batch_steps = 32;
+  if (e->cfg.hlld_tol <= 0.0) e->cfg.hlld_tol = 1.0e-4;
+  if (!(e->cfg.lowbeta_ratio > 0.0)) e->cfg.lowbeta_ratio = 16.0;
+  if (e->cfg.tile < 0 || e->cfg.tile >= mhd::TILE_COUNT) {
+    long long biggest = 0;
+    for (int i = 0; i < n; i++) biggest = std::max(biggest, (long long)nx[i] * ny[i]);
+    e->cfg.tile = tuned_tile(cfg->precision, biggest);
+  }
+  e->nx.assign(nx, nx + n);
+  e->ny.assign(ny, ny + n);
+  e->u.assign(n, nullptr);
+  e->u1.assign(n, nullptr);
+  e->have.assign(n, 0);
+  e->plane.resize(n);
+  int rc = 0;
+  auto hipok = [&](hipError_t x, const char* what) {
+    if (x != hipSuccess && rc == 0) rc = efail(e, 9, std::string(what) + ": " + hipGetErrorString(x));
+    return rc == 0;
+  };
+  hipok(hipStreamCreate(&e->stream), "hipStreamCreate");
+  for (int i = 0; i < n && rc == 0; i++) {
+    e->plane[i] = ((size_t)nx[i] * ny[i] + 63) / 64 * 64;
+    const size_t bytes = (size_t)e->L->state_planes * e->plane[i] * e->L->state_elem;
+    hipok(hipMalloc(&e->u[i], bytes), "hipMalloc");
+    if (rc == 0) hipok(hipMalloc(&e->u1[i], bytes), "hipMalloc");
+  }
+  const size_t cb = (size_t)n * e->L->ctrl_bytes;
+  if (rc == 0) hipok(hipMalloc(&e->ctrls, cb), "hipMalloc");
+  if (rc == 0) hipok(hipHostMalloc(&e->ctrls_host, cb, hipHostMallocDefault), "hipHostMalloc");
+  if (rc == 0) hipok(hipMalloc(&e->desc, (size_t)n * e->L->desc_bytes), "hipMalloc");
+  if (rc == 0) hipok(hipMalloc((void**)&e->prefix, (size_t)(n + 1) * sizeof(int)), "hipMalloc");
+  if (rc == 0) hipok(hipMalloc((void**)&e->queue, 16 * sizeof(unsigned)), "hipMalloc");
+  if (rc == 0) {
+    std::vector<char> hdesc((size_t)n * e->L->desc_bytes);
+    std::vector<int> prefix(n + 1, 0);
+    long long total = 0;
+    for (int i = 0; i < n; i++) {
+      prefix[i] = (int)total;
+      total += e->L->ens_fill(hdesc.data(), i, e->u[i], e->u1[i], e->plane[i], nx[i], ny[i],
+                              ctrl_at(e->ctrls, e, i), e->cfg.tile);
+    }
+    if (total > INT_MAX) rc = efail(e, 1, "ensemble too large");
+    prefix[n] = (int)total;
+    // Every instance starts as finished, so stray launches do nothing until
+    // a state is set.
+    for (int i = 0; i < n && rc == 0; i++) {
+      mhd_cfg ci = e->cfg;
+      ci.nx = nx[i];
+      ci.ny = ny[i];
+      if (e->L->time_is_float) init_ctrl((Ctrl<float>*)ctrl_at(e->ctrls_host, e, i), ci);
+      else init_ctrl((Ctrl<double>*)ctrl_at(e->ctrls_host, e, i), ci);
+    }
+    const unsigned zeros[16] = {0};
+    if (rc == 0) hipok(hipMemcpy(e->desc, hdesc.data(), hdesc.size(), hipMemcpyHostToDevice), "desc");
+    if (rc == 0) hipok(hipMemcpy(e->prefix, prefix.data(), prefix.size() * sizeof(int),
+                                 hipMemcpyHostToDevice), "prefix");
+    if (rc == 0) hipok(hipMemcpy(e->queue, zeros, sizeof(zeros), hipMemcpyHostToDevice), "queue");
+    if (rc == 0) hipok(hipMemcpy(e->ctrls, e->ctrls_host, cb, hipMemcpyHostToDevice), "ctrl");
+    int dev = 0, cus = 256;
+    if (rc == 0) hipok(hipGetDevice(&dev), "hipGetDevice");
+    if (rc == 0) hipok(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev),
+                       "hipDeviceGetAttribute");
+    e->el.desc = e->desc;
+    e->el.prefix = e->prefix;
+    e->el.n = n;
+    e->el.total = (int)total;
+    e->el.queue = e->queue;
+    e->el.ctrls = e->ctrls;
+    e->el.tol = e->cfg.hlld_tol;
+    e->el.alpha = e->cfg.alpha;
+    e->el.lowbeta_ratio = e->cfg.lowbeta_ratio;
+    e->el.muscl = e->cfg.muscl ? 1 : 0;
+    e->el.hlld = e->cfg.hlld ? 1 : 0;
+    e->el.tile = e->cfg.tile;
+    e->el.grid_blocks = 0;
+    if (const char* g = std::getenv("MHD_PERSISTENT_BLOCKS")) e->el.grid_blocks = std::atoi(g);
+    e->el.cus = cus;
+  }
+  if (rc != 0) {
+    std::fprintf(stderr, "mhd_ensemble_create: %s\n", e->err.c_str());
+    mhd_ensemble_destroy(e);
+    return rc;
+  }
+  *out = e;
+  return 0;
+}
+
+int mhd_ensemble_set_state(mhd_ensemble* e, int i, const void* soa9, size_t ld) {
+  if (!e || i < 0 || i >= e->n) return 1;
+  const size_t cells = (size_t)e->nx[i] * e->ny[i];
+  if (!soa9 || ld < cells) return efail(e, 1, "mhd_ensemble_set_state: bad buffer");
+  EHIP(e, upload_state(e->L, e->u[i], e->plane[i], cells, soa9, ld, e->stream));
+  mhd_cfg ci = e->cfg;
+  ci.nx = e->nx[i];
+  ci.ny = e->ny[i];
+  void* h = ctrl_at(e->ctrls_host, e, i);
+  EHIP(e, hipStreamSynchronize(e->stream));
+  if (e->L->time_is_float) init_ctrl((Ctrl<float>*)h, ci);
+  else init_ctrl((Ctrl<double>*)h, ci);
+  void* d = ctrl_at(e->ctrls, e, i);
+  EHIP(e, hipMemcpy(d, h, e->L->ctrl_bytes, hipMemcpyHostToDevice));
+  const mhd::GridDesc g = inst_grid(e, i);
+  EHIP(e, e->L->speed(g, e->u[i], d, e->stream));
+  EHIP(e, e->L->commit(d, e->stream));
+  EHIP(e, hipStreamSynchronize(e->stream));
+  e->have[i] = 1;
+  return 0;
+}
+
+int mhd_ensemble_init_problem(mhd_ensemble* e, int i, const char* problem, const double* params,
+                              int nparams) {
+  if (!e || i < 0 || i >= e->n) return 1;
+  const size_t cells = (size_t)e->nx[i] * e->ny[i];
+  std::vector<char> host(9 * cells * io_elem(e->cfg.precision));
+  int rc = mhd_problem_state_ex(problem, params, nparams, e->nx[i], e->ny[i], e->cfg.precision,
+                                host.data(), cells);
+  if (rc) return efail(e, rc, "mhd_ensemble_init_problem: bad problem or parameters");
+  return mhd_ensemble_set_state(e, i, host.data(), cells);
+}
+
+int mhd_ensemble_enqueue_steps(mhd_ensemble* e, double t_end, int nsteps) {
+  if (!e) return 1;
+  EHIP(e, e->L->ens_set_target(e->ctrls, e->n, t_end, e->queue, e->stream));
+  const int batch = e->cfg.batch_steps;
+  int left = nsteps;
+  if (use_graphs() && left >= batch) {
+    if (int rc = ens_ensure_graph(e, batch)) return rc;
+    for (; left >= batch; left -= batch) EHIP(e, hipGraphLaunch(e->graph, e->stream));
+  }
+  for (; left > 0; left--) EHIP(e, ens_enqueue_step(e));
+  return 0;
+}
+
+int mhd_ensemble_advance(mhd_ensemble* e, double t_end, long long max_steps, mhd_status* st) {
+  if (!e) return 1;
+  EHIP(e, e->L->ens_set_target(e->ctrls, e->n, t_end, e->queue, e->stream));
+  if (int rc = ens_read_ctrls(e)) return rc;
+  long long left = max_steps < 0 ? LLONG_MAX : max_steps;
+  const int batch = e->cfg.batch_steps;
+  const bool graphs = use_graphs();
+  auto running = [&]() {
+    for (int i = 0; i < e->n; i++) {
+      mhd_status s;
+      ens_status(e, i, &s);
+      if (e->have[i] && s.code == 0 && !s.done) return true;
+    }
+    return false;
+  };
+  while (left > 0 && running()) {
+    const int nb = (int)(left < batch ? left : batch);
+    if (graphs && nb == batch) {
+      if (int rc = ens_ensure_graph(e, batch)) return rc;
+      EHIP(e, hipGraphLaunch(e->graph, e->stream));
+    } else {
+      for (int q = 0; q < nb; q++) EHIP(e, ens_enqueue_step(e));
+    }
+    left -= nb;
+    if (int rc = ens_read_ctrls(e)) return rc;
+  }
+  int worst = 0;
+  for (int i = 0; i < e->n; i++) {
+    mhd_status s;
+    ens_status(e, i, &s);
+    if (st) st[i] = s;
+    if (s.code > worst) worst = s.code;
+  }
+  return worst;
+}
+
+int mhd_ensemble_get_status(mhd_ensemble* e, int i, mhd_status* st) {
+  if (!e || i < 0 || i >= e->n || !st) return 1;
+  if (int rc = ens_read_ctrls(e)) return rc;
+  ens_status(e, i, st);
+  return st->code;
+}
+
+int mhd_ensemble_get_state(mhd_ensemble* e, int i, int which, void* soa9, size_t ld) {
+  if (!e || i < 0 || i >= e->n || (which != 0 && which != 1)) return 1;
+  const size_t cells = (size_t)e->nx[i] * e->ny[i];
+  if (!soa9 || ld < cells) return efail(e, 1, "mhd_ensemble_get_state: bad buffer");
+  EHIP(e, download_state(e->L, which == 0 ? e->u[i] : e->u1[i], e->plane[i], cells, soa9, ld,
+                         e->stream));
+  return 0;
+}
+
+int mhd_ensemble_time_kernels(mhd_ensemble* e, double t_end, int nsteps, double* avg_ms) {
+  if (!e || !avg_ms || nsteps <= 0) return 1;
+  EHIP(e, e->L->ens_set_target(e->ctrls, e->n, t_end, e->queue, e->stream));
+  std::vector<hipEvent_t> ev((size_t)nsteps * 4, nullptr);
+  for (auto& x : ev) EHIP(e, hipEventCreate(&x));
+  int rc = 0;
+  for (int q = 0; q < nsteps && rc == 0; q++) {
+    hipError_t x = ens_enqueue_step(e, &ev[(size_t)q * 4]);
+    if (x != hipSuccess) rc = efail(e, 9, "timed ensemble step");
+  }
+  if (rc == 0 && hipStreamSynchronize(e->stream) != hipSuccess) rc = efail(e, 9, "sync");
+  if (rc == 0) {
+    double sum[3] = {0, 0, 0};
+    for (int q = 0; q < nsteps; q++)
+      for (int k = 0; k < 3; k++) {
+        float ms = 0.f;
+        (void)hipEventElapsedTime(&ms, ev[(size_t)q * 4 + k], ev[(size_t)q * 4 + k + 1]);
+        sum[k] += ms;
+      }
+    for (int k = 0; k < 3; k++) avg_ms[k] = sum[k] / nsteps;
+  }
+  for (auto& x : ev)
+    if (x) (void)hipEventDestroy(x);
+  return rc;
 }
 
 }  // extern "C"

@@ -67,6 +67,25 @@ enum TileShape {
   TILE_16x7 = 5, TILE_COUNT = 6
 };
 constexpr int kDefaultTile = TILE_16x15;
+inline void tile_dims(int tile, int& tx, int& ty) {
+  static const int dims[TILE_COUNT][2] = {{16, 16}, {32, 8}, {16, 8}, {16, 15}, {32, 7}, {16, 7}};
+  if (tile < 0 || tile >= TILE_COUNT) tile = kDefaultTile;
+  tx = dims[tile][0];
+  ty = dims[tile][1];
+}
+
+// Device description of an ensemble (type-erased; built by mhd_api.cpp).
+struct EnsLaunch {
+  const void* desc;     // device array of per-instance descriptors (desc_bytes each)
+  const int* prefix;    // device: first work item of each instance, then the total
+  int n, total;         // instances, work items (tiles over all instances)
+  unsigned* queue;      // device: 16 work-queue counters
+  void* ctrls;          // device: n control blocks, ctrl_bytes apart
+  double tol, alpha, lowbeta_ratio;
+  int muscl, hlld, tile;
+  int grid_blocks;      // persistent grid; <= 0: occupancy x CUs
+  int cus;              // compute units of the device
+};
 
 // Launchers of one arithmetic form.  All launches go to `stream`; none
 // synchronises.  State buffers hold `state_planes` planes of `plane`
@@ -84,6 +103,14 @@ struct Launchers {
   // kernels and after the last (ev[0..3]).
   hipError_t (*step)(const GridDesc&, void* u, void* u1, void* ctrl, hipStream_t,
                      hipEvent_t* ev);
+  // Ensembles.
+  size_t desc_bytes;
+  // Fills descriptor i of a host array; returns the instance's tile count.
+  int (*ens_fill)(void* host_desc, int i, void* u, void* u1, size_t plane, int nx, int ny,
+                  void* ctrl, int tile);
+  // One step of every running instance (persistent stage kernels + finish).
+  hipError_t (*ens_step)(const EnsLaunch&, hipStream_t, hipEvent_t* ev);
+  hipError_t (*ens_set_target)(void* ctrls, int n, double t_end, unsigned* queue, hipStream_t);
 };
 
 const Launchers* launchers_exact(int mode);  // MODE_FP64, MODE_FP32

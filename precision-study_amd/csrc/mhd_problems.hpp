@@ -9,6 +9,9 @@
 //         shear vx=0.5(tanh((y-.25)/.05)-tanh((y-.75)/.05)-1), Bx=0.1,
 //         vy=0.01 sin(4 pi x)(exp(-(y-.25)^2/.005)+exp(-(y-.75)^2/.005)));
 //         no reference implementation exists for it.
+//   otv : Orszag-Tang with the ensemble workload's variations: params =
+//         {B amplitude, phase of vx, vy, Bx, By} added to the sine arguments;
+//         with {1, 0, 0, 0, 0} it is bitwise `ot`.
 #ifndef MHD_PROBLEMS_HPP
 #define MHD_PROBLEMS_HPP
 
@@ -18,13 +21,17 @@
 namespace mhd {
 
 inline bool problem_known(const char* p) {
-  return p && (!std::strcmp(p, "ot") || !std::strcmp(p, "kh") || !std::strcmp(p, "khb"));
+  return p && (!std::strcmp(p, "ot") || !std::strcmp(p, "kh") || !std::strcmp(p, "khb") ||
+               !std::strcmp(p, "otv"));
 }
+
+// Number of parameters a problem takes.
+inline int problem_nparams(const char* p) { return (p && !std::strcmp(p, "otv")) ? 5 : 0; }
 
 // Primitive state (rho, vx, vy, vz, p, Bx, By, Bz, psi) of cell (i,j).
 template <class R>
 inline void problem_primitive(const char* problem, int i, int j, R dx, R dy,
-                              R P[9]) {
+                              R P[9], const double* params = nullptr) {
   const R x = (R)0.0 + ((R)i + (R)0.5) * dx;
   const R y = (R)0.0 + ((R)j + (R)0.5) * dy;
   const R pi = std::acos((R)-1.0);
@@ -53,6 +60,16 @@ inline void problem_primitive(const char* problem, int i, int j, R dx, R dy,
     P[4] = (R)2.5;
     P[5] = (R)0.1;
     P[6] = (R)0.0;
+  } else if (!std::strcmp(problem, "otv")) {
+    const R amp = (R)params[0];
+    const R inv_sqrt_4pi = (R)1.0 / std::sqrt((R)4.0 * pi);
+    P[0] = (R)25.0 / ((R)36.0 * pi);
+    P[1] = -std::sin(two_pi * y + (R)params[1]);
+    P[2] = std::sin(two_pi * x + (R)params[2]);
+    P[3] = (R)0.0;
+    P[4] = (R)5.0 / ((R)12.0 * pi);
+    P[5] = -amp * std::sin(two_pi * y + (R)params[3]) * inv_sqrt_4pi;
+    P[6] = amp * std::sin((R)2.0 * two_pi * x + (R)params[4]) * inv_sqrt_4pi;
   } else {
     const R inv_sqrt_4pi = (R)1.0 / std::sqrt((R)4.0 * pi);
     P[0] = (R)25.0 / ((R)36.0 * pi);

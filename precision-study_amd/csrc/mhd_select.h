@@ -1,0 +1,43 @@
+// mhd_select.h -- internal interface of the learned precision selector
+// kernels (k_select.hip).  Not part of the public C ABI.
+#ifndef MHD_SELECT_H
+#define MHD_SELECT_H
+
+#include <hip/hip_runtime_api.h>
+
+#include <cstddef>
+
+namespace mhd {
+namespace sel {
+constexpr int NFEAT = 12;   // features per tile
+constexpr int NIN = 16;     // features padded to the MFMA K dimension
+constexpr int H = 64;       // hidden width
+constexpr int kParams = NIN * H + H + H * H + H + H + 1;  // FP32 master weights
+}  // namespace sel
+
+// feats: ntiles x NIN raw features of the FP64 state (c_h read from *ctrl_ch).
+hipError_t sel_tile_features(const double* u, size_t plane, int nx, int ny, int tx, int ty,
+                             const void* ctrl_ch, float* feats, hipStream_t s);
+// err[t]: max over tile t's cells and variables of |a-b| / (|b| + rms_k(b)).
+hipError_t sel_tile_errors(const double* a, const double* b, size_t plane, int nx, int ny, int tx,
+                           int ty, double* sumsq9, float* err, hipStream_t s);
+// prob/flags of n samples (raw features standardised with mean/istd when
+// mean is non-null); flags[s] = prob > threshold || sticky[s].
+hipError_t sel_mlp_infer(const float* raw, int n, const float* params, const float* mean,
+                         const float* istd, float threshold, const unsigned char* sticky,
+                         float* prob, unsigned char* flags, hipStream_t s);
+// Adds the minibatch gradient (samples idx[offset .. offset+count)) into grad
+// and the weighted loss into *loss.
+hipError_t sel_mlp_grad(const float* X, const float* Y, const float* W, const int* idx, int offset,
+                        int count, float wsum, const float* params, float* grad, float* loss,
+                        hipStream_t s);
+hipError_t sel_adam(float* params, float* grad, float* m, float* v, int step, float lr,
+                    hipStream_t s);
+hipError_t sel_audit_pick(const unsigned char* flags, int ntiles, unsigned step, float frac,
+                          unsigned char* audit, hipStream_t s);
+hipError_t sel_audit_compare(const float* err, const unsigned char* audit, int ntiles, float budget,
+                             unsigned char* sticky, unsigned long long* counters, hipStream_t s);
+
+}  // namespace mhd
+
+#endif  // MHD_SELECT_H

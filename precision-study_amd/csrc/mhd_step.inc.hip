@@ -452,29 +452,41 @@ __device__ __forceinline__ void update_cell(const StageArgs<KahanPol>& a,
 template <class Pol> struct RhsType { using type = typename Pol::T; };
 template <> struct RhsType<KahanPol> { using type = float; };
 
+// LDS footprint of one tile.
+template <class R, int TX, int TY>
+struct TileShared {
+  static constexpr int NT = TX * TY;
+  static constexpr int HX = TX + 4, HY = TY + 4, HN = HX * HY;
+  static constexpr int NXF = (TX + 1) * TY, NYF = TX * (TY + 1), NF = NXF + NYF;
+  static constexpr int NW = (NT + 63) / 64;
+  static constexpr int NTHR = NW * 64;  // threads per block: whole waves
+  R sP[9 * HN];
+  R sF[9 * NF];
+  unsigned long long sRed[3 * NW];
+};
+
+// One stage on one tile (tile index `tile` of the instance described by
+// `a`), by one workgroup.  The caller has checked that the instance is
+// running; every thread of the block must call this.
 template <class Pol, bool MUSCL, bool HLLD, int STAGE, int TX, int TY>
-__global__ __launch_bounds__((TX * TY + 63) / 64 * 64) void k_stage(StageArgs<Pol> a) {
+__device__ __forceinline__ void stage_tile(const StageArgs<Pol>& a, int tile,
+                                           TileShared<typename Pol::R, TX, TY>& sh) {
   using R = typename Pol::R;
   using T = typename Pol::T;
   using A = typename RhsType<Pol>::type;
-  constexpr int NT = TX * TY;
-  constexpr int HX = TX + 4, HY = TY + 4, HN = HX * HY;
-  constexpr int NXF = (TX + 1) * TY, NYF = TX * (TY + 1), NF = NXF + NYF;
-  constexpr int NW = (NT + 63) / 64;
-  constexpr int NTHR = NW * 64;  // threads per block: whole waves
-  __shared__ R sP[9 * HN];
-  __shared__ R sF[9 * NF];
-  __shared__ unsigned long long sRed[3 * NW];
+  using Sh = TileShared<R, TX, TY>;
+  constexpr int NT = Sh::NT, HX = Sh::HX, HN = Sh::HN;
+  constexpr int NXF = Sh::NXF, NF = Sh::NF, NW = Sh::NW, NTHR = Sh::NTHR;
+  R* sP = sh.sP;
+  R* sF = sh.sF;
+  unsigned long long* sRed = sh.sRed;
 
   Ctrl<T>* ctrl = a.ctrl;
-  if (ctrl->run != RUN_ACTIVE) return;
-  if (STAGE == 2 && ctrl->bad[0] != INT_MAX) return;
   const T ch_t = ctrl->ch, dt = ctrl->dt, dx = ctrl->dx, dy = ctrl->dy;
   const R ch = (R)ch_t;
   const Phys<R> c = make_phys<R>(a.tol);
   const int nx = a.nx, ny = a.ny;
   const size_t ps = a.plane;
-  const int tile = xcd_tile((int)blockIdx.x, a.ntiles);
   const int ty = tile / a.tiles_x, tx = tile - ty * a.tiles_x;
   const int i0 = tx * TX, j0 = ty * TY;
   const int tid = (int)threadIdx.x;
@@ -636,6 +648,16 @@ __global__ __launch_bounds__((TX * TY + 63) / 64 * 64) void k_stage(StageArgs<Po
   }
 }
 
+// One instance, one block per tile (XCD-banded tile order).
+template <class Pol, bool MUSCL, bool HLLD, int STAGE, int TX, int TY>
+__global__ __launch_bounds__((TX * TY + 63) / 64 * 64) void k_stage(StageArgs<Pol> a) {
+  __shared__ TileShared<typename Pol::R, TX, TY> sh;
+  Ctrl<typename Pol::T>* ctrl = a.ctrl;
+  if (ctrl->run != RUN_ACTIVE) return;
+  if (STAGE == 2 && ctrl->bad[0] != INT_MAX) return;
+  stage_tile<Pol, MUSCL, HLLD, STAGE, TX, TY>(a, xcd_tile((int)blockIdx.x, a.ntiles), sh);
+}
+
 // Signal-speed maximum of a state (initial state / after set_state), in T.
 template <class Pol>
 __device__ __forceinline__ void load_wide(const typename Pol::S* src, size_t ps, size_t g,
@@ -732,9 +754,10 @@ __global__ void k_set_target(Ctrl<T>* ctrl, double t_end) {
   prepare_step(ctrl);
 }
 
+// End of a step for one instance: verdict of the positivity checks, then
+// t += dt, step++ and the next step's c_h and dt (mhd2d_glm_cpu.cpp:538-569).
 template <class T>
-__global__ void k_finish(Ctrl<T>* ctrl) {
-  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+__device__ __forceinline__ void finish_one(Ctrl<T>* ctrl) {
   if (ctrl->run != RUN_ACTIVE) return;
   if (ctrl->bad[0] != INT_MAX || ctrl->bad[1] != INT_MAX) {
     const bool first = ctrl->bad[0] != INT_MAX;
@@ -754,6 +777,120 @@ __global__ void k_finish(Ctrl<T>* ctrl) {
   ctrl->valid_cur = ctrl->valid_acc;
   ctrl->speed_acc = 0;
   ctrl->valid_acc = 0;
+  prepare_step(ctrl);
+}
+
+template <class T>
+__global__ void k_finish(Ctrl<T>* ctrl) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  finish_one(ctrl);
+}
+
+// ---------------------------------------------------------------------------
+// Ensembles: many instances of uneven sizes, each with its own dt and
+// control block, advanced by one persistent kernel per stage that pulls
+// (instance, tile) items from a work queue.  The queue is split into 8
+// contiguous chunks of the item list, one per XCD (blocks are dealt to the
+// XCDs round-robin), so neighbouring tiles share an XCD's L2; a block that
+// drains its chunk steals from the others.
+// ---------------------------------------------------------------------------
+template <class Pol>
+struct InstDesc {
+  typename Pol::S* u;
+  typename Pol::S* u1;
+  size_t plane;
+  int nx, ny, tiles_x, ntiles;
+  int pow2;
+  double inv_dx, inv_dy;
+  Ctrl<typename Pol::T>* ctrl;
+};
+
+template <class Pol>
+struct EnsArgs {
+  const InstDesc<Pol>* desc;
+  const int* prefix;  // prefix[i] = first item of instance i; prefix[n] = total
+  int n, total;
+  unsigned* queue;    // 2 stages x 8 chunk counters, zeroed by the finish kernel
+  double tol, alpha;
+  float lowbeta_ratio;
+};
+
+__device__ __forceinline__ int find_instance(const int* prefix, int n, int item) {
+  int lo = 0, hi = n - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (prefix[mid] <= item) lo = mid; else hi = mid - 1;
+  }
+  return lo;
+}
+
+template <class Pol, bool MUSCL, bool HLLD, int STAGE, int TX, int TY>
+__global__ __launch_bounds__((TX * TY + 63) / 64 * 64) void k_stage_queue(EnsArgs<Pol> e) {
+  __shared__ TileShared<typename Pol::R, TX, TY> sh;
+  __shared__ int s_item;
+  unsigned* q = e.queue + (STAGE - 1) * 8;
+  const int home = (int)(blockIdx.x & 7);
+  int chunk = 0;
+  for (;;) {
+    if (threadIdx.x == 0) {
+      int item = -1;
+      while (chunk < 8) {
+        const int x = (home + chunk) & 7;
+        const int lo = (int)((long long)e.total * x / 8);
+        const int hi = (int)((long long)e.total * (x + 1) / 8);
+        const unsigned k = atomicAdd(&q[x], 1u);
+        if ((long long)lo + k < (long long)hi) {
+          item = lo + (int)k;
+          break;
+        }
+        chunk++;
+      }
+      s_item = item;
+    }
+    __syncthreads();
+    const int item = s_item;
+    __syncthreads();
+    if (item < 0) break;  // every chunk drained: every block gets here
+    const int inst = find_instance(e.prefix, e.n, item);
+    const InstDesc<Pol>& d = e.desc[inst];
+    if (d.ctrl->run != RUN_ACTIVE) continue;
+    if (STAGE == 2 && d.ctrl->bad[0] != INT_MAX) continue;
+    StageArgs<Pol> a;
+    a.plane = d.plane;
+    a.nx = d.nx;
+    a.ny = d.ny;
+    a.tiles_x = d.tiles_x;
+    a.ntiles = d.ntiles;
+    a.pow2 = d.pow2;
+    a.inv_dx = d.inv_dx;
+    a.inv_dy = d.inv_dy;
+    a.tol = e.tol;
+    a.alpha = e.alpha;
+    a.lowbeta_ratio = e.lowbeta_ratio;
+    a.ctrl = d.ctrl;
+    a.src = STAGE == 1 ? d.u : d.u1;
+    a.uold = d.u;
+    a.dst = STAGE == 1 ? d.u1 : d.u;
+    stage_tile<Pol, MUSCL, HLLD, STAGE, TX, TY>(a, item - e.prefix[inst], sh);
+  }
+}
+
+template <class T>
+__global__ void k_finish_multi(Ctrl<T>* ctrls, int n, unsigned* queue) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) finish_one(&ctrls[i]);
+  if (i < 16) queue[i] = 0;
+}
+
+template <class T>
+__global__ void k_set_target_multi(Ctrl<T>* ctrls, int n, double t_end, unsigned* queue) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < 16) queue[i] = 0;
+  if (i >= n) return;
+  Ctrl<T>* ctrl = &ctrls[i];
+  if (ctrl->run == RUN_HALTED) return;
+  ctrl->t_end = (T)t_end;
+  ctrl->run = RUN_ACTIVE;
   prepare_step(ctrl);
 }
 
@@ -852,6 +989,87 @@ static hipError_t launch_target(void* ctrl, double t_end, hipStream_t s) {
 }
 
 template <class Pol>
+static int ens_fill(void* host_desc, int i, void* u, void* u1, size_t plane, int nx, int ny,
+                    void* ctrl, int tile) {
+  using T = typename Pol::T;
+  int tx, ty;
+  tile_dims(tile, tx, ty);
+  InstDesc<Pol>& d = ((InstDesc<Pol>*)host_desc)[i];
+  d.u = (typename Pol::S*)u;
+  d.u1 = (typename Pol::S*)u1;
+  d.plane = plane;
+  d.nx = nx;
+  d.ny = ny;
+  d.tiles_x = (nx + tx - 1) / tx;
+  d.ntiles = d.tiles_x * ((ny + ty - 1) / ty);
+  const T dx = ((T)1.0 - (T)0.0) / (T)nx, dy = ((T)1.0 - (T)0.0) / (T)ny;
+  d.pow2 = ((nx & (nx - 1)) == 0 && (ny & (ny - 1)) == 0) ? 1 : 0;
+  d.inv_dx = (double)((T)1.0 / dx);
+  d.inv_dy = (double)((T)1.0 / dy);
+  d.ctrl = (Ctrl<T>*)ctrl;
+  return d.ntiles;
+}
+
+template <class Pol, bool MUSCL, bool HLLD, int TX, int TY>
+static hipError_t ens_step_t(const EnsLaunch& L, hipStream_t s, hipEvent_t* ev) {
+  using T = typename Pol::T;
+  constexpr int nthr = (TX * TY + 63) / 64 * 64;
+  EnsArgs<Pol> e;
+  e.desc = (const InstDesc<Pol>*)L.desc;
+  e.prefix = L.prefix;
+  e.n = L.n;
+  e.total = L.total;
+  e.queue = L.queue;
+  e.tol = L.tol;
+  e.alpha = L.alpha;
+  e.lowbeta_ratio = (float)L.lowbeta_ratio;
+  int grid = L.grid_blocks;
+  if (grid <= 0) {
+    static int per_cu = 0;
+    if (!per_cu) {
+      if (hipOccupancyMaxActiveBlocksPerMultiprocessor(
+              &per_cu, reinterpret_cast<const void*>(&k_stage_queue<Pol, MUSCL, HLLD, 1, TX, TY>),
+              nthr, 0) != hipSuccess || per_cu < 1)
+        per_cu = 1;
+    }
+    grid = per_cu * (L.cus > 0 ? L.cus : 256);
+  }
+  if (ev) (void)hipEventRecord(ev[0], s);
+  hipLaunchKernelGGL((k_stage_queue<Pol, MUSCL, HLLD, 1, TX, TY>), dim3(grid), dim3(nthr), 0, s, e);
+  if (ev) (void)hipEventRecord(ev[1], s);
+  hipLaunchKernelGGL((k_stage_queue<Pol, MUSCL, HLLD, 2, TX, TY>), dim3(grid), dim3(nthr), 0, s, e);
+  if (ev) (void)hipEventRecord(ev[2], s);
+  hipLaunchKernelGGL(k_finish_multi<T>, dim3((L.n + 63) / 64 > 0 ? (L.n + 63) / 64 : 1), dim3(64),
+                     0, s, (Ctrl<T>*)L.ctrls, L.n, L.queue);
+  if (ev) (void)hipEventRecord(ev[3], s);
+  return hipGetLastError();
+}
+
+template <class Pol>
+static hipError_t ens_step(const EnsLaunch& L, hipStream_t s, hipEvent_t* ev) {
+  if (!(L.muscl && L.hlld)) {
+    if (L.muscl) return ens_step_t<Pol, true, false, 16, 15>(L, s, ev);
+    if (L.hlld) return ens_step_t<Pol, false, true, 16, 15>(L, s, ev);
+    return ens_step_t<Pol, false, false, 16, 15>(L, s, ev);
+  }
+  switch (L.tile) {
+    case TILE_16x16: return ens_step_t<Pol, true, true, 16, 16>(L, s, ev);
+    case TILE_32x8: return ens_step_t<Pol, true, true, 32, 8>(L, s, ev);
+    case TILE_16x8: return ens_step_t<Pol, true, true, 16, 8>(L, s, ev);
+    case TILE_32x7: return ens_step_t<Pol, true, true, 32, 7>(L, s, ev);
+    case TILE_16x7: return ens_step_t<Pol, true, true, 16, 7>(L, s, ev);
+    default: return ens_step_t<Pol, true, true, 16, 15>(L, s, ev);
+  }
+}
+
+template <class T>
+static hipError_t ens_set_target(void* ctrls, int n, double t_end, unsigned* queue, hipStream_t s) {
+  hipLaunchKernelGGL(k_set_target_multi<T>, dim3((n + 63) / 64 > 0 ? (n + 63) / 64 : 1), dim3(64), 0,
+                     s, (Ctrl<T>*)ctrls, n, t_end, queue);
+  return hipGetLastError();
+}
+
+template <class Pol>
 static Launchers make_launchers(int mode) {
   Launchers L;
   L.mode = mode;
@@ -863,6 +1081,10 @@ static Launchers make_launchers(int mode) {
   L.commit = &launch_commit<typename Pol::T>;
   L.set_target = &launch_target<typename Pol::T>;
   L.step = &launch_step<Pol>;
+  L.desc_bytes = sizeof(InstDesc<Pol>);
+  L.ens_fill = &ens_fill<Pol>;
+  L.ens_step = &ens_step<Pol>;
+  L.ens_set_target = &ens_set_target<typename Pol::T>;
   return L;
 }
 

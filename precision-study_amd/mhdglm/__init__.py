@@ -42,7 +42,11 @@ EXPORTED = (
     "mhd_set_state", "mhd_init_problem", "mhd_advance", "mhd_get_status",
     "mhd_get_state", "mhd_device_state", "mhd_stream", "mhd_enqueue_steps",
     "mhd_time_kernels", "mhd_checkpoint_save", "mhd_checkpoint_load", "mhd_glm_flux_batch", "mhd_cons_to_prim_batch",
-    "mhd_fast_speed_batch",
+    "mhd_fast_speed_batch", "mhd_problem_state_ex",
+    "mhd_ensemble_create", "mhd_ensemble_destroy", "mhd_ensemble_last_error",
+    "mhd_ensemble_set_state", "mhd_ensemble_init_problem", "mhd_ensemble_advance",
+    "mhd_ensemble_get_status", "mhd_ensemble_get_state", "mhd_ensemble_enqueue_steps",
+    "mhd_ensemble_time_kernels", "mhd_autotuned_tile",
 )
 
 
@@ -128,6 +132,30 @@ def load(path: str = LIB_PATH) -> ctypes.CDLL:
     lib.mhd_cons_to_prim_batch.restype = I
     lib.mhd_fast_speed_batch.argtypes = [I, I, I, P, P]
     lib.mhd_fast_speed_batch.restype = I
+    lib.mhd_autotuned_tile.argtypes = [I, I, I]
+    lib.mhd_autotuned_tile.restype = I
+    lib.mhd_problem_state_ex.argtypes = [ctypes.c_char_p, P, I, I, I, I, P, S]
+    lib.mhd_problem_state_ex.restype = I
+    lib.mhd_ensemble_create.argtypes = [ctypes.POINTER(Cfg), I, P, P, ctypes.POINTER(P)]
+    lib.mhd_ensemble_create.restype = I
+    lib.mhd_ensemble_destroy.argtypes = [P]
+    lib.mhd_ensemble_destroy.restype = None
+    lib.mhd_ensemble_last_error.argtypes = [P]
+    lib.mhd_ensemble_last_error.restype = ctypes.c_char_p
+    lib.mhd_ensemble_set_state.argtypes = [P, I, P, S]
+    lib.mhd_ensemble_set_state.restype = I
+    lib.mhd_ensemble_init_problem.argtypes = [P, I, ctypes.c_char_p, P, I]
+    lib.mhd_ensemble_init_problem.restype = I
+    lib.mhd_ensemble_advance.argtypes = [P, D, ctypes.c_longlong, P]
+    lib.mhd_ensemble_advance.restype = I
+    lib.mhd_ensemble_get_status.argtypes = [P, I, ctypes.POINTER(Status)]
+    lib.mhd_ensemble_get_status.restype = I
+    lib.mhd_ensemble_get_state.argtypes = [P, I, I, P, S]
+    lib.mhd_ensemble_get_state.restype = I
+    lib.mhd_ensemble_enqueue_steps.argtypes = [P, D, I]
+    lib.mhd_ensemble_enqueue_steps.restype = I
+    lib.mhd_ensemble_time_kernels.argtypes = [P, D, I, ctypes.POINTER(ctypes.c_double)]
+    lib.mhd_ensemble_time_kernels.restype = I
     _LIB = lib
     return lib
 
@@ -143,6 +171,20 @@ def problem_state(problem: str, nx: int, ny: int, precision: int = PREC_FP64) ->
     rc = lib.mhd_problem_state(problem.encode(), nx, ny, precision, _ptr(out), nx * ny)
     if rc:
         raise MhdError(rc, f"unknown problem {problem!r} or bad size")
+    return out
+
+
+def problem_state_ex(problem: str, params, nx: int, ny: int,
+                     precision: int = PREC_FP64) -> np.ndarray:
+    """Initial state of a problem with parameters ("otv": B amplitude and
+    the phases of the vx, vy, Bx, By sine terms), shape (9, ny, nx)."""
+    lib = load()
+    out = np.empty((9, ny, nx), dtype=DTYPES[precision])
+    p = np.ascontiguousarray(params, dtype=np.float64)
+    rc = lib.mhd_problem_state_ex(problem.encode(), _ptr(p), p.size, nx, ny, precision,
+                                  _ptr(out), nx * ny)
+    if rc:
+        raise MhdError(rc, f"bad problem {problem!r} or parameters")
     return out
 
 
@@ -167,8 +209,7 @@ class Solver:
     def __init__(self, cfg: SolverConfig):
         self.lib = load()
         self.cfg = cfg
-        c = Cfg(cfg.nx, cfg.ny, cfg.cfl, cfg.alpha, int(cfg.muscl), int(cfg.hlld),
-                cfg.precision, cfg.hlld_tol, cfg.tile, cfg.batch_steps, cfg.lowbeta_ratio)
+        c = _cfg(cfg)
         h = ctypes.c_void_p()
         rc = self.lib.mhd_create(ctypes.byref(c), ctypes.byref(h))
         if rc:
@@ -232,6 +273,79 @@ class Solver:
 
     def stream(self) -> int:
         return self.lib.mhd_stream(self.h)
+
+
+def _cfg(cfg: "SolverConfig", nx: int = 0, ny: int = 0) -> Cfg:
+    return Cfg(nx or cfg.nx, ny or cfg.ny, cfg.cfl, cfg.alpha, int(cfg.muscl), int(cfg.hlld),
+               cfg.precision, cfg.hlld_tol, cfg.tile, cfg.batch_steps, cfg.lowbeta_ratio)
+
+
+class Ensemble:
+    """Many instances of uneven sizes advanced together (persistent
+    work-queue kernels, per-instance dt) behind the C ABI."""
+
+    def __init__(self, cfg: SolverConfig, sizes):
+        self.lib = load()
+        self.cfg = cfg
+        self.sizes = [(int(a), int(b)) for a, b in sizes]
+        nx = np.array([a for a, _ in self.sizes], dtype=np.int32)
+        ny = np.array([b for _, b in self.sizes], dtype=np.int32)
+        c = _cfg(cfg, 2, 2)
+        h = ctypes.c_void_p()
+        rc = self.lib.mhd_ensemble_create(ctypes.byref(c), len(self.sizes), _ptr(nx), _ptr(ny),
+                                          ctypes.byref(h))
+        if rc:
+            raise MhdError(rc, "mhd_ensemble_create failed")
+        self.h = h
+        self.dtype = DTYPES[cfg.precision]
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.mhd_ensemble_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def _check(self, rc, allow=(0,)):
+        if rc not in allow:
+            raise MhdError(rc, self.lib.mhd_ensemble_last_error(self.h).decode())
+        return rc
+
+    def set_state(self, i: int, u: np.ndarray):
+        u = np.ascontiguousarray(u, dtype=self.dtype).reshape(9, -1)
+        self._check(self.lib.mhd_ensemble_set_state(self.h, i, _ptr(u), u.shape[1]))
+
+    def init_problem(self, i: int, problem: str = "ot", params=()):
+        p = np.ascontiguousarray(params, dtype=np.float64)
+        self._check(self.lib.mhd_ensemble_init_problem(self.h, i, problem.encode(),
+                                                       _ptr(p) if p.size else None, p.size))
+
+    def advance(self, t_end: float, max_steps: int = -1):
+        st = (Status * len(self.sizes))()
+        self._check(self.lib.mhd_ensemble_advance(self.h, t_end, max_steps, st),
+                    allow=(0, 2, 3, 4))
+        return list(st)
+
+    def status(self, i: int) -> Status:
+        st = Status()
+        self._check(self.lib.mhd_ensemble_get_status(self.h, i, ctypes.byref(st)),
+                    allow=(0, 2, 3, 4))
+        return st
+
+    def get_state(self, i: int, which: int = 0) -> np.ndarray:
+        nx, ny = self.sizes[i]
+        out = np.empty((9, ny, nx), dtype=self.dtype)
+        self._check(self.lib.mhd_ensemble_get_state(self.h, i, which, _ptr(out), nx * ny))
+        return out
+
+    def enqueue_steps(self, t_end: float, nsteps: int):
+        self._check(self.lib.mhd_ensemble_enqueue_steps(self.h, t_end, nsteps))
+
+    def time_kernels(self, t_end: float, nsteps: int):
+        out = (ctypes.c_double * 3)()
+        self._check(self.lib.mhd_ensemble_time_kernels(self.h, t_end, nsteps, out))
+        return tuple(out)
 
 
 def _aos(a, dtype) -> np.ndarray:

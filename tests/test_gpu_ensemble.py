@@ -1,0 +1,94 @@
+"""GPU tests of ensembles (persistent work-queue stage kernels).
+
+Contract: an instance advanced inside an ensemble is bitwise the same
+instance advanced alone by a single context (same form, same tile shape),
+for uneven sizes, instances that finish at different step counts, and an
+instance that halts (which must not disturb the others).
+"""
+import sys
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+def single(mhd, cfg, nx, ny, problem, params, t_end, steps=-1, state=None):
+    s = mhd.Solver(mhd.SolverConfig(**{**cfg.__dict__, "nx": nx, "ny": ny}))
+    if state is not None:
+        s.set_state(state)
+    elif params:
+        s.set_state(mhd.problem_state_ex(problem, params, nx, ny, cfg.precision))
+    else:
+        s.init_problem(problem)
+    st = s.advance(t_end, steps)
+    u = s.get_state(0)
+    s.close()
+    return u, st
+
+
+def bits(u):
+    return np.ascontiguousarray(u).view(np.uint8)
+
+
+@pytest.mark.parametrize("form", ["fp64", "mixed"])
+def test_ensemble_matches_single(mhd, form):
+    cfg = mhd.SolverConfig(nx=2, ny=2, precision=mhd.PRECISIONS[form])
+    sizes = [(64, 64), (48, 80), (96, 32), (40, 40), (33, 57)]
+    params = [(1.0, 0, 0, 0, 0), (0.9, 0.3, 1.1, 2.0, 5.5), (1.2, 6.0, 0.1, 3.3, 0.7),
+              (0.8, 1.0, 2.0, 3.0, 4.0), (1.05, 0.5, 0.5, 0.5, 0.5)]
+    say(f"ensemble {form} {sizes}")
+    ens = mhd.Ensemble(cfg, sizes)
+    for i, p in enumerate(params):
+        ens.init_problem(i, "otv", p)
+    sts = ens.advance(0.05)
+    for i, ((nx, ny), p) in enumerate(zip(sizes, params)):
+        u_e = ens.get_state(i)
+        u_s, st_s = single(mhd, cfg, nx, ny, "otv", p, 0.05)
+        assert sts[i].code == 0 and sts[i].done == 1
+        assert sts[i].steps == st_s.steps and sts[i].t == st_s.t
+        assert (sts[i].degenerate, sts[i].nonphysical) == (st_s.degenerate, st_s.nonphysical)
+        assert np.array_equal(bits(u_e), bits(u_s)), f"instance {i} {nx}x{ny}"
+    # Different sizes take different step counts to t_end.
+    assert len({s.steps for s in sts}) > 1
+    ens.close()
+
+
+def test_ensemble_halted_instance_isolated(mhd, oracle):
+    cfg = mhd.SolverConfig(nx=2, ny=2)
+    sizes = [(32, 32), (40, 24), (32, 32)]
+    ens = mhd.Ensemble(cfg, sizes)
+    ens.init_problem(0, "ot")
+    bad = mhd.problem_state("ot", 40, 24)
+    bad[4, 10, 7] = 1e-8     # E far below kinetic + magnetic energy
+    ens.set_state(1, bad)
+    ens.init_problem(2, "ot")
+    sts = ens.advance(0.02)
+    assert sts[1].code == 3 and (sts[1].bad_i, sts[1].bad_j) == (7, 10) and sts[1].steps == 0
+    for i in (0, 2):
+        assert sts[i].code == 0 and sts[i].done == 1
+        u_s, st_s = single(mhd, cfg, 32, 32, "ot", None, 0.02)
+        assert np.array_equal(bits(ens.get_state(i)), bits(u_s))
+    ens.close()
+
+
+def test_ensemble_steps_limit_and_graph(mhd):
+    """max_steps stops every instance at the same step count; batches replayed
+    as graphs give the same states as single launches."""
+    cfg = mhd.SolverConfig(nx=2, ny=2, batch_steps=8)
+    sizes = [(64, 48), (48, 64)]
+    ens = mhd.Ensemble(cfg, sizes)
+    for i in range(2):
+        ens.init_problem(i, "ot")
+    sts = ens.advance(0.5, 20)
+    assert [s.steps for s in sts] == [20, 20]
+    for i, (nx, ny) in enumerate(sizes):
+        u_s, _ = single(mhd, mhd.SolverConfig(nx=2, ny=2, batch_steps=1), nx, ny, "ot", None, 0.5,
+                        steps=20)
+        assert np.array_equal(bits(ens.get_state(i)), bits(u_s))
+    ens.close()

@@ -73,3 +73,47 @@ def test_cli_usage_errors(mhd, tmp_path, args, msg):
 def test_cli_rejects_bad_precision(mhd, tmp_path):
     r = run_cli(mhd, ["8", "8"], tmp_path, precision="fp16")
     assert r.returncode == 1
+
+
+def test_otv_problem_reduces_to_ot(mhd):
+    for prec in (0, 1):
+        a = mhd.problem_state("ot", 40, 36, prec)
+        b = mhd.problem_state_ex("otv", (1.0, 0.0, 0.0, 0.0, 0.0), 40, 36, prec)
+        assert np.array_equal(a.view(np.uint8), b.view(np.uint8))
+
+
+def test_otv_problem_closed_form(mhd):
+    """Amplitude and phases enter as documented (independent numpy form)."""
+    amp, p1, p2, p3, p4 = 0.9, 0.3, 1.1, 2.0, 5.5
+    n = 32
+    u = mhd.problem_state_ex("otv", (amp, p1, p2, p3, p4), n, n)
+    x = (np.arange(n) + 0.5) / n
+    X, Y = np.meshgrid(x, x)
+    rho = 25.0 / (36.0 * np.pi)
+    c = 1.0 / np.sqrt(4.0 * np.pi)
+    assert np.allclose(u[0], rho, rtol=1e-15)
+    assert np.allclose(u[1] / rho, -np.sin(2 * np.pi * Y + p1), atol=1e-14)
+    assert np.allclose(u[2] / rho, np.sin(2 * np.pi * X + p2), atol=1e-14)
+    assert np.allclose(u[5], -amp * np.sin(2 * np.pi * Y + p3) * c, atol=1e-14)
+    assert np.allclose(u[6], amp * np.sin(4 * np.pi * X + p4) * c, atol=1e-14)
+    with pytest.raises(mhd.MhdError):
+        mhd.problem_state_ex("otv", (1.0, 0.0), n, n)
+
+
+def test_autotune_table_lookup(tmp_path):
+    """The library picks the first matching row of the autotuner's table."""
+    import subprocess
+    import sys
+    from mhdglm import autotune
+
+    table = tmp_path / "autotune.txt"
+    autotune.write_table([(0, 256 * 256, 1), (0, 1024 * 1024, 0), (0, 4096 * 4096, 4),
+                          (2, 512 * 512, 5)], str(table), "test")
+    code = ("import sys; sys.path.insert(0, sys.argv[1]); import mhdglm; l = mhdglm.load(); "
+            "print([l.mhd_autotuned_tile(p, n, n) for p, n in "
+            "((0, 128), (0, 256), (0, 512), (0, 1024), (0, 8192), (2, 512), (2, 4096), (1, 64))])")
+    env = dict(os.environ, MHD_AUTOTUNE_FILE=str(table))
+    r = subprocess.run([sys.executable, "-c", code, os.path.join(ROOT, "precision-study_amd")],
+                       env=env, capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
+    assert r.stdout.strip() == "[1, 1, 0, 0, 4, 5, 5, 3]"

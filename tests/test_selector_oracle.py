@@ -1,0 +1,81 @@
+"""CPU tests pinning the selector oracle (oracle/selector_oracle.py).
+
+  * bf16 rounding: known bit patterns, round-to-nearest-even ties;
+  * MLP gradients: central finite differences of the loss (float64 path);
+  * tile features: closed forms on uniform and single-mode states.
+"""
+import numpy as np
+import pytest
+
+import selector_oracle as so
+
+
+def test_bf16_rounding_bits():
+    vals = np.array([1.0, 1.0 + 2.0**-8, 1.0 + 3 * 2.0**-9, 1.0 + 2.0**-9, -2.5, 0.0], dtype=np.float32)
+    got = so.bf16(vals)
+    # 1+2^-8 is exactly representable in bf16 (8 mantissa bits -> spacing 2^-7? no: 7 bits)
+    # bf16 has 7 stored mantissa bits: spacing at 1.0 is 2^-7.
+    exp = np.array([1.0, 1.0, 1.0 + 2.0**-7, 1.0, -2.5, 0.0], dtype=np.float32)
+    assert np.array_equal(got, exp)
+    # ties round to even: 1 + 2^-8 is halfway between 1 and 1+2^-7 -> 1 (even)
+    assert so.bf16(np.float32(1.0 + 2.0**-8)) == np.float32(1.0)
+    assert so.bf16(np.float32(1.0 + 2.0**-7 + 2.0**-8)) == np.float32(1.0 + 2.0**-6)
+    assert np.isnan(so.bf16(np.float32(np.nan)))
+
+
+def test_mlp_gradients_finite_differences():
+    rng = np.random.default_rng(0)
+    p = {k: v.astype(np.float64) for k, v in so.init_params(3).items()}
+    x = rng.standard_normal((40, so.NFEAT))
+    y = (rng.random(40) < 0.3).astype(np.float64)
+    w = np.where(y > 0, 3.0, 1.0)
+
+    def loss(q):
+        return so.mlp_grads(x, y, w, q, bf16_ops=False)[1]
+
+    g, _ = so.mlp_grads(x, y, w, p, bf16_ops=False)
+    eps = 1e-6
+    for name in ("W1", "b1", "W2", "b2", "w3", "b3"):
+        flat = p[name].reshape(-1)
+        idx = rng.choice(flat.size, size=min(12, flat.size), replace=False)
+        for k in idx:
+            if name == "W1" and k // so.H >= so.NFEAT:
+                continue  # padding rows carry zero input
+            q = {kk: vv.copy() for kk, vv in p.items()}
+            q[name].reshape(-1)[k] += eps
+            lp = loss(q)
+            q[name].reshape(-1)[k] -= 2 * eps
+            lm = loss(q)
+            num = (lp - lm) / (2 * eps)
+            ana = np.asarray(g[name]).reshape(-1)[k]
+            assert abs(num - ana) <= 1e-6 + 1e-4 * abs(num), (name, k, num, ana)
+
+
+def test_features_uniform_state():
+    """A uniform state: zero neighbour jumps, beta from the closed form."""
+    n = 32
+    rho, p, vx, bx, by = 1.3, 0.7, 0.4, 0.3, 0.4
+    u = np.zeros((9, n, n))
+    u[0] = rho
+    u[1] = rho * vx
+    u[5], u[6] = bx, by
+    u[4] = p / (so.GAMMA - 1) + 0.5 * rho * vx * vx + 0.5 * (bx * bx + by * by)
+    f = so.tile_features(u, 16, 15, ch=2.0)
+    assert f.shape == (2 * 3, so.NFEAT)
+    beta = 2 * p / (bx * bx + by * by)
+    assert np.allclose(f[:, 0], np.log10(beta)) and np.allclose(f[:, 1], np.log10(beta))
+    assert np.allclose(f[:, 2:5], -12.0)
+    assert np.allclose(f[:, 5], vx / np.sqrt(so.GAMMA * p / rho))
+    assert np.allclose(f[:, 8], np.log10(min(bx, by) / 0.5 + 1e-12))
+    assert np.allclose(f[:, 9], vx / 2.0)
+    assert np.allclose(f[:, 11], np.log10(0.5 * rho * vx * vx / (0.5 * 0.25) + 1e-12))
+
+
+def test_tile_errors_closed_form():
+    b = np.ones((9, 8, 8))
+    a = b.copy()
+    a[4, 2, 3] += 0.5          # one cell of tile 0 (tiles 4x4)
+    e = so.tile_errors(a, b, 4, 4)
+    assert e.shape == (4,)
+    assert e[0] == pytest.approx(0.25)   # 0.5 / (|1| + rms 1)
+    assert np.all(e[1:] == 0.0)
