@@ -17,6 +17,10 @@ reported on the same JSON line:
   accuracy     -- relative L2 against FP64 of rho, p, |B| and psi after whole
                   Orszag-Tang runs to t=0.5 at --accuracy-n (default 1024^2),
                   for each form, beside the FP64 noise floor (FMA vs no FMA);
+  ensemble     -- the batched workload: 64 Orszag-Tang instances (seeded
+                  sizes in {128..1024}^2, B amplitude and sine phases varied)
+                  advanced together to t=0.5 by the persistent work-queue
+                  kernels, each with its own dt; aggregate cell-updates/s;
   roofline     -- the dominant (stage) kernel's algorithmic HBM bytes per
                   launch over its average duration measured here with HIP
                   events on the solver stream, against 8 TB/s;
@@ -130,6 +134,47 @@ def accuracy_check(mhdglm, n, forms):
     return out
 
 
+ENSEMBLE_SIZES = (128, 192, 256, 384, 512, 768, 1024)
+
+
+def ensemble_workload(n, seed=2026):
+    """The batched-ensemble workload: n Orszag-Tang instances with sizes drawn
+    from ENSEMBLE_SIZES, B amplitude uniform in [0.8, 1.2] and the phases of
+    the four sine terms uniform in [0, 2 pi), from a seeded generator."""
+    import numpy as np
+
+    rng = np.random.default_rng(seed)
+    sizes = [int(x) for x in rng.choice(ENSEMBLE_SIZES, size=n)]
+    params = [(float(rng.uniform(0.8, 1.2)),) + tuple(float(x) for x in rng.uniform(0, 2 * np.pi, 4))
+              for _ in range(n)]
+    return sizes, params
+
+
+def run_ensemble(mhdglm, torch, n, form, t_end):
+    sizes, params = ensemble_workload(n)
+    log(f"bench: ensemble of {n} OT instances ({form}), sizes {sorted(set(sizes))}, to t={t_end}")
+    ens = mhdglm.Ensemble(mhdglm.SolverConfig(nx=2, ny=2, precision=mhdglm.PRECISIONS[form]),
+                          [(s, s) for s in sizes])
+    for i, p in enumerate(params):
+        ens.init_problem(i, "otv", p)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    sts = ens.advance(t_end)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    ens.close()
+    codes = [s.code for s in sts]
+    updates = sum(s * s * st.steps for s, st in zip(sizes, sts))
+    return {
+        "instances": n, "form": form, "t_end": t_end, "seed": 2026,
+        "sizes": {str(k): sizes.count(k) for k in sorted(set(sizes))},
+        "wall_s": round(wall, 3), "cell_updates": updates,
+        "value": round(updates / wall, 1), "unit": "cell-updates/s",
+        "all_done": all(st.done == 1 for st in sts), "codes": sorted(set(codes)),
+        "steps_min_max": [min(st.steps for st in sts), max(st.steps for st in sts)],
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -145,6 +190,10 @@ def main():
                     help="other forms timed beside --precision ('' for none)")
     ap.add_argument("--accuracy-n", type=int, default=1024,
                     help="grid of the whole-run accuracy check (0 to skip)")
+    ap.add_argument("--ensemble", type=int, default=64,
+                    help="instances of the batched-ensemble workload (0 to skip)")
+    ap.add_argument("--ensemble-form", default="mixed")
+    ap.add_argument("--ensemble-t-end", type=float, default=0.5)
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="only run a plain device copy of this many MiB (HBM counter calibration)")
     args = ap.parse_args()
@@ -203,6 +252,10 @@ def main():
         accuracy = accuracy_check(mhdglm, args.accuracy_n,
                                   [f for f in ("fp64_fma", "mixed", "mixed_kahan", "fp32")
                                    if f in forms or f == "fp64_fma"])
+    ensemble = None
+    if args.ensemble:
+        ensemble = run_ensemble(mhdglm, torch, args.ensemble, args.ensemble_form,
+                                args.ensemble_t_end)
     cpu = None if args.no_cpu else cpu_baseline(args.cpu_t_end)
     out = {
         "metric": METRIC,
@@ -219,6 +272,7 @@ def main():
                   "mixed": "f32 flux / f64 state+update", "mixed_kahan": "f32 flux / f32x2 state"}[args.precision],
         "forms": forms,
         "accuracy": accuracy,
+        "ensemble": ensemble,
         "data": "synthetic: Orszag-Tang initial condition evaluated on the host",
         "config": {
             "workload": f"Orszag-Tang {n}x{n}, MUSCL+HLLD+GLM SSPRK2, {args.precision}",
