@@ -12,8 +12,9 @@ the reference's "Mcell-steps/s" unit (mhd2d_glm_cpu.cpp:589-591) times 1e6.
 `value` is the plain FP64 path (bitwise the reference CPU build).  Also
 reported on the same JSON line:
   forms        -- throughput of the other arithmetic forms on the same
-                  workload (FP64 with FMA, mixed, mixed + compensated, FP32)
-                  and the mixed-vs-FP64 speedup;
+                  workload (FP64 with FMA, mixed, mixed with the learned
+                  tile selector, mixed + compensated, FP32), the
+                  mixed-vs-FP64 speedup and the selector's FP64 tile share;
   accuracy     -- relative L2 against FP64 of rho, p, |B| and psi after whole
                   Orszag-Tang runs to t=0.5 at --accuracy-n (default 1024^2),
                   for each form, beside the FP64 noise floor (FMA vs no FMA);
@@ -80,12 +81,22 @@ def cpu_baseline(t_end: float):
     }
 
 
+def make_solver(mhdglm, form, n, tile=-1):
+    """OT n^2 solver in an arithmetic form; "selector" is the mixed form with
+    the committed selector model deciding FP64 tiles."""
+    prec = mhdglm.PREC_MIXED if form == "selector" else mhdglm.PRECISIONS[form]
+    solver = mhdglm.Solver(mhdglm.SolverConfig(nx=n, ny=n, precision=prec, tile=tile))
+    solver.init_problem("ot")
+    if form == "selector":
+        from mhdglm import selector
+        solver.configure_selector(selector.load(selector.DEFAULT_MODEL))
+    return solver
+
+
 def time_form(mhdglm, torch, form, n, steps, warmup, tile=-1):
     """Times `steps` SSPRK2 steps of OT n^2 in one arithmetic form."""
     log(f"bench: OT {n}x{n} {form}, warmup {warmup}, steps {steps}")
-    solver = mhdglm.Solver(mhdglm.SolverConfig(nx=n, ny=n, precision=mhdglm.PRECISIONS[form],
-                                               tile=tile))
-    solver.init_problem("ot")
+    solver = make_solver(mhdglm, form, n, tile)
     t_end = 0.5
     solver.enqueue_steps(t_end, warmup)
     torch.cuda.synchronize()
@@ -100,10 +111,18 @@ def time_form(mhdglm, torch, form, n, steps, warmup, tile=-1):
         sys.exit(1)
     wall = t1 - t0
     value = n * n * steps / wall
-    ms = solver.time_kernels(t_end, 20)
+    extra = {}
+    if form == "selector":
+        sel = solver.selector_stats()
+        tiles = solver.selector_tiles()
+        extra = {"fp64_tile_fraction": sel["fp64_tile_steps"] / max(sel["steps"] * tiles, 1),
+                 "audits": sel["audits"], "audit_misses": sel["misses"]}
+        ms = None
+    else:
+        ms = solver.time_kernels(t_end, 20)
     solver.close()
-    log(f"bench: {form}: {steps} steps in {wall:.3f} s -> {value:.4e} cell-updates/s")
-    return {"value": value, "wall": wall, "kernel_ms": ms}
+    log(f"bench: {form}: {steps} steps in {wall:.3f} s -> {value:.4e} cell-updates/s {extra}")
+    return {"value": value, "wall": wall, "kernel_ms": ms, **extra}
 
 
 def accuracy_check(mhdglm, n, forms):
@@ -112,8 +131,7 @@ def accuracy_check(mhdglm, n, forms):
 
     def run(form):
         t0 = time.perf_counter()
-        s = mhdglm.Solver(mhdglm.SolverConfig(nx=n, ny=n, precision=mhdglm.PRECISIONS[form]))
-        s.init_problem("ot")
+        s = make_solver(mhdglm, form, n)
         while True:  # chunks, so that long runs report progress
             st = s.advance(0.5, 2000)
             if st.code != 0 or st.done:
@@ -186,7 +204,7 @@ def main():
     ap.add_argument("--tile", type=int, default=-1)
     ap.add_argument("--cpu-t-end", type=float, default=0.2)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--forms", default="fp64_fma,mixed,mixed_kahan,fp32",
+    ap.add_argument("--forms", default="fp64_fma,mixed,selector,mixed_kahan,fp32",
                     help="other forms timed beside --precision ('' for none)")
     ap.add_argument("--accuracy-n", type=int, default=1024,
                     help="grid of the whole-run accuracy check (0 to skip)")
@@ -241,8 +259,11 @@ def main():
     }
     forms = {args.precision: round(value, 1)}
     for form in [f for f in args.forms.split(",") if f and f != args.precision]:
-        forms[form] = round(time_form(mhdglm, torch, form, n, args.steps, args.warmup,
-                                      args.tile)["value"], 1)
+        r = time_form(mhdglm, torch, form, n, args.steps, args.warmup, args.tile)
+        forms[form] = round(r["value"], 1)
+        if form == "selector":
+            forms["selector_fp64_tile_fraction"] = round(r["fp64_tile_fraction"], 5)
+            forms["selector_audits"] = [r["audits"], r["audit_misses"]]
     if "mixed" in forms and args.precision in ("fp64", "mixed"):
         forms["mixed_vs_fp64_speedup"] = round(forms["mixed"] / forms.get("fp64", value), 3)
     if "mixed" in forms and "fp64_fma" in forms:
@@ -250,7 +271,8 @@ def main():
     accuracy = None
     if args.accuracy_n:
         accuracy = accuracy_check(mhdglm, args.accuracy_n,
-                                  [f for f in ("fp64_fma", "mixed", "mixed_kahan", "fp32")
+                                  [f for f in ("fp64_fma", "mixed", "selector", "mixed_kahan",
+                                               "fp32")
                                    if f in forms or f == "fp64_fma"])
     ensemble = None
     if args.ensemble:

@@ -161,6 +161,43 @@ int mhd_cons_to_prim_batch(int precision, int n, const void* U, void* W);
 int mhd_fast_speed_batch(int precision, int dir, int n, const void* W,
                          void* out);
 
+/* Learned precision selector (MIXED form; no reference counterpart).  Per
+ * stage-kernel tile and step an MLP 12 -> 64 -> 64 -> 1 (BF16 MFMA, FP32
+ * accumulate, sigmoid) scores 12 tile features; tiles scoring above the
+ * threshold are evaluated by the FP64 kernels, the rest by the FP32-flux
+ * kernels.  A fraction of the FP32 tiles is recomputed in FP64 each step
+ * (audit); a scaled difference above the budget makes the tile FP64 for the
+ * rest of the run.  params holds mhd_mlp_param_count() FP32 weights in the
+ * layout W1[16][64], b1[64], W2[64][64], b2[64], w3[64], b3; mean/stdev
+ * standardise the 12 features.  params NULL turns the selector off. */
+int mhd_selector_configure(mhd_ctx* ctx, const float* params, const float* mean,
+                           const float* stdev, float threshold, float audit_frac,
+                           float audit_budget);
+/* out[0..3]: audited tiles, audit misses, FP64-routed tile-steps, steps. */
+int mhd_selector_stats(mhd_ctx* ctx, unsigned long long* out4);
+/* Tiles of the selector's decomposition of this context (-1 if not MIXED). */
+int mhd_selector_tiles(mhd_ctx* ctx);
+/* Training data from the solver itself: the 12 raw features of every tile
+ * (tiles x 16 floats, last 4 zero) of the current state, and every tile's
+ * largest scaled difference between the FP32-flux and the FP64-flux stage-1
+ * update of the current state (max over cells and the 8 MHD variables --
+ * psi, which has no scale of its own, excluded -- of |a-b|/(|b|+rms_k),
+ * rms_k the RMS of variable k of the FP64 update).  Audits use the same
+ * measure with the RMS of the state at mhd_selector_configure.  Neither
+ * call advances the run. */
+int mhd_tile_features(mhd_ctx* ctx, float* feats);
+int mhd_pair_tile_errors(mhd_ctx* ctx, float* err);
+/* The MLP on the device: forward of n standardised samples (n x 16 floats),
+ * gradients of the weighted binary cross-entropy, and Adam training
+ * (epochs over shuffled minibatches; params updated in place). */
+int mhd_mlp_param_count(void);
+int mhd_mlp_forward(const float* params, const float* x, int n, float* prob);
+int mhd_mlp_gradients(const float* params, const float* x, const float* y,
+                      const float* w, int n, float* grad, float* loss);
+int mhd_mlp_train(float* params, const float* x, const float* y, const float* w,
+                  int n, int epochs, int batch, float lr, unsigned seed,
+                  float* epoch_loss);
+
 /* Ensembles (no reference counterpart: the reference runs one grid per
  * process).  n instances of sizes nx[i] x ny[i] share the configuration in
  * cfg (scheme, precision, tile) and advance together: each step is one

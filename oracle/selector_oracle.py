@@ -8,7 +8,8 @@ what the selector's GPU kernels compute, so that tests can check them:
     and the gradients of the weighted binary cross-entropy with respect to
     the FP32 master weights (straight-through the BF16 rounding);
   * tile_errors -- the per-tile label source: the largest scaled
-    difference between an FP32-flux and an FP64-flux stage update.
+    difference between an FP32-flux and an FP64-flux stage update (MHD
+    variables; psi excluded).
 Only tests/ use it.
 """
 from __future__ import annotations
@@ -92,9 +93,10 @@ def tile_features(u, tx, ty, ch):
 
 def tile_errors(u_fp32flux, u_fp64flux, tx, ty):
     """Largest scaled stage-update difference per tile: max over the tile's
-    cells and the 9 variables of |a - b| / (|b| + rms_k(b))."""
-    a = np.asarray(u_fp32flux, dtype=np.float64)
-    b = np.asarray(u_fp64flux, dtype=np.float64)
+    cells and the 8 MHD variables (rho, momentum, E, B; not the GLM
+    potential psi, which has no scale of its own) of |a - b| / (|b| + rms_k(b))."""
+    a = np.asarray(u_fp32flux, dtype=np.float64)[:8]
+    b = np.asarray(u_fp64flux, dtype=np.float64)[:8]
     ny, nx = b.shape[1:]
     rms = np.sqrt(np.mean(b * b, axis=(1, 2)))[:, None, None] + 1e-30
     e = (np.abs(a - b) / (np.abs(b) + rms)).max(axis=0)

@@ -169,21 +169,26 @@ __global__ __launch_bounds__(256) void k_sumsq(const double* u, size_t ps, long 
   }
 }
 
-// Per-tile max over cells and variables of |a-b| / (|b| + rms_k(b)).
+// Per-tile max over cells and the 8 MHD variables of |a-b| / (|b| + rms_k),
+// rms_k = sqrt(sumsq9[k] / cells).  The GLM potential psi (plane 8) is left
+// out: it has no scale of its own (it starts at zero and stays near the
+// round-off level of div B), so a relative measure of it says nothing about
+// the accuracy of the flow.
 __global__ __launch_bounds__(256) void k_tile_errors(const double* a, const double* b, size_t ps,
                                                      int nx, int ny, int tx, int ty, int tiles_x,
                                                      int ntiles, const double* sumsq9,
-                                                     float* err) {
+                                                     const unsigned char* mask, float* err) {
   __shared__ double red[4];
   const int tile = blockIdx.x;
   if (tile >= ntiles) return;
+  if (mask && !mask[tile]) return;
   const int j0 = (tile / tiles_x) * ty, i0 = (tile % tiles_x) * tx;
   const int t = threadIdx.x, i = i0 + t % tx, j = j0 + t / tx;
   double e = 0.0;
   if (t < tx * ty && i < nx && j < ny) {
     const size_t g = (size_t)j * nx + i;
     const double cells = (double)nx * (double)ny;
-    for (int k = 0; k < 9; k++) {
+    for (int k = 0; k < 8; k++) {
       const double rms = sqrt(sumsq9[k] / cells) + 1e-30;
       const double bv = b[k * ps + g];
       e = fmax(e, fabs(a[k * ps + g] - bv) / (fabs(bv) + rms));
@@ -417,16 +422,27 @@ __global__ void k_audit_compare(const float* err, const unsigned char* audit, in
   }
 }
 
-// Audit sample of a step: unflagged tiles whose hash falls under `frac`.
-__global__ void k_audit_pick(const unsigned char* flags, int ntiles, unsigned step, float frac,
-                             unsigned char* audit) {
+// Audit sample of a step (*step read on the device, so a replayed graph
+// samples new tiles): unflagged tiles whose hash falls under `frac`.  Also
+// counts the step's FP64-routed tiles into counters[2].
+__global__ void k_audit_pick(const unsigned char* flags, int ntiles, const int* step, float frac,
+                             unsigned char* audit, unsigned long long* counters) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= ntiles) return;
-  unsigned h = (unsigned)t * 2654435761u ^ (step * 2246822519u + 0x9E3779B9u);
+  const unsigned st = (unsigned)*step;
+  unsigned h = (unsigned)t * 2654435761u ^ (st * 2246822519u + 0x9E3779B9u);
   h ^= h >> 15;
   h *= 2246822519u;
   h ^= h >> 13;
   audit[t] = (!flags[t] && (float)(h & 0xFFFFFF) < frac * 16777216.f) ? 1 : 0;
+  if (flags[t]) atomicAdd(&counters[2], 1ull);
+  if (t == 0) atomicAdd(&counters[3], 1ull);
+}
+
+// Word copy (control block snapshots inside captured steps).
+__global__ void k_copy_words(const unsigned* src, unsigned* dst, int nwords) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < nwords) dst[i] = src[i];
 }
 
 }  // namespace sel
@@ -442,17 +458,21 @@ hipError_t sel_tile_features(const double* u, size_t plane, int nx, int ny, int 
   return hipGetLastError();
 }
 
-hipError_t sel_tile_errors(const double* a, const double* b, size_t plane, int nx, int ny, int tx,
-                           int ty, double* sumsq9, float* err, hipStream_t s) {
-  const int tiles_x = (nx + tx - 1) / tx, ntiles = tiles_x * ((ny + ty - 1) / ty);
-  const long long cells = (long long)nx * ny;
+hipError_t sel_sumsq(const double* u, size_t plane, long long cells, double* sumsq9, hipStream_t s) {
   hipError_t e = hipMemsetAsync(sumsq9, 0, 9 * sizeof(double), s);
   if (e != hipSuccess) return e;
   long long blocks = (cells + 255) / 256;
   if (blocks > 4096) blocks = 4096;
-  hipLaunchKernelGGL(sel::k_sumsq, dim3((unsigned)blocks), dim3(256), 0, s, b, plane, cells, sumsq9);
+  hipLaunchKernelGGL(sel::k_sumsq, dim3((unsigned)blocks), dim3(256), 0, s, u, plane, cells, sumsq9);
+  return hipGetLastError();
+}
+
+hipError_t sel_tile_errors(const double* a, const double* b, size_t plane, int nx, int ny, int tx,
+                           int ty, const double* sumsq9, const unsigned char* mask, float* err,
+                           hipStream_t s) {
+  const int tiles_x = (nx + tx - 1) / tx, ntiles = tiles_x * ((ny + ty - 1) / ty);
   hipLaunchKernelGGL(sel::k_tile_errors, dim3(ntiles), dim3(256), 0, s, a, b, plane, nx, ny, tx, ty,
-                     tiles_x, ntiles, sumsq9, err);
+                     tiles_x, ntiles, sumsq9, mask, err);
   return hipGetLastError();
 }
 
@@ -481,10 +501,17 @@ hipError_t sel_adam(float* params, float* grad, float* m, float* v, int step, fl
   return hipGetLastError();
 }
 
-hipError_t sel_audit_pick(const unsigned char* flags, int ntiles, unsigned step, float frac,
-                          unsigned char* audit, hipStream_t s) {
+hipError_t sel_audit_pick(const unsigned char* flags, int ntiles, const int* step, float frac,
+                          unsigned char* audit, unsigned long long* counters, hipStream_t s) {
   hipLaunchKernelGGL(sel::k_audit_pick, dim3((ntiles + 255) / 256), dim3(256), 0, s, flags, ntiles,
-                     step, frac, audit);
+                     step, frac, audit, counters);
+  return hipGetLastError();
+}
+
+hipError_t sel_copy_words(const void* src, void* dst, size_t bytes, hipStream_t s) {
+  const int nwords = (int)((bytes + 3) / 4);
+  hipLaunchKernelGGL(sel::k_copy_words, dim3((nwords + 255) / 256), dim3(256), 0, s,
+                     (const unsigned*)src, (unsigned*)dst, nwords);
   return hipGetLastError();
 }
 

@@ -13,6 +13,7 @@
 #include "mhd_glm.h"
 #include "mhd_kernels.h"
 #include "mhd_problems.hpp"
+#include "mhd_select.h"
 
 #include <hip/hip_runtime_api.h>
 
@@ -23,6 +24,9 @@
 #include <cstdlib>
 #include <cstring>
 #include <algorithm>
+#include <cstddef>
+#include <numeric>
+#include <random>
 #include <string>
 #include <vector>
 
@@ -43,6 +47,25 @@ struct mhd_ctx {
   hipGraphExec_t graph = nullptr;  // `batch` steps, replayed
   int graph_steps = 0;
   std::string err;
+  // Learned precision selector (MIXED form): device buffers, allocated on
+  // first use.
+  struct Selector {
+    bool on = false;
+    int ntiles = 0, tx = 16, ty = 15;
+    float threshold = 0.5f, audit_frac = 0.01f, audit_budget = 1e-6f;
+    float* params = nullptr;        // kParams FP32 weights
+    float* mean = nullptr;          // NFEAT
+    float* istd = nullptr;          // NFEAT
+    float* feats = nullptr;         // ntiles x NIN
+    float* err = nullptr;           // ntiles
+    unsigned char* flags = nullptr;   // FP64 routing of the step
+    unsigned char* sticky = nullptr;  // tiles forced to FP64 by an audit miss
+    unsigned char* audit = nullptr;   // audited tiles of the step
+    double* sumsq = nullptr;        // error scale (per-plane sum of squares)
+    unsigned long long* counters = nullptr;  // audits, misses, FP64 tile-steps, steps
+    void* u1x = nullptr;            // FP64 scratch stage output
+    void* ctrl_scratch = nullptr;   // control-block snapshot for side evaluations
+  } sel;
 };
 
 namespace {
@@ -194,7 +217,48 @@ int status_of(mhd_ctx* c, mhd_status* st) {
   return tmp.code;
 }
 
+// One step with the learned selector: tile features -> MLP -> FP64 flags;
+// the mixed kernels take the unflagged tiles and the FP64 kernels the
+// flagged ones; a random sample of the unflagged tiles is recomputed by the
+// FP64 kernel into scratch and compared (an audit miss makes the tile FP64
+// from then on).
+hipError_t enqueue_step_selected(mhd_ctx* c) {
+  auto& S = c->sel;
+  const mhd::Launchers* F64 = mhd::launchers_exact(mhd::MODE_FP64);
+  const char* ctrl = (const char*)c->ctrl;
+  hipError_t e = mhd::sel_tile_features((const double*)c->u, c->plane, c->cfg.nx, c->cfg.ny, S.tx,
+                                        S.ty, ctrl + offsetof(Ctrl<double>, ch), S.feats, c->stream);
+  if (e == hipSuccess)
+    e = mhd::sel_mlp_infer(S.feats, S.ntiles, S.params, S.mean, S.istd, S.threshold, S.sticky,
+                           nullptr, S.flags, c->stream);
+  if (e == hipSuccess)
+    e = mhd::sel_audit_pick(S.flags, S.ntiles, (const int*)(ctrl + offsetof(Ctrl<double>, step)),
+                            S.audit_frac, S.audit, S.counters, c->stream);
+  mhd::GridDesc gm = c->grid, g64 = c->grid, ga = c->grid;
+  gm.flags = S.flags;
+  gm.flag_mode = 1;
+  g64.flags = S.flags;
+  g64.flag_mode = 2;
+  ga.flags = S.audit;
+  ga.flag_mode = 2;
+  if (e == hipSuccess) e = c->L->stage(gm, 1, c->u, c->u, c->u1, c->ctrl, c->stream);
+  if (e == hipSuccess) e = F64->stage(g64, 1, c->u, c->u, c->u1, c->ctrl, c->stream);
+  if (e == hipSuccess) e = mhd::sel_copy_words(c->ctrl, S.ctrl_scratch, c->L->ctrl_bytes, c->stream);
+  if (e == hipSuccess) e = F64->stage(ga, 1, c->u, c->u, S.u1x, S.ctrl_scratch, c->stream);
+  if (e == hipSuccess)
+    e = mhd::sel_tile_errors((const double*)c->u1, (const double*)S.u1x, c->plane, c->cfg.nx,
+                             c->cfg.ny, S.tx, S.ty, S.sumsq, S.audit, S.err, c->stream);
+  if (e == hipSuccess)
+    e = mhd::sel_audit_compare(S.err, S.audit, S.ntiles, S.audit_budget, S.sticky, S.counters,
+                               c->stream);
+  if (e == hipSuccess) e = c->L->stage(gm, 2, c->u1, c->u, c->u, c->ctrl, c->stream);
+  if (e == hipSuccess) e = F64->stage(g64, 2, c->u1, c->u, c->u, c->ctrl, c->stream);
+  if (e == hipSuccess) e = c->L->finish(c->ctrl, c->stream);
+  return e;
+}
+
 hipError_t enqueue_step(mhd_ctx* c, hipEvent_t* ev = nullptr) {
+  if (c->sel.on && !ev) return enqueue_step_selected(c);
   return c->L->step(c->grid, c->u, c->u1, c->ctrl, c->stream, ev);
 }
 
@@ -322,6 +386,9 @@ int mhd_create(const mhd_cfg* cfg, mhd_ctx** out) {
   if (c->cfg.hlld_tol <= 0.0) c->cfg.hlld_tol = 1.0e-4;
   if (c->cfg.tile < 0 || c->cfg.tile >= mhd::TILE_COUNT)
     c->cfg.tile = tuned_tile(cfg->precision, (long long)cfg->nx * cfg->ny);
+  // Only the MUSCL + HLLD scheme has every tile shape; the others run the
+  // default tile (the selector's per-tile flags follow this shape).
+  if (!(cfg->muscl && cfg->hlld)) c->cfg.tile = mhd::kDefaultTile;
   // Planes start on 512-byte boundaries.
   const size_t cells = (size_t)cfg->nx * cfg->ny;
   c->plane = (cells + 63) / 64 * 64;
@@ -354,6 +421,11 @@ void mhd_destroy(mhd_ctx* c) {
   if (!c) return;
   if (c->graph) (void)hipGraphExecDestroy(c->graph);
   if (c->stream) (void)hipStreamSynchronize(c->stream);
+  for (void* p : {(void*)c->sel.params, (void*)c->sel.mean, (void*)c->sel.istd,
+                  (void*)c->sel.feats, (void*)c->sel.err, (void*)c->sel.flags,
+                  (void*)c->sel.sticky, (void*)c->sel.audit, (void*)c->sel.sumsq,
+                  (void*)c->sel.counters, c->sel.u1x, c->sel.ctrl_scratch})
+    if (p) (void)hipFree(p);
   if (c->u) (void)hipFree(c->u);
   if (c->u1) (void)hipFree(c->u1);
   if (c->ctrl) (void)hipFree(c->ctrl);
@@ -1054,6 +1126,260 @@ int mhd_ensemble_time_kernels(mhd_ensemble* e, double t_end, int nsteps, double*
   for (auto& x : ev)
     if (x) (void)hipEventDestroy(x);
   return rc;
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Learned precision selector and its MLP.
+// ---------------------------------------------------------------------------
+namespace {
+
+int sel_alloc(mhd_ctx* c) {
+  auto& S = c->sel;
+  if (S.params) return 0;
+  int tx, ty;
+  mhd::tile_dims(c->cfg.tile, tx, ty);
+  S.tx = tx;
+  S.ty = ty;
+  S.ntiles = ((c->cfg.nx + tx - 1) / tx) * ((c->cfg.ny + ty - 1) / ty);
+  const size_t nt = (size_t)S.ntiles;
+  HIP_OR_FAIL(c, hipMalloc((void**)&S.params, mhd::sel::kParams * sizeof(float)));
+  HIP_OR_FAIL(c, hipMalloc((void**)&S.mean, mhd::sel::NFEAT * sizeof(float)));
+  HIP_OR_FAIL(c, hipMalloc((void**)&S.istd, mhd::sel::NFEAT * sizeof(float)));
+  HIP_OR_FAIL(c, hipMalloc((void**)&S.feats, nt * mhd::sel::NIN * sizeof(float)));
+  HIP_OR_FAIL(c, hipMalloc((void**)&S.err, nt * sizeof(float)));
+  HIP_OR_FAIL(c, hipMalloc((void**)&S.flags, nt));
+  HIP_OR_FAIL(c, hipMalloc((void**)&S.sticky, nt));
+  HIP_OR_FAIL(c, hipMalloc((void**)&S.audit, nt));
+  HIP_OR_FAIL(c, hipMalloc((void**)&S.sumsq, 9 * sizeof(double)));
+  HIP_OR_FAIL(c, hipMalloc((void**)&S.counters, 4 * sizeof(unsigned long long)));
+  HIP_OR_FAIL(c, hipMalloc(&S.u1x, 9 * c->plane * sizeof(double)));
+  HIP_OR_FAIL(c, hipMalloc(&S.ctrl_scratch, c->L->ctrl_bytes));
+  HIP_OR_FAIL(c, hipMemset(S.sticky, 0, nt));
+  HIP_OR_FAIL(c, hipMemset(S.counters, 0, 4 * sizeof(unsigned long long)));
+  return 0;
+}
+
+// Copy of the control block, set up for a step from the current state
+// (c_h and dt from the committed signal speed, no time limit), for the
+// selector's stand-alone feature and label evaluations.
+hipError_t prepared_snapshot(mhd_ctx* c) {
+  hipError_t e = mhd::sel_copy_words(c->ctrl, c->sel.ctrl_scratch, c->L->ctrl_bytes, c->stream);
+  if (e == hipSuccess) e = c->L->set_target(c->sel.ctrl_scratch, 1.0e300, c->stream);
+  return e;
+}
+
+}  // namespace
+
+extern "C" {
+
+int mhd_selector_tiles(mhd_ctx* c) {
+  if (!c || c->cfg.precision != MHD_PREC_MIXED) return -1;
+  int tx, ty;
+  mhd::tile_dims(c->cfg.tile, tx, ty);
+  return ((c->cfg.nx + tx - 1) / tx) * ((c->cfg.ny + ty - 1) / ty);
+}
+
+int mhd_tile_features(mhd_ctx* c, float* feats) {
+  if (!c || !feats) return 1;
+  if (c->cfg.precision != MHD_PREC_MIXED) return fail(c, 1, "selector needs the MIXED form");
+  if (!c->have_state) return fail(c, 1, "mhd_tile_features: no state");
+  if (int rc = read_ctrl(c)) return rc;
+  if (status_of(c, nullptr) != 0) return fail(c, 1, "mhd_tile_features: the run has halted");
+  if (int rc = sel_alloc(c)) return rc;
+  auto& S = c->sel;
+  HIP_OR_FAIL(c, prepared_snapshot(c));
+  HIP_OR_FAIL(c, mhd::sel_tile_features((const double*)c->u, c->plane, c->cfg.nx, c->cfg.ny, S.tx,
+                                        S.ty, (const char*)S.ctrl_scratch + offsetof(Ctrl<double>, ch),
+                                        S.feats, c->stream));
+  HIP_OR_FAIL(c, hipMemcpyAsync(feats, S.feats, (size_t)S.ntiles * mhd::sel::NIN * sizeof(float),
+                                hipMemcpyDeviceToHost, c->stream));
+  HIP_OR_FAIL(c, hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+int mhd_pair_tile_errors(mhd_ctx* c, float* err) {
+  if (!c || !err) return 1;
+  if (c->cfg.precision != MHD_PREC_MIXED) return fail(c, 1, "selector needs the MIXED form");
+  if (!c->have_state) return fail(c, 1, "mhd_pair_tile_errors: no state");
+  if (int rc = read_ctrl(c)) return rc;
+  if (status_of(c, nullptr) != 0) return fail(c, 1, "mhd_pair_tile_errors: the run has halted");
+  if (int rc = sel_alloc(c)) return rc;
+  auto& S = c->sel;
+  const mhd::Launchers* F64 = mhd::launchers_exact(mhd::MODE_FP64);
+  mhd::GridDesc g = c->grid;
+  g.flags = nullptr;
+  g.flag_mode = 0;
+  // Both stage-1 updates of the current state, against a prepared snapshot
+  // of the control block (the run's own counters and verdicts are
+  // untouched).  The FP32-flux update goes to the context's stage buffer,
+  // which the next step overwrites.
+  HIP_OR_FAIL(c, prepared_snapshot(c));
+  HIP_OR_FAIL(c, c->L->stage(g, 1, c->u, c->u, c->u1, S.ctrl_scratch, c->stream));
+  HIP_OR_FAIL(c, prepared_snapshot(c));
+  HIP_OR_FAIL(c, F64->stage(g, 1, c->u, c->u, S.u1x, S.ctrl_scratch, c->stream));
+  double* sumsq = S.sumsq;
+  HIP_OR_FAIL(c, mhd::sel_sumsq((const double*)S.u1x, c->plane, (long long)c->cfg.nx * c->cfg.ny,
+                                sumsq, c->stream));
+  HIP_OR_FAIL(c, mhd::sel_tile_errors((const double*)c->u1, (const double*)S.u1x, c->plane,
+                                      c->cfg.nx, c->cfg.ny, S.tx, S.ty, sumsq, nullptr, S.err,
+                                      c->stream));
+  HIP_OR_FAIL(c, hipMemcpyAsync(err, S.err, (size_t)S.ntiles * sizeof(float),
+                                hipMemcpyDeviceToHost, c->stream));
+  HIP_OR_FAIL(c, hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+int mhd_selector_configure(mhd_ctx* c, const float* params, const float* mean, const float* stdev,
+                           float threshold, float audit_frac, float audit_budget) {
+  if (!c) return 1;
+  if (!params) {
+    c->sel.on = false;
+    if (c->graph) {
+      (void)hipGraphExecDestroy(c->graph);
+      c->graph = nullptr;
+    }
+    return 0;
+  }
+  if (c->cfg.precision != MHD_PREC_MIXED) return fail(c, 1, "selector needs the MIXED form");
+  if (!c->have_state) return fail(c, 1, "mhd_selector_configure: set a state first");
+  if (int rc = sel_alloc(c)) return rc;
+  auto& S = c->sel;
+  float istd[mhd::sel::NFEAT], mu[mhd::sel::NFEAT];
+  for (int k = 0; k < mhd::sel::NFEAT; k++) {
+    mu[k] = mean ? mean[k] : 0.f;
+    istd[k] = (stdev && stdev[k] > 0.f) ? 1.f / stdev[k] : 1.f;
+  }
+  HIP_OR_FAIL(c, hipMemcpy(S.params, params, mhd::sel::kParams * sizeof(float), hipMemcpyHostToDevice));
+  HIP_OR_FAIL(c, hipMemcpy(S.mean, mu, sizeof(mu), hipMemcpyHostToDevice));
+  HIP_OR_FAIL(c, hipMemcpy(S.istd, istd, sizeof(istd), hipMemcpyHostToDevice));
+  HIP_OR_FAIL(c, hipMemset(S.sticky, 0, (size_t)S.ntiles));
+  HIP_OR_FAIL(c, hipMemset(S.counters, 0, 4 * sizeof(unsigned long long)));
+  // Error scale of the audits: the current state's per-plane RMS.
+  HIP_OR_FAIL(c, mhd::sel_sumsq((const double*)c->u, c->plane, (long long)c->cfg.nx * c->cfg.ny,
+                                S.sumsq, c->stream));
+  HIP_OR_FAIL(c, hipStreamSynchronize(c->stream));
+  S.threshold = threshold;
+  S.audit_frac = audit_frac;
+  S.audit_budget = audit_budget;
+  S.on = true;
+  if (c->graph) {  // captured steps of the other path
+    (void)hipGraphExecDestroy(c->graph);
+    c->graph = nullptr;
+  }
+  return 0;
+}
+
+int mhd_selector_stats(mhd_ctx* c, unsigned long long* out4) {
+  if (!c || !out4) return 1;
+  if (!c->sel.counters) {
+    for (int k = 0; k < 4; k++) out4[k] = 0;
+    return 0;
+  }
+  HIP_OR_FAIL(c, hipMemcpyAsync(out4, c->sel.counters, 4 * sizeof(unsigned long long),
+                                hipMemcpyDeviceToHost, c->stream));
+  HIP_OR_FAIL(c, hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+int mhd_mlp_param_count(void) { return mhd::sel::kParams; }
+
+int mhd_mlp_forward(const float* params, const float* x, int n, float* prob) {
+  if (!params || n < 0 || (n > 0 && (!x || !prob))) return 1;
+  if (n == 0) return 0;
+  DevBuf p, xb, pb;
+  if (hipMalloc(&p.p, mhd::sel::kParams * sizeof(float)) != hipSuccess ||
+      hipMalloc(&xb.p, (size_t)n * mhd::sel::NIN * sizeof(float)) != hipSuccess ||
+      hipMalloc(&pb.p, (size_t)n * sizeof(float)) != hipSuccess)
+    return 9;
+  if (hipMemcpy(p.p, params, mhd::sel::kParams * sizeof(float), hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(xb.p, x, (size_t)n * mhd::sel::NIN * sizeof(float), hipMemcpyHostToDevice) != hipSuccess)
+    return 9;
+  if (mhd::sel_mlp_infer((const float*)xb.p, n, (const float*)p.p, nullptr, nullptr, 2.f, nullptr,
+                         (float*)pb.p, nullptr, nullptr) != hipSuccess)
+    return 9;
+  if (hipMemcpy(prob, pb.p, (size_t)n * sizeof(float), hipMemcpyDeviceToHost) != hipSuccess) return 9;
+  return 0;
+}
+
+int mhd_mlp_gradients(const float* params, const float* x, const float* y, const float* w, int n,
+                      float* grad, float* loss) {
+  if (!params || !x || !y || !w || !grad || n <= 0) return 1;
+  DevBuf p, xb, yb, wb, ib, gb, lb;
+  const size_t np = mhd::sel::kParams;
+  if (hipMalloc(&p.p, np * 4) != hipSuccess || hipMalloc(&xb.p, (size_t)n * mhd::sel::NIN * 4) != hipSuccess ||
+      hipMalloc(&yb.p, (size_t)n * 4) != hipSuccess || hipMalloc(&wb.p, (size_t)n * 4) != hipSuccess ||
+      hipMalloc(&ib.p, (size_t)n * 4) != hipSuccess || hipMalloc(&gb.p, np * 4) != hipSuccess ||
+      hipMalloc(&lb.p, 4) != hipSuccess)
+    return 9;
+  std::vector<int> idx(n);
+  std::iota(idx.begin(), idx.end(), 0);
+  double ws = 0;
+  for (int s = 0; s < n; s++) ws += w[s];
+  if (hipMemcpy(p.p, params, np * 4, hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(xb.p, x, (size_t)n * mhd::sel::NIN * 4, hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(yb.p, y, (size_t)n * 4, hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(wb.p, w, (size_t)n * 4, hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(ib.p, idx.data(), (size_t)n * 4, hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemset(gb.p, 0, np * 4) != hipSuccess || hipMemset(lb.p, 0, 4) != hipSuccess)
+    return 9;
+  if (mhd::sel_mlp_grad((const float*)xb.p, (const float*)yb.p, (const float*)wb.p, (const int*)ib.p,
+                        0, n, (float)ws, (const float*)p.p, (float*)gb.p, (float*)lb.p,
+                        nullptr) != hipSuccess)
+    return 9;
+  if (hipMemcpy(grad, gb.p, np * 4, hipMemcpyDeviceToHost) != hipSuccess) return 9;
+  if (loss && hipMemcpy(loss, lb.p, 4, hipMemcpyDeviceToHost) != hipSuccess) return 9;
+  return 0;
+}
+
+int mhd_mlp_train(float* params, const float* x, const float* y, const float* w, int n,
+                  int epochs, int batch, float lr, unsigned seed, float* epoch_loss) {
+  if (!params || !x || !y || !w || n <= 0 || epochs <= 0 || batch <= 0) return 1;
+  const size_t np = mhd::sel::kParams;
+  DevBuf p, xb, yb, wb, ib, gb, mb, vb, lb;
+  if (hipMalloc(&p.p, np * 4) != hipSuccess || hipMalloc(&xb.p, (size_t)n * mhd::sel::NIN * 4) != hipSuccess ||
+      hipMalloc(&yb.p, (size_t)n * 4) != hipSuccess || hipMalloc(&wb.p, (size_t)n * 4) != hipSuccess ||
+      hipMalloc(&ib.p, (size_t)n * 4) != hipSuccess || hipMalloc(&gb.p, np * 4) != hipSuccess ||
+      hipMalloc(&mb.p, np * 4) != hipSuccess || hipMalloc(&vb.p, np * 4) != hipSuccess ||
+      hipMalloc(&lb.p, (size_t)epochs * 4) != hipSuccess)
+    return 9;
+  if (hipMemcpy(p.p, params, np * 4, hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(xb.p, x, (size_t)n * mhd::sel::NIN * 4, hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(yb.p, y, (size_t)n * 4, hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(wb.p, w, (size_t)n * 4, hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemset(gb.p, 0, np * 4) != hipSuccess || hipMemset(mb.p, 0, np * 4) != hipSuccess ||
+      hipMemset(vb.p, 0, np * 4) != hipSuccess || hipMemset(lb.p, 0, (size_t)epochs * 4) != hipSuccess)
+    return 9;
+  std::mt19937 rng(seed);
+  std::vector<int> idx(n);
+  std::iota(idx.begin(), idx.end(), 0);
+  int step = 0;
+  for (int ep = 0; ep < epochs; ep++) {
+    std::shuffle(idx.begin(), idx.end(), rng);
+    if (hipMemcpy(ib.p, idx.data(), (size_t)n * 4, hipMemcpyHostToDevice) != hipSuccess) return 9;
+    for (int off = 0; off < n; off += batch) {
+      const int cnt = std::min(batch, n - off);
+      double ws = 0;
+      for (int q = 0; q < cnt; q++) ws += w[idx[off + q]];
+      // The epoch loss is the mean of the minibatch losses (weighted by count).
+      if (mhd::sel_mlp_grad((const float*)xb.p, (const float*)yb.p, (const float*)wb.p,
+                            (const int*)ib.p, off, cnt, (float)ws, (const float*)p.p, (float*)gb.p,
+                            (float*)lb.p + ep, nullptr) != hipSuccess)
+        return 9;
+      if (mhd::sel_adam((float*)p.p, (float*)gb.p, (float*)mb.p, (float*)vb.p, ++step, lr,
+                        nullptr) != hipSuccess)
+        return 9;
+    }
+  }
+  if (hipDeviceSynchronize() != hipSuccess) return 9;
+  if (hipMemcpy(params, p.p, np * 4, hipMemcpyDeviceToHost) != hipSuccess) return 9;
+  if (epoch_loss) {
+    if (hipMemcpy(epoch_loss, lb.p, (size_t)epochs * 4, hipMemcpyDeviceToHost) != hipSuccess) return 9;
+    const int nb = (n + batch - 1) / batch;
+    for (int ep = 0; ep < epochs; ep++) epoch_loss[ep] /= (float)nb;
+  }
+  return 0;
 }
 
 }  // extern "C"

@@ -57,6 +57,11 @@ struct GridDesc {
   double tol;            // HLLD degeneracy tolerance
   double lowbeta_ratio;  // mixed forms: FP64 recovery where E > ratio * e_int
   int tile;              // tile-shape id
+  // Per-tile precision routing (learned selector): with flags non-null a
+  // stage kernel evaluates only the tiles with flags[t] == 0 (flag_mode 1)
+  // or only those with flags[t] != 0 (flag_mode 2).
+  const unsigned char* flags = nullptr;
+  int flag_mode = 0;
 };
 
 // Tile shapes of the stage kernels (cells TX x TY).  The face list of a
@@ -103,6 +108,11 @@ struct Launchers {
   // kernels and after the last (ev[0..3]).
   hipError_t (*step)(const GridDesc&, void* u, void* u1, void* ctrl, hipStream_t,
                      hipEvent_t* ev);
+  // One stage (1 or 2) with explicit buffers: src (state L() is evaluated
+  // on), uold (stage 2: U^n), dst; and the end-of-step kernel.
+  hipError_t (*stage)(const GridDesc&, int stage, const void* src, const void* uold, void* dst,
+                      void* ctrl, hipStream_t);
+  hipError_t (*finish)(void* ctrl, hipStream_t);
   // Ensembles.
   size_t desc_bytes;
   // Fills descriptor i of a host array; returns the instance's tile count.

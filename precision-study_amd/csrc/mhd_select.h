@@ -18,9 +18,15 @@ constexpr int kParams = NIN * H + H + H * H + H + H + 1;  // FP32 master weights
 // feats: ntiles x NIN raw features of the FP64 state (c_h read from *ctrl_ch).
 hipError_t sel_tile_features(const double* u, size_t plane, int nx, int ny, int tx, int ty,
                              const void* ctrl_ch, float* feats, hipStream_t s);
-// err[t]: max over tile t's cells and variables of |a-b| / (|b| + rms_k(b)).
+// Per-plane sums of squares of an FP64 state (the error scale).
+hipError_t sel_sumsq(const double* u, size_t plane, long long cells, double* sumsq9, hipStream_t s);
+// err[t]: max over tile t's cells and the 8 MHD variables (psi excluded) of
+// |a-b| / (|b| + rms_k),
+// rms_k = sqrt(sumsq9[k] / cells); tiles with mask[t] == 0 are skipped
+// (mask may be null).
 hipError_t sel_tile_errors(const double* a, const double* b, size_t plane, int nx, int ny, int tx,
-                           int ty, double* sumsq9, float* err, hipStream_t s);
+                           int ty, const double* sumsq9, const unsigned char* mask, float* err,
+                           hipStream_t s);
 // prob/flags of n samples (raw features standardised with mean/istd when
 // mean is non-null); flags[s] = prob > threshold || sticky[s].
 hipError_t sel_mlp_infer(const float* raw, int n, const float* params, const float* mean,
@@ -33,8 +39,9 @@ hipError_t sel_mlp_grad(const float* X, const float* Y, const float* W, const in
                         hipStream_t s);
 hipError_t sel_adam(float* params, float* grad, float* m, float* v, int step, float lr,
                     hipStream_t s);
-hipError_t sel_audit_pick(const unsigned char* flags, int ntiles, unsigned step, float frac,
-                          unsigned char* audit, hipStream_t s);
+hipError_t sel_audit_pick(const unsigned char* flags, int ntiles, const int* step, float frac,
+                          unsigned char* audit, unsigned long long* counters, hipStream_t s);
+hipError_t sel_copy_words(const void* src, void* dst, size_t bytes, hipStream_t s);
 hipError_t sel_audit_compare(const float* err, const unsigned char* audit, int ntiles, float budget,
                              unsigned char* sticky, unsigned long long* counters, hipStream_t s);
 

@@ -282,6 +282,8 @@ struct StageArgs {
   double alpha;
   float lowbeta_ratio;      // mixed: FP64 recovery where E > ratio * e_internal
   Ctrl<typename Pol::T>* ctrl;
+  const unsigned char* flags;  // per-tile routing (null: every tile)
+  int flag_mode;               // 1: tiles with flag 0; 2: tiles with flag != 0
 };
 
 // Cell state as loaded from HBM, widened to double where the policy stores
@@ -655,7 +657,9 @@ __global__ __launch_bounds__((TX * TY + 63) / 64 * 64) void k_stage(StageArgs<Po
   Ctrl<typename Pol::T>* ctrl = a.ctrl;
   if (ctrl->run != RUN_ACTIVE) return;
   if (STAGE == 2 && ctrl->bad[0] != INT_MAX) return;
-  stage_tile<Pol, MUSCL, HLLD, STAGE, TX, TY>(a, xcd_tile((int)blockIdx.x, a.ntiles), sh);
+  const int tile = xcd_tile((int)blockIdx.x, a.ntiles);
+  if (a.flags && ((a.flags[tile] != 0) != (a.flag_mode == 2))) return;
+  stage_tile<Pol, MUSCL, HLLD, STAGE, TX, TY>(a, tile, sh);
 }
 
 // Signal-speed maximum of a state (initial state / after set_state), in T.
@@ -868,6 +872,8 @@ __global__ __launch_bounds__((TX * TY + 63) / 64 * 64) void k_stage_queue(EnsArg
     a.alpha = e.alpha;
     a.lowbeta_ratio = e.lowbeta_ratio;
     a.ctrl = d.ctrl;
+    a.flags = nullptr;
+    a.flag_mode = 0;
     a.src = STAGE == 1 ? d.u : d.u1;
     a.uold = d.u;
     a.dst = STAGE == 1 ? d.u1 : d.u;
@@ -897,10 +903,8 @@ __global__ void k_set_target_multi(Ctrl<T>* ctrls, int n, double t_end, unsigned
 // ---------------------------------------------------------------------------
 // Host launchers.
 // ---------------------------------------------------------------------------
-template <class Pol, bool MUSCL, bool HLLD, int TX, int TY>
-static hipError_t launch_step_t(const GridDesc& g, void* u, void* u1, void* ctrl,
-                                hipStream_t s, hipEvent_t* ev) {
-  using S = typename Pol::S;
+template <class Pol, int TX, int TY>
+static StageArgs<Pol> stage_args(const GridDesc& g, void* ctrl) {
   using T = typename Pol::T;
   const int tiles_x = (g.nx + TX - 1) / TX, tiles_y = (g.ny + TY - 1) / TY;
   StageArgs<Pol> a;
@@ -913,6 +917,8 @@ static hipError_t launch_step_t(const GridDesc& g, void* u, void* u1, void* ctrl
   a.alpha = g.alpha;
   a.lowbeta_ratio = (float)g.lowbeta_ratio;
   a.ctrl = (Ctrl<T>*)ctrl;
+  a.flags = g.flags;
+  a.flag_mode = g.flag_mode;
   // dx = (1-0)/nx in the control type, as the reference computes it.  When
   // nx and ny are powers of two, dx and dy are exact powers of two and
   // dividing by them equals multiplying by their exact inverses.
@@ -920,6 +926,15 @@ static hipError_t launch_step_t(const GridDesc& g, void* u, void* u1, void* ctrl
   a.pow2 = ((g.nx & (g.nx - 1)) == 0 && (g.ny & (g.ny - 1)) == 0) ? 1 : 0;
   a.inv_dx = (double)((T)1.0 / dx);
   a.inv_dy = (double)((T)1.0 / dy);
+  return a;
+}
+
+template <class Pol, bool MUSCL, bool HLLD, int TX, int TY>
+static hipError_t launch_step_t(const GridDesc& g, void* u, void* u1, void* ctrl,
+                                hipStream_t s, hipEvent_t* ev) {
+  using S = typename Pol::S;
+  using T = typename Pol::T;
+  StageArgs<Pol> a = stage_args<Pol, TX, TY>(g, ctrl);
   a.src = (const S*)u;
   a.uold = (const S*)u;
   a.dst = (S*)u1;
@@ -936,6 +951,48 @@ static hipError_t launch_step_t(const GridDesc& g, void* u, void* u1, void* ctrl
   if (ev) (void)hipEventRecord(ev[2], s);
   hipLaunchKernelGGL(k_finish<T>, dim3(1), dim3(64), 0, s, (Ctrl<T>*)ctrl);
   if (ev) (void)hipEventRecord(ev[3], s);
+  return hipGetLastError();
+}
+
+template <class Pol, bool MUSCL, bool HLLD, int TX, int TY>
+static hipError_t launch_stage_t(const GridDesc& g, int stage, const void* src, const void* uold,
+                                 void* dst, void* ctrl, hipStream_t s) {
+  using S = typename Pol::S;
+  StageArgs<Pol> a = stage_args<Pol, TX, TY>(g, ctrl);
+  a.src = (const S*)src;
+  a.uold = (const S*)uold;
+  a.dst = (S*)dst;
+  constexpr int nthr = (TX * TY + 63) / 64 * 64;
+  if (stage == 1)
+    hipLaunchKernelGGL((k_stage<Pol, MUSCL, HLLD, 1, TX, TY>), dim3(a.ntiles), dim3(nthr), 0, s,
+                       a);
+  else
+    hipLaunchKernelGGL((k_stage<Pol, MUSCL, HLLD, 2, TX, TY>), dim3(a.ntiles), dim3(nthr), 0, s,
+                       a);
+  return hipGetLastError();
+}
+
+template <class Pol>
+static hipError_t launch_stage(const GridDesc& g, int stage, const void* src, const void* uold,
+                               void* dst, void* ctrl, hipStream_t s) {
+  if (!(g.muscl && g.hlld)) {
+    if (g.muscl) return launch_stage_t<Pol, true, false, 16, 15>(g, stage, src, uold, dst, ctrl, s);
+    if (g.hlld) return launch_stage_t<Pol, false, true, 16, 15>(g, stage, src, uold, dst, ctrl, s);
+    return launch_stage_t<Pol, false, false, 16, 15>(g, stage, src, uold, dst, ctrl, s);
+  }
+  switch (g.tile) {
+    case TILE_16x16: return launch_stage_t<Pol, true, true, 16, 16>(g, stage, src, uold, dst, ctrl, s);
+    case TILE_32x8: return launch_stage_t<Pol, true, true, 32, 8>(g, stage, src, uold, dst, ctrl, s);
+    case TILE_16x8: return launch_stage_t<Pol, true, true, 16, 8>(g, stage, src, uold, dst, ctrl, s);
+    case TILE_32x7: return launch_stage_t<Pol, true, true, 32, 7>(g, stage, src, uold, dst, ctrl, s);
+    case TILE_16x7: return launch_stage_t<Pol, true, true, 16, 7>(g, stage, src, uold, dst, ctrl, s);
+    default: return launch_stage_t<Pol, true, true, 16, 15>(g, stage, src, uold, dst, ctrl, s);
+  }
+}
+
+template <class T>
+static hipError_t launch_finish(void* ctrl, hipStream_t s) {
+  hipLaunchKernelGGL(k_finish<T>, dim3(1), dim3(64), 0, s, (Ctrl<T>*)ctrl);
   return hipGetLastError();
 }
 
@@ -1081,6 +1138,8 @@ static Launchers make_launchers(int mode) {
   L.commit = &launch_commit<typename Pol::T>;
   L.set_target = &launch_target<typename Pol::T>;
   L.step = &launch_step<Pol>;
+  L.stage = &launch_stage<Pol>;
+  L.finish = &launch_finish<typename Pol::T>;
   L.desc_bytes = sizeof(InstDesc<Pol>);
   L.ens_fill = &ens_fill<Pol>;
   L.ens_step = &ens_step<Pol>;

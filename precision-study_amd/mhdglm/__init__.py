@@ -47,6 +47,9 @@ EXPORTED = (
     "mhd_ensemble_set_state", "mhd_ensemble_init_problem", "mhd_ensemble_advance",
     "mhd_ensemble_get_status", "mhd_ensemble_get_state", "mhd_ensemble_enqueue_steps",
     "mhd_ensemble_time_kernels", "mhd_autotuned_tile",
+    "mhd_selector_configure", "mhd_selector_stats", "mhd_selector_tiles", "mhd_tile_features",
+    "mhd_pair_tile_errors", "mhd_mlp_param_count", "mhd_mlp_forward", "mhd_mlp_gradients",
+    "mhd_mlp_train",
 )
 
 
@@ -132,6 +135,25 @@ def load(path: str = LIB_PATH) -> ctypes.CDLL:
     lib.mhd_cons_to_prim_batch.restype = I
     lib.mhd_fast_speed_batch.argtypes = [I, I, I, P, P]
     lib.mhd_fast_speed_batch.restype = I
+    F = ctypes.c_float
+    lib.mhd_selector_configure.argtypes = [P, P, P, P, F, F, F]
+    lib.mhd_selector_configure.restype = I
+    lib.mhd_selector_stats.argtypes = [P, P]
+    lib.mhd_selector_stats.restype = I
+    lib.mhd_selector_tiles.argtypes = [P]
+    lib.mhd_selector_tiles.restype = I
+    lib.mhd_tile_features.argtypes = [P, P]
+    lib.mhd_tile_features.restype = I
+    lib.mhd_pair_tile_errors.argtypes = [P, P]
+    lib.mhd_pair_tile_errors.restype = I
+    lib.mhd_mlp_param_count.argtypes = []
+    lib.mhd_mlp_param_count.restype = I
+    lib.mhd_mlp_forward.argtypes = [P, P, I, P]
+    lib.mhd_mlp_forward.restype = I
+    lib.mhd_mlp_gradients.argtypes = [P, P, P, P, I, P, P]
+    lib.mhd_mlp_gradients.restype = I
+    lib.mhd_mlp_train.argtypes = [P, P, P, P, I, I, I, F, ctypes.c_uint, P]
+    lib.mhd_mlp_train.restype = I
     lib.mhd_autotuned_tile.argtypes = [I, I, I]
     lib.mhd_autotuned_tile.restype = I
     lib.mhd_problem_state_ex.argtypes = [ctypes.c_char_p, P, I, I, I, I, P, S]
@@ -265,6 +287,39 @@ class Solver:
         """Restores a checkpoint written by a context of the same shape/form."""
         self._check(self.lib.mhd_checkpoint_load(self.h, os.fsencode(path)))
 
+    # --- learned precision selector (MIXED form) -------------------------
+    def selector_tiles(self) -> int:
+        return self.lib.mhd_selector_tiles(self.h)
+
+    def tile_features(self) -> np.ndarray:
+        out = np.empty((self.selector_tiles(), 16), dtype=np.float32)
+        self._check(self.lib.mhd_tile_features(self.h, _ptr(out)))
+        return out
+
+    def pair_tile_errors(self) -> np.ndarray:
+        out = np.empty(self.selector_tiles(), dtype=np.float32)
+        self._check(self.lib.mhd_pair_tile_errors(self.h, _ptr(out)))
+        return out
+
+    def configure_selector(self, model, threshold=None, audit_frac=0.01, audit_budget=None):
+        """model: dict params/mean/std/threshold/budget (mhdglm.selector); None: off."""
+        if model is None:
+            self._check(self.lib.mhd_selector_configure(self.h, None, None, None, 0.0, 0.0, 0.0))
+            return
+        p = np.ascontiguousarray(model["params"], dtype=np.float32)
+        m = np.ascontiguousarray(model["mean"], dtype=np.float32)
+        sd = np.ascontiguousarray(model["std"], dtype=np.float32)
+        th = float(model["threshold"] if threshold is None else threshold)
+        bud = float(model["budget"] if audit_budget is None else audit_budget)
+        self._check(self.lib.mhd_selector_configure(self.h, _ptr(p), _ptr(m), _ptr(sd), th,
+                                                    float(audit_frac), bud))
+
+    def selector_stats(self) -> dict:
+        out = np.zeros(4, dtype=np.uint64)
+        self._check(self.lib.mhd_selector_stats(self.h, _ptr(out)))
+        return {"audits": int(out[0]), "misses": int(out[1]), "fp64_tile_steps": int(out[2]),
+                "steps": int(out[3])}
+
     def time_kernels(self, t_end: float, nsteps: int):
         """Average ms of (stage-1, stage-2, finish) kernels over nsteps steps."""
         out = (ctypes.c_double * 3)()
@@ -273,6 +328,53 @@ class Solver:
 
     def stream(self) -> int:
         return self.lib.mhd_stream(self.h)
+
+
+def mlp_param_count() -> int:
+    return load().mhd_mlp_param_count()
+
+
+def mlp_forward(params, x) -> np.ndarray:
+    """Device MLP forward: x (n, 16) standardised features -> probabilities."""
+    lib = load()
+    p = np.ascontiguousarray(params, dtype=np.float32)
+    xs = np.ascontiguousarray(x, dtype=np.float32)
+    out = np.empty(xs.shape[0], dtype=np.float32)
+    rc = lib.mhd_mlp_forward(_ptr(p), _ptr(xs), xs.shape[0], _ptr(out))
+    if rc:
+        raise MhdError(rc, "mhd_mlp_forward failed")
+    return out
+
+
+def mlp_gradients(params, x, y, w):
+    """Device gradients of the weighted BCE: (grad[kParams], loss)."""
+    lib = load()
+    p = np.ascontiguousarray(params, dtype=np.float32)
+    xs = np.ascontiguousarray(x, dtype=np.float32)
+    ys = np.ascontiguousarray(y, dtype=np.float32)
+    ws = np.ascontiguousarray(w, dtype=np.float32)
+    g = np.empty(p.size, dtype=np.float32)
+    loss = np.zeros(1, dtype=np.float32)
+    rc = lib.mhd_mlp_gradients(_ptr(p), _ptr(xs), _ptr(ys), _ptr(ws), xs.shape[0], _ptr(g),
+                               _ptr(loss))
+    if rc:
+        raise MhdError(rc, "mhd_mlp_gradients failed")
+    return g, float(loss[0])
+
+
+def mlp_train(params, x, y, w, epochs=20, batch=1024, lr=1e-3, seed=0):
+    """Adam training on the device; returns (params, per-epoch loss)."""
+    lib = load()
+    p = np.array(params, dtype=np.float32, copy=True)
+    xs = np.ascontiguousarray(x, dtype=np.float32)
+    ys = np.ascontiguousarray(y, dtype=np.float32)
+    ws = np.ascontiguousarray(w, dtype=np.float32)
+    losses = np.zeros(epochs, dtype=np.float32)
+    rc = lib.mhd_mlp_train(_ptr(p), _ptr(xs), _ptr(ys), _ptr(ws), xs.shape[0], epochs, batch, lr,
+                           seed, _ptr(losses))
+    if rc:
+        raise MhdError(rc, "mhd_mlp_train failed")
+    return p, losses
 
 
 def _cfg(cfg: "SolverConfig", nx: int = 0, ny: int = 0) -> Cfg:

@@ -1,0 +1,125 @@
+"""Learned precision selector: training data, training, model files.
+
+The solver produces its own training data (mhd_tile_features and
+mhd_pair_tile_errors of a MIXED context): for every tile of a state, 12
+features and the largest scaled difference between the FP32-flux and the
+FP64-flux stage update.  A tile is "unsafe" (label 1) when that difference
+exceeds the budget.  The MLP is trained on the device (mhd_mlp_train); the
+model (weights, feature standardisation, threshold, budget) is kept as an
+.npz file.
+
+    python -m mhdglm.selector --out model.npz [--n 512] [--steps 600] [--every 4]
+"""
+from __future__ import annotations
+
+import argparse
+import os
+import sys
+
+import numpy as np
+
+from . import PREC_MIXED, Solver, SolverConfig, mlp_forward, mlp_param_count, mlp_train
+
+NFEAT = 12
+NIN = 16
+H = 64
+# Model trained by `python -m mhdglm.selector` (OT and KHB, 512^2), shipped
+# with the package.
+DEFAULT_MODEL = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))),
+                             "selector_model.npz")
+
+
+def init_params(seed: int = 0) -> np.ndarray:
+    """He-initialised FP32 weights in the device layout (padding rows zero)."""
+    rng = np.random.default_rng(seed)
+    w1 = rng.standard_normal((NIN, H)) * np.sqrt(2.0 / NFEAT)
+    w1[NFEAT:] = 0.0
+    parts = [w1.ravel(), np.zeros(H), (rng.standard_normal((H, H)) * np.sqrt(2.0 / H)).ravel(),
+             np.zeros(H), rng.standard_normal(H) * np.sqrt(1.0 / H), np.zeros(1)]
+    p = np.concatenate(parts).astype(np.float32)
+    assert p.size == mlp_param_count()
+    return p
+
+
+def collect(problem="ot", params=(), n=512, steps=600, every=4, log=print):
+    """Features and pair errors of every tile, every `every` steps of a MIXED
+    run of `problem` at n x n from t=0."""
+    s = Solver(SolverConfig(nx=n, ny=n, precision=PREC_MIXED))
+    if params:
+        from . import problem_state_ex
+        s.set_state(problem_state_ex(problem, params, n, n))
+    else:
+        s.init_problem(problem)
+    X, E = [], []
+    for k in range(0, steps, every):
+        X.append(s.tile_features())
+        E.append(s.pair_tile_errors())
+        st = s.advance(10.0, every)
+        if st.code != 0:
+            break
+    s.close()
+    X, E = np.concatenate(X), np.concatenate(E)
+    log(f"selector: collected {len(E)} tiles from {problem} {n}^2")
+    return X, E
+
+
+def fit(X, E, budget=None, unsafe_fraction=0.05, epochs=30, batch=2048, lr=2e-3, seed=0,
+        log=print):
+    """Trains the MLP on (features, pair errors).  budget defaults to the
+    (1 - unsafe_fraction) quantile of the errors.  Returns the model dict."""
+    X = np.asarray(X, dtype=np.float32)
+    E = np.asarray(E, dtype=np.float64)
+    if budget is None:
+        budget = float(np.quantile(E, 1.0 - unsafe_fraction))
+    y = (E > budget).astype(np.float32)
+    mean = X[:, :NFEAT].mean(axis=0)
+    std = X[:, :NFEAT].std(axis=0) + 1e-6
+    Xs = np.zeros_like(X)
+    Xs[:, :NFEAT] = (X[:, :NFEAT] - mean) / std
+    pos = max(float(y.sum()), 1.0)
+    w = np.where(y > 0, (len(y) - pos) / pos, 1.0).astype(np.float32)
+    params, losses = mlp_train(init_params(seed), Xs, y, w, epochs=epochs, batch=batch, lr=lr,
+                               seed=seed)
+    prob = mlp_forward(params, Xs)
+    threshold = 0.5
+    pred = prob > threshold
+    tp = float(np.sum(pred & (y > 0)))
+    recall = tp / pos
+    precision = tp / max(float(pred.sum()), 1.0)
+    log(f"selector: budget {budget:.3e}, unsafe {pos / len(y):.3%}, loss {losses[0]:.4f} -> "
+        f"{losses[-1]:.4f}, recall {recall:.3f}, precision {precision:.3f}, "
+        f"flagged {pred.mean():.3%}")
+    return {"params": params, "mean": mean.astype(np.float32), "std": std.astype(np.float32),
+            "threshold": np.float32(threshold), "budget": np.float32(budget),
+            "losses": losses, "recall": recall, "precision": precision,
+            "flagged": float(pred.mean())}
+
+
+def save(model, path):
+    np.savez(path, **{k: np.asarray(v) for k, v in model.items()})
+
+
+def load(path):
+    with np.load(path, allow_pickle=False) as z:
+        return {k: z[k] for k in z.files}
+
+
+def main(argv=None) -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--out", required=True)
+    ap.add_argument("--n", type=int, default=512)
+    ap.add_argument("--steps", type=int, default=600)
+    ap.add_argument("--every", type=int, default=4)
+    ap.add_argument("--epochs", type=int, default=30)
+    args = ap.parse_args(argv)
+    log = lambda m: print(m, file=sys.stderr, flush=True)
+    Xo, Eo = collect("ot", n=args.n, steps=args.steps, every=args.every, log=log)
+    Xk, Ek = collect("khb", n=args.n, steps=args.steps, every=args.every, log=log)
+    model = fit(np.concatenate([Xo, Xk]), np.concatenate([Eo, Ek]), epochs=args.epochs, log=log)
+    save(model, args.out)
+    log(f"selector: model written to {args.out}")
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -1,0 +1,265 @@
+"""GPU tests of the learned precision selector (MIXED form).
+
+  * MLP forward on MFMA (BF16 operands, FP32 accumulation) against the CPU
+    oracle's BF16 network (oracle/selector_oracle.py), ragged batch sizes;
+  * MLP gradients (device backward, FP32 atomics) against the oracle's
+    float64 gradients of the same BF16 forward; device training lowers the
+    loss on a separable problem;
+  * tile features and FP32-vs-FP64 tile errors against the oracle;
+  * routing: every tile FP64 is bitwise the FP64 form, no tile FP64 is
+    bitwise the mixed form, a partial selection takes each tile's stage-1
+    update from the kernel the flags name, audits leave the state alone and
+    a zero budget makes audited tiles sticky;
+  * gain: the trained network, on a held-out Orszag-Tang variant, catches
+    far more of the tiles whose FP32 flux error exceeds the budget than a
+    random choice of the same number of FP64 tiles.
+"""
+import sys
+
+import numpy as np
+import pytest
+
+import selector_oracle as so
+
+pytestmark = pytest.mark.gpu
+
+
+def say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+def flat(p):
+    return np.concatenate([np.asarray(p[k], dtype=np.float32).ravel()
+                           for k in ("W1", "b1", "W2", "b2", "w3", "b3")])
+
+
+def unflat(g):
+    H, NIN = so.H, so.NIN
+    o, out = 0, {}
+    for k, n in (("W1", NIN * H), ("b1", H), ("W2", H * H), ("b2", H), ("w3", H), ("b3", 1)):
+        out[k] = np.asarray(g[o:o + n], dtype=np.float64)
+        o += n
+    out["W1"] = out["W1"].reshape(NIN, H)
+    out["W2"] = out["W2"].reshape(H, H)
+    return out
+
+
+def padded(x):
+    xp = np.zeros((x.shape[0], so.NIN), dtype=np.float32)
+    xp[:, :so.NFEAT] = x
+    return xp
+
+
+@pytest.mark.parametrize("n", [1, 17, 200, 1000])
+def test_mlp_forward_matches_oracle(mhd, n):
+    say(f"mlp forward n={n}")
+    rng = np.random.default_rng(n)
+    p = so.init_params(7)
+    p["b1"] = (0.1 * rng.standard_normal(so.H)).astype(np.float32)
+    p["b2"] = (0.1 * rng.standard_normal(so.H)).astype(np.float32)
+    p["b3"] = np.array([0.2], dtype=np.float32)
+    x = rng.standard_normal((n, so.NFEAT)).astype(np.float32)
+    got = mhd.mlp_forward(flat(p), padded(x))
+    ref = so.mlp_forward(x, p, bf16_ops=True)["p"]
+    f64 = so.mlp_forward(x, {k: v.astype(np.float64) for k, v in p.items()}, bf16_ops=False)["p"]
+    d = np.abs(got - ref).max()
+    say(f"  max |prob - oracle| {d:.2e}; vs unrounded network {np.abs(got - f64).max():.2e}")
+    assert d <= 1e-4
+
+
+def test_mlp_gradients_match_oracle(mhd):
+    say("mlp gradients n=300")
+    rng = np.random.default_rng(11)
+    p = so.init_params(5)
+    p["b1"] = (0.1 * rng.standard_normal(so.H)).astype(np.float32)
+    x = rng.standard_normal((300, so.NFEAT)).astype(np.float32)
+    y = (rng.random(300) < 0.3).astype(np.float32)
+    w = np.where(y > 0, 2.5, 1.0).astype(np.float32)
+    g, loss = mhd.mlp_gradients(flat(p), padded(x), y, w)
+    gref, lref = so.mlp_grads(x, y.astype(np.float64), w.astype(np.float64), p, bf16_ops=True)
+    assert loss == pytest.approx(lref, rel=1e-4)
+    gd = unflat(g)
+    for k in ("W1", "b1", "W2", "b2", "w3", "b3"):
+        ref = np.asarray(gref[k], dtype=np.float64).reshape(gd[k].shape)
+        scale = np.abs(ref).max()
+        err = np.abs(gd[k] - ref).max()
+        say(f"  {k}: max |g| {scale:.3e}, max err {err:.3e}")
+        assert scale > 0
+        assert err <= 2e-3 * scale + 1e-7, k
+    assert np.all(gd["W1"][so.NFEAT:] == 0.0)  # padding rows see zero input
+
+
+def test_mlp_training_reduces_loss(mhd):
+    from mhdglm import selector
+
+    rng = np.random.default_rng(3)
+    n = 6000
+    x = rng.standard_normal((n, so.NFEAT)).astype(np.float32)
+    y = ((x[:, 0] + 0.5 * x[:, 3] - 0.3 * x[:, 7]) > 0.7).astype(np.float32)
+    w = np.ones(n, dtype=np.float32)
+    p, losses = mhd.mlp_train(selector.init_params(0), padded(x), y, w, epochs=25, batch=256,
+                              lr=3e-3, seed=1)
+    acc = np.mean((mhd.mlp_forward(p, padded(x)) > 0.5) == (y > 0))
+    say(f"training: loss {losses[0]:.4f} -> {losses[-1]:.4f}, accuracy {acc:.4f}")
+    assert losses[-1] < 0.3 * losses[0]
+    assert acc > 0.97
+    assert np.all(p[so.NFEAT * so.H:so.NIN * so.H] == 0.0)
+
+
+def mixed_solver(mhd, nx, ny, problem="ot", steps=0, **kw):
+    s = mhd.Solver(mhd.SolverConfig(nx=nx, ny=ny, precision=mhd.PREC_MIXED, **kw))
+    s.init_problem(problem)
+    if steps:
+        s.advance(10.0, steps)
+    return s
+
+
+def test_tile_features_match_oracle(mhd):
+    say("tile features OT 96x80 after 40 steps")
+    s = mixed_solver(mhd, 96, 80, steps=40)
+    f = s.tile_features()
+    ch = s.status().ch_last
+    u = s.get_state(0)
+    s.close()
+    ref = so.tile_features(u, 16, 15, ch)
+    assert f.shape == (6 * 6, so.NIN)
+    assert np.all(f[:, so.NFEAT:] == 0.0)
+    d = np.abs(f[:, :so.NFEAT] - ref) / (1.0 + np.abs(ref))
+    say(f"  max scaled feature difference {d.max():.2e}")
+    assert d.max() <= 1e-5
+
+
+def test_pair_tile_errors_match_oracle(mhd):
+    """FP32-flux vs FP64-flux stage-1 difference per tile, against the oracle
+    applied to the stage-1 states of a FP64 and a mixed context."""
+    say("pair tile errors OT 96x80")
+    s = mixed_solver(mhd, 96, 80)
+    err = s.pair_tile_errors()
+    s.advance(10.0, 1)
+    u1m = s.get_state(1)
+    s.close()
+    s64 = mhd.Solver(mhd.SolverConfig(nx=96, ny=80, precision=mhd.PREC_FP64))
+    s64.init_problem("ot")
+    s64.advance(10.0, 1)
+    u164 = s64.get_state(1)
+    s64.close()
+    ref = so.tile_errors(u1m, u164, 16, 15)
+    say(f"  errors {err.min():.2e} .. {err.max():.2e}")
+    assert err.max() > 0
+    assert np.allclose(err, ref, rtol=1e-6, atol=0)
+
+
+def run_steps(mhd, precision, nx, ny, steps, model=None, problem="ot", which=0, **sel):
+    s = mhd.Solver(mhd.SolverConfig(nx=nx, ny=ny, precision=precision))
+    s.init_problem(problem)
+    if model is not None:
+        s.configure_selector(model, **sel)
+    st = s.advance(10.0, steps)
+    assert st.code == 0
+    u = s.get_state(which)
+    stats = s.selector_stats() if model is not None else None
+    s.close()
+    return u, stats
+
+
+def random_model(seed=0):
+    from mhdglm import selector
+
+    return {"params": selector.init_params(seed), "mean": np.zeros(so.NFEAT, np.float32),
+            "std": np.ones(so.NFEAT, np.float32), "threshold": 0.5, "budget": 1e30}
+
+
+@pytest.mark.parametrize("steps", [1, 40])
+def test_selector_all_fp64_is_fp64(mhd, steps):
+    say(f"selector threshold -1 (all FP64), {steps} steps")
+    u64, _ = run_steps(mhd, mhd.PREC_FP64, 96, 80, steps)
+    us, st = run_steps(mhd, mhd.PREC_MIXED, 96, 80, steps, random_model(), threshold=-1.0,
+                       audit_frac=0.0)
+    assert np.array_equal(u64.view(np.uint64), us.view(np.uint64))
+    assert st["fp64_tile_steps"] == 36 * steps and st["steps"] == steps and st["audits"] == 0
+
+
+@pytest.mark.parametrize("audit_frac", [0.0, 0.3])
+def test_selector_none_flagged_is_mixed(mhd, audit_frac):
+    say(f"selector threshold 2 (no FP64 tile), audit fraction {audit_frac}")
+    um, _ = run_steps(mhd, mhd.PREC_MIXED, 96, 80, 40)
+    us, st = run_steps(mhd, mhd.PREC_MIXED, 96, 80, 40, random_model(), threshold=2.0,
+                       audit_frac=audit_frac, audit_budget=1e30)
+    assert np.array_equal(um.view(np.uint64), us.view(np.uint64))
+    assert st["fp64_tile_steps"] == 0 and st["misses"] == 0
+    if audit_frac > 0:
+        say(f"  audits {st['audits']} of {36 * 40} tile-steps")
+        assert 0.15 * 36 * 40 < st["audits"] < 0.45 * 36 * 40
+    else:
+        assert st["audits"] == 0
+
+
+def test_selector_partial_routing(mhd):
+    """Stage 1 of one step: each tile's update comes from the FP64 kernel if
+    flagged, else from the mixed kernel (flags predicted on the host from
+    the same features and network)."""
+    nx, ny = 96, 80
+    model = random_model(4)
+    model["mean"] = np.zeros(so.NFEAT, np.float32)
+    s = mixed_solver(mhd, nx, ny)
+    feats = s.tile_features()
+    s.close()
+    prob = mhd.mlp_forward(model["params"], feats)
+    thr = float(np.median(prob))
+    flags = prob > thr
+    say(f"partial routing: {flags.sum()} of {flags.size} tiles FP64 (threshold {thr:.6f})")
+    assert 0 < flags.sum() < flags.size
+    u64, _ = run_steps(mhd, mhd.PREC_FP64, nx, ny, 1, which=1)
+    um, _ = run_steps(mhd, mhd.PREC_MIXED, nx, ny, 1, which=1)
+    us, st = run_steps(mhd, mhd.PREC_MIXED, nx, ny, 1, model, which=1, threshold=thr,
+                       audit_frac=0.0)
+    assert st["fp64_tile_steps"] == flags.sum()
+    assert not np.array_equal(u64, um)
+    for t in range(flags.size):
+        j0, i0 = (t // 6) * 15, (t % 6) * 16
+        sl = (slice(None), slice(j0, j0 + 15), slice(i0, i0 + 16))
+        want = u64[sl] if flags[t] else um[sl]
+        assert np.array_equal(us[sl].view(np.uint64), want.view(np.uint64)), t
+
+
+def test_selector_audit_misses_make_tiles_sticky(mhd):
+    say("audits with zero budget")
+    u64, _ = run_steps(mhd, mhd.PREC_FP64, 96, 80, 30)
+    us, st = run_steps(mhd, mhd.PREC_MIXED, 96, 80, 30, random_model(), threshold=2.0,
+                       audit_frac=0.2, audit_budget=0.0)
+    say(f"  stats {st}")
+    assert st["audits"] > 0 and st["misses"] > 0
+    assert st["fp64_tile_steps"] > 0
+    # once every tile has missed, the run is the FP64 one from then on; here
+    # only part of the tiles are sticky, so the state lies between the two.
+    assert not np.array_equal(us, u64)
+
+
+def test_selector_gain_over_random(mhd):
+    """Train on OT and KHB tiles, evaluate on a held-out OT variant: the
+    fraction of over-budget tiles the network sends to FP64 (recall) against
+    the fraction a random choice of the same number of tiles catches (equal
+    to the flagged fraction in expectation)."""
+    from mhdglm import selector
+
+    n = 192
+    Xo, Eo = selector.collect("ot", n=n, steps=240, every=3, log=say)
+    Xk, Ek = selector.collect("khb", n=n, steps=240, every=3, log=say)
+    model = selector.fit(np.concatenate([Xo, Xk]), np.concatenate([Eo, Ek]), epochs=30, log=say)
+    Xv, Ev = selector.collect("otv", params=(0.85, 0.5, 1.2, 0.3, 2.0), n=n, steps=240, every=4,
+                              log=say)
+    Xs = np.zeros_like(Xv)
+    Xs[:, :so.NFEAT] = (Xv[:, :so.NFEAT] - model["mean"]) / model["std"]
+    prob = mhd.mlp_forward(model["params"], Xs)
+    unsafe = Ev > model["budget"]
+    flagged = prob > model["threshold"]
+    frac = flagged.mean()
+    recall = (flagged & unsafe).sum() / max(unsafe.sum(), 1)
+    rng = np.random.default_rng(0)
+    rand = np.array([(rng.random(len(Ev)) < frac)[unsafe].mean() for _ in range(20)])
+    say(f"held-out: unsafe {unsafe.mean():.3%}, flagged {frac:.3%}, recall {recall:.3f}, "
+        f"random recall {rand.mean():.3f} +- {rand.std():.3f}")
+    assert unsafe.sum() > 50
+    assert recall >= 0.6
+    assert recall >= 3.0 * rand.max()

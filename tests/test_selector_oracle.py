@@ -79,3 +79,6 @@ def test_tile_errors_closed_form():
     assert e.shape == (4,)
     assert e[0] == pytest.approx(0.25)   # 0.5 / (|1| + rms 1)
     assert np.all(e[1:] == 0.0)
+    # the GLM potential does not enter
+    a[8, 5, 5] += 3.0
+    assert np.array_equal(so.tile_errors(a, b, 4, 4), e)
