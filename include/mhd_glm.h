@@ -62,6 +62,12 @@ typedef struct mhd_cfg {
   int batch_steps;   /* steps launched per host check; <= 0 = default */
   double lowbeta_ratio; /* MIXED*: FP64 recovery where E > ratio * e_int;
                            <= 0 = default (16) */
+  int fallback;      /* MIXED: a stage tile whose FP32-flux update fails the
+                        positivity check is re-evaluated by the FP64 kernel
+                        (only an FP64 failure halts the run).  0 = on
+                        (default), 1 = off (the failure halts, as in the
+                        other forms), 2 = every tile (check mode: the step
+                        is then the FP64 step) */
 } mhd_cfg;
 
 /* Progress and diagnostics of a context. */
@@ -76,6 +82,8 @@ typedef struct mhd_status {
   unsigned long long degenerate, nonphysical;      /* HLLD->HLL fallbacks */
   int bad_i, bad_j;         /* first unusable cell on halt 3/4, else -1 */
   int fail_step;            /* step index of the failure */
+  unsigned long long fallback_tiles; /* MIXED: stage tiles re-evaluated in
+                                        FP64 by the fallback, whole run */
 } mhd_status;
 
 typedef struct mhd_ctx mhd_ctx;

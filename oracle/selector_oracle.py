@@ -94,12 +94,16 @@ def tile_features(u, tx, ty, ch):
 def tile_errors(u_fp32flux, u_fp64flux, tx, ty):
     """Largest scaled stage-update difference per tile: max over the tile's
     cells and the 8 MHD variables (rho, momentum, E, B; not the GLM
-    potential psi, which has no scale of its own) of |a - b| / (|b| + rms_k(b))."""
+    potential psi, which has no scale of its own) of |a - b| / (|b| + s_k),
+    s_k the grid RMS (of b) of rho, |momentum|, E or |B| for variable k."""
     a = np.asarray(u_fp32flux, dtype=np.float64)[:8]
     b = np.asarray(u_fp64flux, dtype=np.float64)[:8]
     ny, nx = b.shape[1:]
-    rms = np.sqrt(np.mean(b * b, axis=(1, 2)))[:, None, None] + 1e-30
-    e = (np.abs(a - b) / (np.abs(b) + rms)).max(axis=0)
+    ms = np.mean(b * b, axis=(1, 2))
+    group = [ms[0], ms[1:4].sum(), ms[1:4].sum(), ms[1:4].sum(), ms[4],
+             ms[5:8].sum(), ms[5:8].sum(), ms[5:8].sum()]
+    scale = (np.sqrt(np.array(group)) + 1e-30)[:, None, None]
+    e = (np.abs(a - b) / (np.abs(b) + scale)).max(axis=0)
     tiles_x, tiles_y = -(-nx // tx), -(-ny // ty)
     out = np.empty(tiles_x * tiles_y)
     for t in range(tiles_x * tiles_y):

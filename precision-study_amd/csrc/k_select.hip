@@ -169,11 +169,13 @@ __global__ __launch_bounds__(256) void k_sumsq(const double* u, size_t ps, long 
   }
 }
 
-// Per-tile max over cells and the 8 MHD variables of |a-b| / (|b| + rms_k),
-// rms_k = sqrt(sumsq9[k] / cells).  The GLM potential psi (plane 8) is left
-// out: it has no scale of its own (it starts at zero and stays near the
-// round-off level of div B), so a relative measure of it says nothing about
-// the accuracy of the flow.
+// Per-tile max over cells and the 8 MHD variables of |a-b| / (|b| + s_k).
+// The scale s_k is the RMS over the grid of the variable's physical
+// quantity: rho, |momentum| (k = 1..3), E, |B| (k = 5..7), so that a vector
+// component that is small everywhere (v_y and B_y of a shear layer start at
+// zero) is measured against its vector.  The GLM potential psi (plane 8) is
+// left out: it has no scale of its own (it starts at zero and stays near the
+// round-off level of div B).
 __global__ __launch_bounds__(256) void k_tile_errors(const double* a, const double* b, size_t ps,
                                                      int nx, int ny, int tx, int ty, int tiles_x,
                                                      int ntiles, const double* sumsq9,
@@ -188,8 +190,12 @@ __global__ __launch_bounds__(256) void k_tile_errors(const double* a, const doub
   if (t < tx * ty && i < nx && j < ny) {
     const size_t g = (size_t)j * nx + i;
     const double cells = (double)nx * (double)ny;
+    const double s_rho = sqrt(sumsq9[0] / cells) + 1e-30;
+    const double s_m = sqrt((sumsq9[1] + sumsq9[2] + sumsq9[3]) / cells) + 1e-30;
+    const double s_e = sqrt(sumsq9[4] / cells) + 1e-30;
+    const double s_b = sqrt((sumsq9[5] + sumsq9[6] + sumsq9[7]) / cells) + 1e-30;
     for (int k = 0; k < 8; k++) {
-      const double rms = sqrt(sumsq9[k] / cells) + 1e-30;
+      const double rms = k == 0 ? s_rho : (k < 4 ? s_m : (k == 4 ? s_e : s_b));
       const double bv = b[k * ps + g];
       e = fmax(e, fabs(a[k * ps + g] - bv) / (fabs(bv) + rms));
     }

@@ -42,6 +42,7 @@ struct mhd_ctx {
   void* u1 = nullptr;      // device state U1  (9 planes)
   void* ctrl = nullptr;    // device Ctrl<R>
   void* ctrl_host = nullptr;  // pinned host copy of the control block
+  int* redo_lists = nullptr;  // MIXED FP64 fallback: 2 x ntiles tile indices
   size_t plane = 0;
   bool have_state = false;
   hipGraphExec_t graph = nullptr;  // `batch` steps, replayed
@@ -141,6 +142,7 @@ void fill_status_of(int hlld, int nx, int ny, const Ctrl<R>* h, mhd_status* st) 
     st->bad_j = h->fail_cell / nx;
   }
   st->fail_step = h->fail_step;
+  st->fallback_tiles = h->redo_tiles;
 }
 
 template <class R>
@@ -413,6 +415,21 @@ int mhd_create(const mhd_cfg* cfg, mhd_ctx** out) {
     mhd_destroy(c);
     return 9;
   }
+  // FP64 fallback of the mixed form: one queue of tile indices per stage.
+  if (cfg->precision == MHD_PREC_MIXED && cfg->fallback != 1) {
+    int tx, ty;
+    mhd::tile_dims(c->cfg.tile, tx, ty);
+    const size_t ntiles = (size_t)((cfg->nx + tx - 1) / tx) * ((cfg->ny + ty - 1) / ty);
+    if ((e = hipMalloc((void**)&c->redo_lists, 2 * ntiles * sizeof(int))) != hipSuccess) {
+      std::fprintf(stderr, "mhd_create: %s\n", hipGetErrorString(e));
+      mhd_destroy(c);
+      return 9;
+    }
+    c->grid.redo_list[0] = c->redo_lists;
+    c->grid.redo_list[1] = c->redo_lists + ntiles;
+    c->grid.redo = mhd::launchers_exact(mhd::MODE_FP64)->redo;
+    c->grid.redo_all = cfg->fallback == 2 ? 1 : 0;
+  }
   *out = c;
   return 0;
 }
@@ -428,6 +445,7 @@ void mhd_destroy(mhd_ctx* c) {
     if (p) (void)hipFree(p);
   if (c->u) (void)hipFree(c->u);
   if (c->u1) (void)hipFree(c->u1);
+  if (c->redo_lists) (void)hipFree(c->redo_lists);
   if (c->ctrl) (void)hipFree(c->ctrl);
   if (c->ctrl_host) (void)hipHostFree(c->ctrl_host);
   if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -616,7 +634,7 @@ int mhd_checkpoint_save(mhd_ctx* c, const char* path) {
   CkHeader h;
   std::memset(&h, 0, sizeof(h));
   std::memcpy(h.magic, kCkMagic, 8);
-  h.version = 1;
+  h.version = 2;
   h.nx = c->cfg.nx;
   h.ny = c->cfg.ny;
   h.precision = c->cfg.precision;
@@ -640,7 +658,7 @@ int mhd_checkpoint_load(mhd_ctx* c, const char* path) {
   const size_t cells = (size_t)c->cfg.nx * c->cfg.ny;
   const size_t es = c->L->state_elem;
   bool ok = std::fread(&h, sizeof(h), 1, f) == 1 && std::memcmp(h.magic, kCkMagic, 8) == 0 &&
-            h.version == 1 && h.nx == c->cfg.nx && h.ny == c->cfg.ny &&
+            h.version == 2 && h.nx == c->cfg.nx && h.ny == c->cfg.ny &&
             h.precision == c->cfg.precision && h.planes == c->L->state_planes &&
             h.elem == (int)es && h.ctrl_bytes == (int)c->L->ctrl_bytes;
   std::vector<char> buf;
@@ -798,6 +816,7 @@ struct mhd_ensemble {
   void* desc = nullptr;        // device descriptors
   int* prefix = nullptr;       // device prefix of tile counts
   unsigned* queue = nullptr;   // device work-queue counters
+  int* redo_items = nullptr;   // MIXED FP64 fallback: 2 x total work items
   mhd::EnsLaunch el;
   hipStream_t stream = nullptr;
   hipGraphExec_t graph = nullptr;
@@ -900,6 +919,7 @@ void mhd_ensemble_destroy(mhd_ensemble* e) {
   if (e->desc) (void)hipFree(e->desc);
   if (e->prefix) (void)hipFree(e->prefix);
   if (e->queue) (void)hipFree(e->queue);
+  if (e->redo_items) (void)hipFree(e->redo_items);
   if (e->stream) (void)hipStreamDestroy(e->stream);
   delete e;
 }
@@ -925,6 +945,7 @@ int mhd_ensemble_create(const mhd_cfg* cfg, int n, const int* nx, const int* ny,
     for (int i = 0; i < n; i++) biggest = std::max(biggest, (long long)nx[i] * ny[i]);
     e->cfg.tile = tuned_tile(cfg->precision, biggest);
   }
+  if (!(cfg->muscl && cfg->hlld)) e->cfg.tile = mhd::kDefaultTile;  // as single contexts
   e->nx.assign(nx, nx + n);
   e->ny.assign(ny, ny + n);
   e->u.assign(n, nullptr);
@@ -948,7 +969,8 @@ int mhd_ensemble_create(const mhd_cfg* cfg, int n, const int* nx, const int* ny,
   if (rc == 0) hipok(hipHostMalloc(&e->ctrls_host, cb, hipHostMallocDefault), "hipHostMalloc");
   if (rc == 0) hipok(hipMalloc(&e->desc, (size_t)n * e->L->desc_bytes), "hipMalloc");
   if (rc == 0) hipok(hipMalloc((void**)&e->prefix, (size_t)(n + 1) * sizeof(int)), "hipMalloc");
-  if (rc == 0) hipok(hipMalloc((void**)&e->queue, 16 * sizeof(unsigned)), "hipMalloc");
+  if (rc == 0)
+    hipok(hipMalloc((void**)&e->queue, mhd::kEnsQueueWords * sizeof(unsigned)), "hipMalloc");
   if (rc == 0) {
     std::vector<char> hdesc((size_t)n * e->L->desc_bytes);
     std::vector<int> prefix(n + 1, 0);
@@ -969,7 +991,7 @@ int mhd_ensemble_create(const mhd_cfg* cfg, int n, const int* nx, const int* ny,
       if (e->L->time_is_float) init_ctrl((Ctrl<float>*)ctrl_at(e->ctrls_host, e, i), ci);
       else init_ctrl((Ctrl<double>*)ctrl_at(e->ctrls_host, e, i), ci);
     }
-    const unsigned zeros[16] = {0};
+    const unsigned zeros[mhd::kEnsQueueWords] = {0};
     if (rc == 0) hipok(hipMemcpy(e->desc, hdesc.data(), hdesc.size(), hipMemcpyHostToDevice), "desc");
     if (rc == 0) hipok(hipMemcpy(e->prefix, prefix.data(), prefix.size() * sizeof(int),
                                  hipMemcpyHostToDevice), "prefix");
@@ -994,6 +1016,15 @@ int mhd_ensemble_create(const mhd_cfg* cfg, int n, const int* nx, const int* ny,
     e->el.grid_blocks = 0;
     if (const char* g = std::getenv("MHD_PERSISTENT_BLOCKS")) e->el.grid_blocks = std::atoi(g);
     e->el.cus = cus;
+    if (rc == 0 && cfg->precision == MHD_PREC_MIXED && cfg->fallback != 1) {
+      hipok(hipMalloc((void**)&e->redo_items, 2 * (size_t)total * sizeof(int)), "hipMalloc");
+      if (rc == 0) {
+        e->el.redo_items[0] = e->redo_items;
+        e->el.redo_items[1] = e->redo_items + total;
+        e->el.redo = mhd::launchers_exact(mhd::MODE_FP64)->ens_redo;
+        e->el.redo_all = cfg->fallback == 2 ? 1 : 0;
+      }
+    }
   }
   if (rc != 0) {
     std::fprintf(stderr, "mhd_ensemble_create: %s\n", e->err.c_str());
@@ -1211,6 +1242,7 @@ int mhd_pair_tile_errors(mhd_ctx* c, float* err) {
   mhd::GridDesc g = c->grid;
   g.flags = nullptr;
   g.flag_mode = 0;
+  g.redo_list[0] = g.redo_list[1] = nullptr;  // both updates of every tile
   // Both stage-1 updates of the current state, against a prepared snapshot
   // of the control block (the run's own counters and verdicts are
   // untouched).  The FP32-flux update goes to the context's stage buffer,

@@ -40,6 +40,11 @@ struct Ctrl {
   int fail_cell;                  // cell reported on halt
   int fail_step;
   T fail_t, fail_dt, fail_ch;
+  // FP64 fallback of the mixed form: tiles queued for re-evaluation in this
+  // step's stage 1 / stage 2 (reset when a step is prepared), and the total
+  // over the run.
+  int redo_count[2];
+  unsigned long long redo_tiles;
   // Per-block partial maxima of the signal speed, spread over kSpeedSlots
   // addresses (block b uses slot b % kSpeedSlots) so that tens of thousands
   // of blocks do not serialise on one atomic; folded into speed_acc by the
@@ -62,6 +67,16 @@ struct GridDesc {
   // or only those with flags[t] != 0 (flag_mode 2).
   const unsigned char* flags = nullptr;
   int flag_mode = 0;
+  // FP64 fallback of the mixed form.  With redo_list[stage-1] non-null, a
+  // mixed stage tile with an unusable cell (the positivity check) writes
+  // nothing and appends its index to that list (count in the control
+  // block); `redo` then re-evaluates the listed tiles with the FP64 kernel,
+  // whose own check halts the run as the FP64 form would.  redo_all queues
+  // every tile (check mode: the step is then the FP64 step).
+  int* redo_list[2] = {nullptr, nullptr};
+  hipError_t (*redo)(const GridDesc&, int stage, const void* src, const void* uold, void* dst,
+                     void* ctrl, hipStream_t) = nullptr;
+  int redo_all = 0;
 };
 
 // Tile shapes of the stage kernels (cells TX x TY).  The face list of a
@@ -90,7 +105,16 @@ struct EnsLaunch {
   int muscl, hlld, tile;
   int grid_blocks;      // persistent grid; <= 0: occupancy x CUs
   int cus;              // compute units of the device
+  // FP64 fallback of the mixed form (as GridDesc): queued work items (global
+  // tile indices) per stage, counts in queue[16 + stage - 1], re-evaluated
+  // by `redo` (the FP64 launchers' ens_redo).
+  int* redo_items[2] = {nullptr, nullptr};
+  int redo_all = 0;
+  hipError_t (*redo)(const EnsLaunch&, int stage, hipStream_t) = nullptr;
 };
+// Work-queue counters of an ensemble: 8 chunks per stage, then the two
+// fallback counts.
+constexpr int kEnsQueueWords = 18;
 
 // Launchers of one arithmetic form.  All launches go to `stream`; none
 // synchronises.  State buffers hold `state_planes` planes of `plane`
@@ -113,6 +137,10 @@ struct Launchers {
   hipError_t (*stage)(const GridDesc&, int stage, const void* src, const void* uold, void* dst,
                       void* ctrl, hipStream_t);
   hipError_t (*finish)(void* ctrl, hipStream_t);
+  // FP64 re-evaluation of the tiles listed in g.redo_list[stage-1] (FP64
+  // launchers only; null elsewhere).
+  hipError_t (*redo)(const GridDesc&, int stage, const void* src, const void* uold, void* dst,
+                     void* ctrl, hipStream_t);
   // Ensembles.
   size_t desc_bytes;
   // Fills descriptor i of a host array; returns the instance's tile count.
@@ -121,6 +149,8 @@ struct Launchers {
   // One step of every running instance (persistent stage kernels + finish).
   hipError_t (*ens_step)(const EnsLaunch&, hipStream_t, hipEvent_t* ev);
   hipError_t (*ens_set_target)(void* ctrls, int n, double t_end, unsigned* queue, hipStream_t);
+  // FP64 re-evaluation of an ensemble's queued items (FP64 launchers only).
+  hipError_t (*ens_redo)(const EnsLaunch&, int stage, hipStream_t);
 };
 
 const Launchers* launchers_exact(int mode);  // MODE_FP64, MODE_FP32

@@ -21,9 +21,9 @@ hipError_t sel_tile_features(const double* u, size_t plane, int nx, int ny, int 
 // Per-plane sums of squares of an FP64 state (the error scale).
 hipError_t sel_sumsq(const double* u, size_t plane, long long cells, double* sumsq9, hipStream_t s);
 // err[t]: max over tile t's cells and the 8 MHD variables (psi excluded) of
-// |a-b| / (|b| + rms_k),
-// rms_k = sqrt(sumsq9[k] / cells); tiles with mask[t] == 0 are skipped
-// (mask may be null).
+// |a-b| / (|b| + s_k), s_k the grid RMS of rho, |momentum|, E or |B| (from
+// the per-plane sums of squares sumsq9); tiles with mask[t] == 0 are
+// skipped (mask may be null).
 hipError_t sel_tile_errors(const double* a, const double* b, size_t plane, int nx, int ny, int tx,
                            int ty, const double* sumsq9, const unsigned char* mask, float* err,
                            hipStream_t s);

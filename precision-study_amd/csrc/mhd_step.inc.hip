@@ -32,6 +32,7 @@
 #include "mhd_kernels.h"
 
 #include <climits>
+#include <type_traits>
 
 #ifndef MHD_NS
 #error "MHD_NS must name the translation unit's namespace"
@@ -251,6 +252,7 @@ struct ExactPol {
   using S = R_;
   static constexpr int kPlanes = 9;
   static constexpr int kMode = sizeof(R_) == 8 ? 0 : 1;
+  static constexpr bool kRedo = false;
 };
 // Mixed: FP32 reconstruction / HLLD / GLM; FP64 state, update accumulation,
 // time; FP64 pressure recovery where FP32 would cancel.
@@ -259,6 +261,7 @@ struct MixedPol {
   using T = double;
   using S = double;
   static constexpr int kPlanes = 9;
+  static constexpr bool kRedo = true;  // FP64 fallback of unusable tiles
 };
 // Compensated: as Mixed, but the state is FP32 hi + FP32 lo (planes 0-8 and
 // 9-17), accumulated with float-float additions.
@@ -267,6 +270,7 @@ struct KahanPol {
   using T = double;
   using S = float;
   static constexpr int kPlanes = 18;
+  static constexpr bool kRedo = false;
 };
 
 template <class Pol>
@@ -284,6 +288,12 @@ struct StageArgs {
   Ctrl<typename Pol::T>* ctrl;
   const unsigned char* flags;  // per-tile routing (null: every tile)
   int flag_mode;               // 1: tiles with flag 0; 2: tiles with flag != 0
+  // FP64 fallback (mixed policy): a tile with an unusable cell appends
+  // redo_base + tile at redo_list[atomicAdd(redo_counter, 1)].
+  int* redo_list;              // null: no fallback
+  int* redo_counter;
+  int redo_base;
+  int redo_all;                // queue every tile (check mode)
 };
 
 // Cell state as loaded from HBM, widened to double where the policy stores
@@ -408,6 +418,8 @@ __device__ __forceinline__ void update_cell(const StageArgs<ExactPol<R>>& a,
   for (int k = 0; k < 9; k++) a.dst[k * a.plane + g] = out[k];
 }
 
+// Mixed policy: computes the new state only; stage_tile stores it once the
+// tile has passed the positivity check (FP64 fallback).
 template <int STAGE>
 __device__ __forceinline__ void update_cell(const StageArgs<MixedPol>& a,
                                             const CellIO<MixedPol>& src, size_t g,
@@ -424,8 +436,6 @@ __device__ __forceinline__ void update_cell(const StageArgs<MixedPol>& a,
     else
       out[k] = 0.5 * old.v[k] + 0.5 * (src.v[k] + dt * rhs);
   }
-#pragma unroll
-  for (int k = 0; k < 9; k++) a.dst[k * a.plane + g] = out[k];
 }
 
 template <int STAGE>
@@ -579,12 +589,15 @@ __device__ __forceinline__ void stage_tile(const StageArgs<Pol>& a, int tile,
   // stage update (528-535 / 549-557) and the positivity check (328-345).
   const int r = tid / TX, cc = tid - r * TX;
   const int gi = i0 + cc, gj = j0 + r;
+  const bool live = tid < NT && gi < nx && gj < ny;
+  const size_t g = (size_t)gj * nx + gi;
   unsigned long long sbits = 0;
   unsigned svalid = 0;
-  if (tid < NT && gi < nx && gj < ny) {
+  T out[9];
+  bool unusable = false;
+  if (live) {
     const int xl = r * (TX + 1) + cc, xr = xl + 1;
     const int yb = NXF + r * TX + cc, yt = yb + TX;
-    const size_t g = (size_t)gj * nx + gi;
     const A adx = (A)dx, ady = (A)dy, aidx = (A)a.inv_dx, aidy = (A)a.inv_dy;
     const bool pow2 = a.pow2 != 0;
     A rhs[9];
@@ -595,7 +608,6 @@ __device__ __forceinline__ void stage_tile(const StageArgs<Pol>& a, int tile,
     CellIO<Pol> src;
     src.load(a.src, ps, g);
     const T damping = ch_t / smax((T)a.alpha, (T)1.0e-12);
-    T out[9];
     update_cell<STAGE>(a, src, g, rhs, dt, damping, out);
     if constexpr (sizeof(R) == sizeof(T)) {
       // Exact forms: the reference's check (mhd2d_glm_cpu.cpp:328-345) and
@@ -612,12 +624,30 @@ __device__ __forceinline__ void stage_tile(const StageArgs<Pol>& a, int tile,
     } else {
       double speed;
       bool valid;
-      if (!usable_and_speed_fp64(out, STAGE == 2, speed, valid))
-        atomicMin(&ctrl->bad[STAGE - 1], (int)g);
+      unusable = !usable_and_speed_fp64(out, STAGE == 2, speed, valid);
+      if (unusable && !(Pol::kRedo && a.redo_list)) atomicMin(&ctrl->bad[STAGE - 1], (int)g);
       if (STAGE == 2) {
         sbits = speed_bits(speed);
         svalid = valid ? 1u : 0u;
       }
+    }
+  }
+  if constexpr (Pol::kRedo) {
+    // FP64 fallback: a tile with an unusable cell writes nothing (its
+    // inputs -- the stage source and, in stage 2, its own cells of U^n --
+    // stay intact) and is queued for the FP64 kernel, which also publishes
+    // its counters and speeds.
+    if (a.redo_list && __syncthreads_or((unusable || a.redo_all) ? 1 : 0)) {
+      if (tid == 0) {
+        const int slot = atomicAdd(a.redo_counter, 1);
+        a.redo_list[slot] = a.redo_base + tile;
+        atomicAdd(&ctrl->redo_tiles, 1ull);
+      }
+      return;
+    }
+    if (live) {
+#pragma unroll
+      for (int k = 0; k < 9; k++) a.dst[k * a.plane + g] = out[k];
     }
   }
 
@@ -660,6 +690,22 @@ __global__ __launch_bounds__((TX * TY + 63) / 64 * 64) void k_stage(StageArgs<Po
   const int tile = xcd_tile((int)blockIdx.x, a.ntiles);
   if (a.flags && ((a.flags[tile] != 0) != (a.flag_mode == 2))) return;
   stage_tile<Pol, MUSCL, HLLD, STAGE, TX, TY>(a, tile, sh);
+}
+
+// FP64 fallback: re-evaluates the tiles a mixed stage kernel queued in
+// `list` (count in the control block), a fixed grid striding over the list.
+template <class Pol, bool MUSCL, bool HLLD, int STAGE, int TX, int TY>
+__global__ __launch_bounds__((TX * TY + 63) / 64 * 64) void k_stage_redo(StageArgs<Pol> a,
+                                                                          const int* list) {
+  __shared__ TileShared<typename Pol::R, TX, TY> sh;
+  Ctrl<typename Pol::T>* ctrl = a.ctrl;
+  if (ctrl->run != RUN_ACTIVE) return;
+  if (STAGE == 2 && ctrl->bad[0] != INT_MAX) return;
+  const int count = ctrl->redo_count[STAGE - 1];
+  for (int e = (int)blockIdx.x; e < count; e += (int)gridDim.x) {
+    stage_tile<Pol, MUSCL, HLLD, STAGE, TX, TY>(a, list[e], sh);
+    __syncthreads();
+  }
 }
 
 // Signal-speed maximum of a state (initial state / after set_state), in T.
@@ -717,6 +763,8 @@ template <class T>
 __device__ void prepare_step(Ctrl<T>* ctrl) {
   ctrl->bad[0] = INT_MAX;
   ctrl->bad[1] = INT_MAX;
+  ctrl->redo_count[0] = 0;
+  ctrl->redo_count[1] = 0;
   if (!(ctrl->t < ctrl->t_end)) {
     ctrl->run = RUN_DONE;
     return;
@@ -814,10 +862,42 @@ struct EnsArgs {
   const InstDesc<Pol>* desc;
   const int* prefix;  // prefix[i] = first item of instance i; prefix[n] = total
   int n, total;
-  unsigned* queue;    // 2 stages x 8 chunk counters, zeroed by the finish kernel
+  unsigned* queue;    // 2 stages x 8 chunk counters + 2 fallback counts, zeroed
+                      // by the finish kernel
   double tol, alpha;
   float lowbeta_ratio;
+  int* redo_items[2];   // FP64 fallback queues (null: none)
+  int redo_all;
 };
+
+// Stage arguments of one ensemble instance.
+template <class Pol, int STAGE>
+__device__ __forceinline__ StageArgs<Pol> inst_args(const EnsArgs<Pol>& e, const InstDesc<Pol>& d,
+                                                    int first_item) {
+  StageArgs<Pol> a;
+  a.plane = d.plane;
+  a.nx = d.nx;
+  a.ny = d.ny;
+  a.tiles_x = d.tiles_x;
+  a.ntiles = d.ntiles;
+  a.pow2 = d.pow2;
+  a.inv_dx = d.inv_dx;
+  a.inv_dy = d.inv_dy;
+  a.tol = e.tol;
+  a.alpha = e.alpha;
+  a.lowbeta_ratio = e.lowbeta_ratio;
+  a.ctrl = d.ctrl;
+  a.flags = nullptr;
+  a.flag_mode = 0;
+  a.redo_list = e.redo_items[STAGE - 1];
+  a.redo_counter = (int*)&e.queue[16 + STAGE - 1];
+  a.redo_base = first_item;
+  a.redo_all = e.redo_all;
+  a.src = STAGE == 1 ? d.u : d.u1;
+  a.uold = d.u;
+  a.dst = STAGE == 1 ? d.u1 : d.u;
+  return a;
+}
 
 __device__ __forceinline__ int find_instance(const int* prefix, int n, int item) {
   int lo = 0, hi = n - 1;
@@ -859,25 +939,30 @@ __global__ __launch_bounds__((TX * TY + 63) / 64 * 64) void k_stage_queue(EnsArg
     const InstDesc<Pol>& d = e.desc[inst];
     if (d.ctrl->run != RUN_ACTIVE) continue;
     if (STAGE == 2 && d.ctrl->bad[0] != INT_MAX) continue;
-    StageArgs<Pol> a;
-    a.plane = d.plane;
-    a.nx = d.nx;
-    a.ny = d.ny;
-    a.tiles_x = d.tiles_x;
-    a.ntiles = d.ntiles;
-    a.pow2 = d.pow2;
-    a.inv_dx = d.inv_dx;
-    a.inv_dy = d.inv_dy;
-    a.tol = e.tol;
-    a.alpha = e.alpha;
-    a.lowbeta_ratio = e.lowbeta_ratio;
-    a.ctrl = d.ctrl;
-    a.flags = nullptr;
-    a.flag_mode = 0;
-    a.src = STAGE == 1 ? d.u : d.u1;
-    a.uold = d.u;
-    a.dst = STAGE == 1 ? d.u1 : d.u;
-    stage_tile<Pol, MUSCL, HLLD, STAGE, TX, TY>(a, item - e.prefix[inst], sh);
+    const int first = e.prefix[inst];
+    stage_tile<Pol, MUSCL, HLLD, STAGE, TX, TY>(inst_args<Pol, STAGE>(e, d, first), item - first,
+                                                sh);
+  }
+}
+
+// FP64 fallback of an ensemble: re-evaluates the work items a mixed stage
+// queued (count in queue[16 + STAGE - 1]); a fixed grid strides over them.
+template <class Pol, bool MUSCL, bool HLLD, int STAGE, int TX, int TY>
+__global__ __launch_bounds__((TX * TY + 63) / 64 * 64) void k_queue_redo(EnsArgs<Pol> e,
+                                                                          const int* list) {
+  __shared__ TileShared<typename Pol::R, TX, TY> sh;
+  const int count = (int)e.queue[16 + STAGE - 1];
+  for (int x = (int)blockIdx.x; x < count; x += (int)gridDim.x) {
+    const int item = list[x];
+    const int inst = find_instance(e.prefix, e.n, item);
+    const InstDesc<Pol>& d = e.desc[inst];
+    if (d.ctrl->run != RUN_ACTIVE) continue;
+    if (STAGE == 2 && d.ctrl->bad[0] != INT_MAX) continue;
+    const int first = e.prefix[inst];
+    StageArgs<Pol> a = inst_args<Pol, STAGE>(e, d, first);
+    a.redo_list = nullptr;
+    stage_tile<Pol, MUSCL, HLLD, STAGE, TX, TY>(a, item - first, sh);
+    __syncthreads();
   }
 }
 
@@ -885,13 +970,13 @@ template <class T>
 __global__ void k_finish_multi(Ctrl<T>* ctrls, int n, unsigned* queue) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) finish_one(&ctrls[i]);
-  if (i < 16) queue[i] = 0;
+  if (i < kEnsQueueWords) queue[i] = 0;
 }
 
 template <class T>
 __global__ void k_set_target_multi(Ctrl<T>* ctrls, int n, double t_end, unsigned* queue) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < 16) queue[i] = 0;
+  if (i < kEnsQueueWords) queue[i] = 0;
   if (i >= n) return;
   Ctrl<T>* ctrl = &ctrls[i];
   if (ctrl->run == RUN_HALTED) return;
@@ -903,6 +988,12 @@ __global__ void k_set_target_multi(Ctrl<T>* ctrls, int n, double t_end, unsigned
 // ---------------------------------------------------------------------------
 // Host launchers.
 // ---------------------------------------------------------------------------
+// Device address of a control block's per-stage fallback counter.
+template <class T>
+static int* redo_counter_of(void* ctrl, int stage) {
+  return (int*)((char*)ctrl + offsetof(Ctrl<T>, redo_count) + (stage - 1) * sizeof(int));
+}
+
 template <class Pol, int TX, int TY>
 static StageArgs<Pol> stage_args(const GridDesc& g, void* ctrl) {
   using T = typename Pol::T;
@@ -919,6 +1010,10 @@ static StageArgs<Pol> stage_args(const GridDesc& g, void* ctrl) {
   a.ctrl = (Ctrl<T>*)ctrl;
   a.flags = g.flags;
   a.flag_mode = g.flag_mode;
+  a.redo_list = nullptr;
+  a.redo_counter = redo_counter_of<T>(ctrl, 1);
+  a.redo_base = 0;
+  a.redo_all = g.redo_all;
   // dx = (1-0)/nx in the control type, as the reference computes it.  When
   // nx and ny are powers of two, dx and dy are exact powers of two and
   // dividing by them equals multiplying by their exact inverses.
@@ -935,19 +1030,26 @@ static hipError_t launch_step_t(const GridDesc& g, void* u, void* u1, void* ctrl
   using S = typename Pol::S;
   using T = typename Pol::T;
   StageArgs<Pol> a = stage_args<Pol, TX, TY>(g, ctrl);
+  const bool redo = Pol::kRedo && g.redo && g.redo_list[0];
   a.src = (const S*)u;
   a.uold = (const S*)u;
   a.dst = (S*)u1;
+  if (redo) a.redo_list = g.redo_list[0];
+  a.redo_counter = redo_counter_of<T>(ctrl, 1);
   constexpr int nthr = (TX * TY + 63) / 64 * 64;
   if (ev) (void)hipEventRecord(ev[0], s);
   hipLaunchKernelGGL((k_stage<Pol, MUSCL, HLLD, 1, TX, TY>), dim3(a.ntiles), dim3(nthr), 0, s,
                      a);
   if (ev) (void)hipEventRecord(ev[1], s);
+  if (redo) (void)g.redo(g, 1, u, u, u1, ctrl, s);
   a.src = (const S*)u1;
   a.uold = (const S*)u;
   a.dst = (S*)u;
+  if (redo) a.redo_list = g.redo_list[1];
+  a.redo_counter = redo_counter_of<T>(ctrl, 2);
   hipLaunchKernelGGL((k_stage<Pol, MUSCL, HLLD, 2, TX, TY>), dim3(a.ntiles), dim3(nthr), 0, s,
                      a);
+  if (redo) (void)g.redo(g, 2, u1, u, u, ctrl, s);
   if (ev) (void)hipEventRecord(ev[2], s);
   hipLaunchKernelGGL(k_finish<T>, dim3(1), dim3(64), 0, s, (Ctrl<T>*)ctrl);
   if (ev) (void)hipEventRecord(ev[3], s);
@@ -959,9 +1061,12 @@ static hipError_t launch_stage_t(const GridDesc& g, int stage, const void* src, 
                                  void* dst, void* ctrl, hipStream_t s) {
   using S = typename Pol::S;
   StageArgs<Pol> a = stage_args<Pol, TX, TY>(g, ctrl);
+  const bool redo = Pol::kRedo && g.redo && g.redo_list[stage - 1];
   a.src = (const S*)src;
   a.uold = (const S*)uold;
   a.dst = (S*)dst;
+  if (redo) a.redo_list = g.redo_list[stage - 1];
+  a.redo_counter = redo_counter_of<typename Pol::T>(ctrl, stage);
   constexpr int nthr = (TX * TY + 63) / 64 * 64;
   if (stage == 1)
     hipLaunchKernelGGL((k_stage<Pol, MUSCL, HLLD, 1, TX, TY>), dim3(a.ntiles), dim3(nthr), 0, s,
@@ -969,7 +1074,55 @@ static hipError_t launch_stage_t(const GridDesc& g, int stage, const void* src, 
   else
     hipLaunchKernelGGL((k_stage<Pol, MUSCL, HLLD, 2, TX, TY>), dim3(a.ntiles), dim3(nthr), 0, s,
                        a);
+  hipError_t e = hipGetLastError();
+  if (redo && e == hipSuccess) e = g.redo(g, stage, src, uold, dst, ctrl, s);
+  return e;
+}
+
+// Grid of the FP64 fallback kernel: enough blocks for a few hundred queued
+// tiles at once; an empty queue costs one short launch.
+constexpr int kRedoBlocks = 512;
+
+template <class Pol, bool MUSCL, bool HLLD, int TX, int TY>
+static hipError_t launch_redo_t(const GridDesc& g, int stage, const void* src, const void* uold,
+                                void* dst, void* ctrl, hipStream_t s) {
+  using S = typename Pol::S;
+  StageArgs<Pol> a = stage_args<Pol, TX, TY>(g, ctrl);
+  a.flags = nullptr;
+  a.flag_mode = 0;
+  a.redo_all = 0;
+  a.src = (const S*)src;
+  a.uold = (const S*)uold;
+  a.dst = (S*)dst;
+  const int* list = g.redo_list[stage - 1];
+  const int grid = a.ntiles < kRedoBlocks ? a.ntiles : kRedoBlocks;
+  constexpr int nthr = (TX * TY + 63) / 64 * 64;
+  if (stage == 1)
+    hipLaunchKernelGGL((k_stage_redo<Pol, MUSCL, HLLD, 1, TX, TY>), dim3(grid), dim3(nthr), 0, s,
+                       a, list);
+  else
+    hipLaunchKernelGGL((k_stage_redo<Pol, MUSCL, HLLD, 2, TX, TY>), dim3(grid), dim3(nthr), 0, s,
+                       a, list);
   return hipGetLastError();
+}
+
+template <class Pol>
+static hipError_t launch_redo(const GridDesc& g, int stage, const void* src, const void* uold,
+                              void* dst, void* ctrl, hipStream_t s) {
+  if (!g.redo_list[stage - 1]) return hipErrorInvalidValue;
+  if (!(g.muscl && g.hlld)) {
+    if (g.muscl) return launch_redo_t<Pol, true, false, 16, 15>(g, stage, src, uold, dst, ctrl, s);
+    if (g.hlld) return launch_redo_t<Pol, false, true, 16, 15>(g, stage, src, uold, dst, ctrl, s);
+    return launch_redo_t<Pol, false, false, 16, 15>(g, stage, src, uold, dst, ctrl, s);
+  }
+  switch (g.tile) {
+    case TILE_16x16: return launch_redo_t<Pol, true, true, 16, 16>(g, stage, src, uold, dst, ctrl, s);
+    case TILE_32x8: return launch_redo_t<Pol, true, true, 32, 8>(g, stage, src, uold, dst, ctrl, s);
+    case TILE_16x8: return launch_redo_t<Pol, true, true, 16, 8>(g, stage, src, uold, dst, ctrl, s);
+    case TILE_32x7: return launch_redo_t<Pol, true, true, 32, 7>(g, stage, src, uold, dst, ctrl, s);
+    case TILE_16x7: return launch_redo_t<Pol, true, true, 16, 7>(g, stage, src, uold, dst, ctrl, s);
+    default: return launch_redo_t<Pol, true, true, 16, 15>(g, stage, src, uold, dst, ctrl, s);
+  }
 }
 
 template <class Pol>
@@ -1067,10 +1220,8 @@ static int ens_fill(void* host_desc, int i, void* u, void* u1, size_t plane, int
   return d.ntiles;
 }
 
-template <class Pol, bool MUSCL, bool HLLD, int TX, int TY>
-static hipError_t ens_step_t(const EnsLaunch& L, hipStream_t s, hipEvent_t* ev) {
-  using T = typename Pol::T;
-  constexpr int nthr = (TX * TY + 63) / 64 * 64;
+template <class Pol>
+static EnsArgs<Pol> ens_args(const EnsLaunch& L) {
   EnsArgs<Pol> e;
   e.desc = (const InstDesc<Pol>*)L.desc;
   e.prefix = L.prefix;
@@ -1080,6 +1231,58 @@ static hipError_t ens_step_t(const EnsLaunch& L, hipStream_t s, hipEvent_t* ev) 
   e.tol = L.tol;
   e.alpha = L.alpha;
   e.lowbeta_ratio = (float)L.lowbeta_ratio;
+  e.redo_items[0] = e.redo_items[1] = nullptr;
+  e.redo_all = 0;
+  return e;
+}
+
+// FP64 fallback of an ensemble stage.  The descriptors were filled by the
+// mixed launchers; InstDesc<MixedPol> and InstDesc<ExactPol<double>> have
+// the same layout (double state planes, Ctrl<double>).
+template <class Pol, bool MUSCL, bool HLLD, int TX, int TY>
+static hipError_t ens_redo_t(const EnsLaunch& L, int stage, hipStream_t s) {
+  constexpr int nthr = (TX * TY + 63) / 64 * 64;
+  const EnsArgs<Pol> e = ens_args<Pol>(L);
+  const int* list = L.redo_items[stage - 1];
+  const int grid = L.total < kRedoBlocks ? L.total : kRedoBlocks;
+  if (stage == 1)
+    hipLaunchKernelGGL((k_queue_redo<Pol, MUSCL, HLLD, 1, TX, TY>), dim3(grid), dim3(nthr), 0, s, e,
+                       list);
+  else
+    hipLaunchKernelGGL((k_queue_redo<Pol, MUSCL, HLLD, 2, TX, TY>), dim3(grid), dim3(nthr), 0, s, e,
+                       list);
+  return hipGetLastError();
+}
+
+template <class Pol>
+static hipError_t ens_redo(const EnsLaunch& L, int stage, hipStream_t s) {
+  if (!L.redo_items[stage - 1] || L.total < 1) return hipErrorInvalidValue;
+  if (!(L.muscl && L.hlld)) {
+    if (L.muscl) return ens_redo_t<Pol, true, false, 16, 15>(L, stage, s);
+    if (L.hlld) return ens_redo_t<Pol, false, true, 16, 15>(L, stage, s);
+    return ens_redo_t<Pol, false, false, 16, 15>(L, stage, s);
+  }
+  switch (L.tile) {
+    case TILE_16x16: return ens_redo_t<Pol, true, true, 16, 16>(L, stage, s);
+    case TILE_32x8: return ens_redo_t<Pol, true, true, 32, 8>(L, stage, s);
+    case TILE_16x8: return ens_redo_t<Pol, true, true, 16, 8>(L, stage, s);
+    case TILE_32x7: return ens_redo_t<Pol, true, true, 32, 7>(L, stage, s);
+    case TILE_16x7: return ens_redo_t<Pol, true, true, 16, 7>(L, stage, s);
+    default: return ens_redo_t<Pol, true, true, 16, 15>(L, stage, s);
+  }
+}
+
+template <class Pol, bool MUSCL, bool HLLD, int TX, int TY>
+static hipError_t ens_step_t(const EnsLaunch& L, hipStream_t s, hipEvent_t* ev) {
+  using T = typename Pol::T;
+  constexpr int nthr = (TX * TY + 63) / 64 * 64;
+  EnsArgs<Pol> e = ens_args<Pol>(L);
+  const bool redo = Pol::kRedo && L.redo && L.redo_items[0] && L.redo_items[1];
+  if (redo) {
+    e.redo_items[0] = L.redo_items[0];
+    e.redo_items[1] = L.redo_items[1];
+    e.redo_all = L.redo_all;
+  }
   int grid = L.grid_blocks;
   if (grid <= 0) {
     static int per_cu = 0;
@@ -1094,7 +1297,9 @@ static hipError_t ens_step_t(const EnsLaunch& L, hipStream_t s, hipEvent_t* ev) 
   if (ev) (void)hipEventRecord(ev[0], s);
   hipLaunchKernelGGL((k_stage_queue<Pol, MUSCL, HLLD, 1, TX, TY>), dim3(grid), dim3(nthr), 0, s, e);
   if (ev) (void)hipEventRecord(ev[1], s);
+  if (redo) (void)L.redo(L, 1, s);
   hipLaunchKernelGGL((k_stage_queue<Pol, MUSCL, HLLD, 2, TX, TY>), dim3(grid), dim3(nthr), 0, s, e);
+  if (redo) (void)L.redo(L, 2, s);
   if (ev) (void)hipEventRecord(ev[2], s);
   hipLaunchKernelGGL(k_finish_multi<T>, dim3((L.n + 63) / 64 > 0 ? (L.n + 63) / 64 : 1), dim3(64),
                      0, s, (Ctrl<T>*)L.ctrls, L.n, L.queue);
@@ -1140,6 +1345,13 @@ static Launchers make_launchers(int mode) {
   L.step = &launch_step<Pol>;
   L.stage = &launch_stage<Pol>;
   L.finish = &launch_finish<typename Pol::T>;
+  if constexpr (std::is_same<Pol, ExactPol<double>>::value) {
+    L.redo = &launch_redo<Pol>;
+    L.ens_redo = &ens_redo<Pol>;
+  } else {
+    L.redo = nullptr;
+    L.ens_redo = nullptr;
+  }
   L.desc_bytes = sizeof(InstDesc<Pol>);
   L.ens_fill = &ens_fill<Pol>;
   L.ens_step = &ens_step<Pol>;

@@ -10,9 +10,11 @@
 // .dat name and 13-column layout, and exit codes (1 usage, 2 rate, 3/4
 // non-physical state, 5 output file, 9 device).  Precision comes from the
 // program name (a name ending in "fp32" runs FP32, as the reference's
-// separate -DUSE_FLOAT binary does) or from MHD_PRECISION=fp64|fp32.
-// MHD_HLLD_TOLERANCE sets the HLLD tolerance (default 1e-4, the value of
-// every reference timing and checksum run).
+// separate -DUSE_FLOAT binary does) or from MHD_PRECISION (fp64, fp32,
+// mixed, mixed_kahan, fp64_fma).  MHD_HLLD_TOLERANCE sets the HLLD tolerance
+// (default 1e-4, the value of every reference timing and checksum run);
+// MHD_TILE the stage tile shape, MHD_LOWBETA_RATIO the mixed form's FP64
+// recovery switch and MHD_FALLBACK its FP64 tile fallback (0 on, 1 off).
 //
 // This program only talks to the solver through include/mhd_glm.h.
 #include "mhd_glm.h"
@@ -149,6 +151,7 @@ int run(const Args& a, int precision, const char* prec_name, double tol) {
   cfg.lowbeta_ratio = 0.0;
   if (const char* s = std::getenv("MHD_LOWBETA_RATIO")) cfg.lowbeta_ratio = std::atof(s);
   if (const char* s = std::getenv("MHD_TILE")) cfg.tile = std::atoi(s);
+  if (const char* s = std::getenv("MHD_FALLBACK")) cfg.fallback = std::atoi(s);
   mhd_ctx* ctx = nullptr;
   int rc = mhd_create(&cfg, &ctx);
   if (rc != 0) {
@@ -233,6 +236,8 @@ int run(const Args& a, int precision, const char* prec_name, double tol) {
               << ", nonphysical=" << st.nonphysical << "), fraction = " << std::scientific
               << frac << std::defaultfloat << std::endl;
   }
+  if (st.fallback_tiles > 0)
+    std::cout << "FP64 fallback stage tiles = " << st.fallback_tiles << std::endl;
 
   const std::string base = a.problem == "ot" ? "orszag_tang" : a.problem;
   const char* out_env = std::getenv("OUTDIR");

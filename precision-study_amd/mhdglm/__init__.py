@@ -68,7 +68,7 @@ class Cfg(ctypes.Structure):
         ("muscl", ctypes.c_int), ("hlld", ctypes.c_int),
         ("precision", ctypes.c_int), ("hlld_tol", ctypes.c_double),
         ("tile", ctypes.c_int), ("batch_steps", ctypes.c_int),
-        ("lowbeta_ratio", ctypes.c_double),
+        ("lowbeta_ratio", ctypes.c_double), ("fallback", ctypes.c_int),
     ]
 
 
@@ -79,6 +79,7 @@ class Status(ctypes.Structure):
         ("x_evaluations", ctypes.c_ulonglong), ("y_evaluations", ctypes.c_ulonglong),
         ("degenerate", ctypes.c_ulonglong), ("nonphysical", ctypes.c_ulonglong),
         ("bad_i", ctypes.c_int), ("bad_j", ctypes.c_int), ("fail_step", ctypes.c_int),
+        ("fallback_tiles", ctypes.c_ulonglong),
     ]
 
     def as_dict(self) -> dict:
@@ -223,6 +224,7 @@ class SolverConfig:
     tile: int = -1
     batch_steps: int = 0
     lowbeta_ratio: float = 0.0
+    fallback: int = 0  # MIXED: 0 FP64 fallback on, 1 off, 2 every tile (check mode)
 
 
 class Solver:
@@ -302,7 +304,9 @@ class Solver:
         return out
 
     def configure_selector(self, model, threshold=None, audit_frac=0.01, audit_budget=None):
-        """model: dict params/mean/std/threshold/budget (mhdglm.selector); None: off."""
+        """model: dict params/mean/std/threshold/budget[/audit_budget]
+        (mhdglm.selector); None: off.  The audit budget defaults to the
+        model's audit_budget, else 10 x its label budget."""
         if model is None:
             self._check(self.lib.mhd_selector_configure(self.h, None, None, None, 0.0, 0.0, 0.0))
             return
@@ -310,7 +314,9 @@ class Solver:
         m = np.ascontiguousarray(model["mean"], dtype=np.float32)
         sd = np.ascontiguousarray(model["std"], dtype=np.float32)
         th = float(model["threshold"] if threshold is None else threshold)
-        bud = float(model["budget"] if audit_budget is None else audit_budget)
+        if audit_budget is None:
+            audit_budget = model.get("audit_budget", 10.0 * float(model["budget"]))
+        bud = float(audit_budget)
         self._check(self.lib.mhd_selector_configure(self.h, _ptr(p), _ptr(m), _ptr(sd), th,
                                                     float(audit_frac), bud))
 
@@ -379,7 +385,8 @@ def mlp_train(params, x, y, w, epochs=20, batch=1024, lr=1e-3, seed=0):
 
 def _cfg(cfg: "SolverConfig", nx: int = 0, ny: int = 0) -> Cfg:
     return Cfg(nx or cfg.nx, ny or cfg.ny, cfg.cfl, cfg.alpha, int(cfg.muscl), int(cfg.hlld),
-               cfg.precision, cfg.hlld_tol, cfg.tile, cfg.batch_steps, cfg.lowbeta_ratio)
+               cfg.precision, cfg.hlld_tol, cfg.tile, cfg.batch_steps, cfg.lowbeta_ratio,
+               cfg.fallback)
 
 
 class Ensemble:

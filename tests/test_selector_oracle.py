@@ -79,6 +79,12 @@ def test_tile_errors_closed_form():
     assert e.shape == (4,)
     assert e[0] == pytest.approx(0.25)   # 0.5 / (|1| + rms 1)
     assert np.all(e[1:] == 0.0)
+    # vector components are scaled by the vector's RMS: |m| = sqrt(3) here
+    a2 = b.copy()
+    a2[2, 6, 6] += 0.5                   # momentum y, tile 3
+    e2 = so.tile_errors(a2, b, 4, 4)
+    assert e2[3] == pytest.approx(0.5 / (1.0 + np.sqrt(3.0)))
+    assert np.all(e2[:3] == 0.0)
     # the GLM potential does not enter
     a[8, 5, 5] += 3.0
     assert np.array_equal(so.tile_errors(a, b, 4, 4), e)
