@@ -171,9 +171,9 @@ int mhd_fast_speed_batch(int precision, int dir, int n, const void* W,
 
 /* Learned precision selector (MIXED form; no reference counterpart).  Per
  * stage-kernel tile and step an MLP 12 -> 64 -> 64 -> 1 (BF16 MFMA, FP32
- * accumulate, sigmoid) scores 12 tile features; tiles scoring above the
- * threshold are evaluated by the FP64 kernels, the rest by the FP32-flux
- * kernels.  A fraction of the FP32 tiles is recomputed in FP64 each step
+ * accumulate, sigmoid) scores 12 tile features (refreshed every 8 steps);
+ * tiles scoring above the threshold are evaluated by the FP64 kernels, the
+ * rest by the FP32-flux kernels.  A fraction of the FP32 tiles is recomputed in FP64 each step
  * (audit); a scaled difference above the budget makes the tile FP64 for the
  * rest of the run.  params holds mhd_mlp_param_count() FP32 weights in the
  * layout W1[16][64], b1[64], W2[64][64], b2[64], w3[64], b3; mean/stdev

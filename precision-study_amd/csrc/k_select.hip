@@ -76,11 +76,24 @@ __device__ __forceinline__ CellPrim prim_at(const double* u, size_t ps, size_t g
   return c;
 }
 
+// Wave reduction of a float (64 lanes, butterfly).
+template <class Op>
+__device__ __forceinline__ float wave_reduce(float v, Op op) {
+  for (int off = 32; off > 0; off >>= 1) v = op(v, __shfl_xor(v, off, 64));
+  return v;
+}
+struct FMax { __device__ float operator()(float a, float b) const { return a < b ? b : a; } };
+struct FMin { __device__ float operator()(float a, float b) const { return b < a ? b : a; } };
+struct FSum { __device__ float operator()(float a, float b) const { return a + b; } };
+
 // One block (256 threads) per tile; thread t handles cell t of the tile.
+// Per-cell quantities are computed in FP64 and reduced in FP32 (the features
+// are FP32): 15 wave reductions, one barrier, thread 0 combines the waves.
 __global__ __launch_bounds__(256) void k_tile_features(const double* u, size_t ps, int nx, int ny,
                                                        int tx, int ty, int tiles_x, int ntiles,
                                                        const void* ctrl_ch, float* feats) {
-  __shared__ double red[4];
+  constexpr int NQ = 15;
+  __shared__ float red[4][NQ];
   const int tile = blockIdx.x;
   if (tile >= ntiles) return;
   const double ch = *(const double*)ctrl_ch;
@@ -89,8 +102,9 @@ __global__ __launch_bounds__(256) void k_tile_features(const double* u, size_t p
   const int i = i0 + li, j = j0 + lj;
   const bool live = t < tx * ty && i < nx && j < ny;
   const double gamma = 5.0 / 3.0;
-  double beta_mn = 1e300, beta_mx = -1e300, drho = 0, dp = 0, db = 0, mach = 0, psi = 0, bmax = 0;
-  double divn = 0, bnmin = 1e300, vch = 0, cf = 0, ke = 0, me = 0, n = 0;
+  // q: min beta, max beta, drho, dp, db, mach, psi, |B|, div B, min bn,
+  //    v/ch, cf/ch, kinetic, magnetic, cells
+  float q[NQ] = {3e38f, -3e38f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 3e38f, 0.f, 0.f, 0.f, 0.f, 0.f};
   if (live) {
     const size_t g = (size_t)j * nx + i;
     const CellPrim c = prim_at(u, ps, g);
@@ -100,53 +114,66 @@ __global__ __launch_bounds__(256) void k_tile_features(const double* u, size_t p
     const double byd = u[6 * ps + (size_t)wrapi(j - 1, ny) * nx + i];
     const double pp = fmax(c.p, 1e-12), bmag = sqrt(c.b2);
     const double beta = fmin(fmax(2.0 * pp / fmax(c.b2, 1e-30), 1e-6), 1e6);
-    beta_mn = beta_mx = beta;
-    drho = fmax(fabs(r.rho - c.rho), fabs(tp.rho - c.rho)) / c.rho;
-    dp = fmax(fabs(fmax(r.p, 1e-12) - pp), fabs(fmax(tp.p, 1e-12) - pp)) / pp;
+    q[0] = q[1] = (float)beta;
+    q[2] = (float)(fmax(fabs(r.rho - c.rho), fabs(tp.rho - c.rho)) / c.rho);
+    q[3] = (float)(fmax(fabs(fmax(r.p, 1e-12) - pp), fabs(fmax(tp.p, 1e-12) - pp)) / pp);
     const double dbr = sqrt((r.bx - c.bx) * (r.bx - c.bx) + (r.by - c.by) * (r.by - c.by) +
                             (r.bz - c.bz) * (r.bz - c.bz));
     const double dbt = sqrt((tp.bx - c.bx) * (tp.bx - c.bx) + (tp.by - c.by) * (tp.by - c.by) +
                             (tp.bz - c.bz) * (tp.bz - c.bz));
-    db = fmax(dbr, dbt) / fmax(bmag, 1e-12);
+    q[4] = (float)(fmax(dbr, dbt) / fmax(bmag, 1e-12));
     const double v2 = c.vx * c.vx + c.vy * c.vy + c.vz * c.vz, cs2 = gamma * pp / c.rho;
-    mach = fmin(sqrt(v2 / cs2), 100.0);
-    psi = fabs(u[8 * ps + g]);
-    bmax = bmag;
+    q[5] = (float)fmin(sqrt(v2 / cs2), 100.0);
+    q[6] = (float)fabs(u[8 * ps + g]);
+    q[7] = (float)bmag;
     const double h = 1.0 / (double)(nx > ny ? nx : ny);
     const double divb = 0.5 * (r.bx - bxl) * nx + 0.5 * (tp.by - byd) * ny;
-    divn = fabs(divb) * h / (bmag + 1e-12);
-    bnmin = fmin(fabs(c.bx), fabs(c.by)) / (bmag + 1e-12);
-    vch = sqrt(v2) / ch;
-    cf = sqrt(cs2 + c.b2 / c.rho) / ch;
-    ke = 0.5 * c.rho * v2;
-    me = 0.5 * c.b2;
-    n = 1.0;
+    q[8] = (float)(fabs(divb) * h / (bmag + 1e-12));
+    q[9] = (float)(fmin(fabs(c.bx), fabs(c.by)) / (bmag + 1e-12));
+    q[10] = (float)(sqrt(v2) / ch);
+    q[11] = (float)(sqrt(cs2 + c.b2 / c.rho) / ch);
+    q[12] = (float)(0.5 * c.rho * v2);
+    q[13] = (float)(0.5 * c.b2);
+    q[14] = 1.f;
   }
-  const Max mx;
-  const Min mn;
-  const Sum sm;
-  const double f0 = block_reduce(beta_mn, red, mn), f1 = block_reduce(beta_mx, red, mx);
-  const double f2 = block_reduce(drho, red, mx), f3 = block_reduce(dp, red, mx);
-  const double f4 = block_reduce(db, red, mx), f5 = block_reduce(mach, red, mx);
-  const double psimax = block_reduce(psi, red, mx), bm = block_reduce(bmax, red, mx);
-  const double f7 = block_reduce(divn, red, mx), f8 = block_reduce(bnmin, red, mn);
-  const double f9 = block_reduce(vch, red, mx), f10 = block_reduce(cf, red, mx);
-  const double kes = block_reduce(ke, red, sm), mes = block_reduce(me, red, sm);
-  const double cnt = block_reduce(n, red, sm);
-  if (threadIdx.x == 0) {
+  const FMax mx;
+  const FMin mn;
+  const FSum sm;
+  q[0] = wave_reduce(q[0], mn);
+  q[9] = wave_reduce(q[9], mn);
+#pragma unroll
+  for (int k = 1; k <= 11; k++)
+    if (k != 9) q[k] = wave_reduce(q[k], mx);
+#pragma unroll
+  for (int k = 12; k < NQ; k++) q[k] = wave_reduce(q[k], sm);
+  const int lane = t & 63, w = t >> 6;
+  if (lane == 0)
+#pragma unroll
+    for (int k = 0; k < NQ; k++) red[w][k] = q[k];
+  __syncthreads();
+  if (t == 0) {
+    float f[NQ];
+#pragma unroll
+    for (int k = 0; k < NQ; k++) {
+      float v = red[0][k];
+      for (int ww = 1; ww < 4; ww++)
+        v = (k == 0 || k == 9) ? mn(v, red[ww][k]) : (k >= 12 ? sm(v, red[ww][k]) : mx(v, red[ww][k]));
+      f[k] = v;
+    }
+    const float cnt = f[14];
     float* o = feats + (size_t)tile * NIN;
-    o[0] = (float)log10(f0);
-    o[1] = (float)log10(f1);
-    o[2] = (float)log10(f2 + 1e-12);
-    o[3] = (float)log10(f3 + 1e-12);
-    o[4] = (float)log10(f4 + 1e-12);
-    o[5] = (float)f5;
-    o[6] = (float)log10(psimax / (bm + 1e-12) + 1e-12);
-    o[7] = (float)log10(f7 + 1e-12);
-    o[8] = (float)log10(f8 + 1e-12);
-    o[9] = (float)f9;
-    o[10] = (float)f10;
-    o[11] = (float)log10(kes / cnt / (mes / cnt + 1e-12) + 1e-12);
+    o[0] = log10f(f[0]);
+    o[1] = log10f(f[1]);
+    o[2] = log10f(f[2] + 1e-12f);
+    o[3] = log10f(f[3] + 1e-12f);
+    o[4] = log10f(f[4] + 1e-12f);
+    o[5] = f[5];
+    o[6] = log10f(f[6] / (f[7] + 1e-12f) + 1e-12f);
+    o[7] = log10f(f[8] + 1e-12f);
+    o[8] = log10f(f[9] + 1e-12f);
+    o[9] = f[10];
+    o[10] = f[11];
+    o[11] = log10f(f[12] / cnt / (f[13] / cnt + 1e-12f) + 1e-12f);
     for (int k = NFEAT; k < NIN; k++) o[k] = 0.f;
   }
 }
@@ -416,14 +443,17 @@ __global__ void k_adam(float* P, float* grad, float* m, float* v, int step, floa
 
 // Audit: tiles flagged for audit (audit[t] = 1) were evaluated by both the
 // FP32 and the FP64 kernels; a scaled difference above `budget` marks the
-// tile sticky (FP64 from then on) and counts a miss.
+// tile sticky (FP64 from then on, starting with the next step's flags) and
+// counts a miss.
 __global__ void k_audit_compare(const float* err, const unsigned char* audit, int ntiles,
-                                float budget, unsigned char* sticky, unsigned long long* counters) {
+                                float budget, unsigned char* sticky, unsigned char* flags,
+                                unsigned long long* counters) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= ntiles || !audit[t]) return;
   atomicAdd(&counters[0], 1ull);
   if (err[t] > budget) {
     sticky[t] = 1;
+    flags[t] = 1;
     atomicAdd(&counters[1], 1ull);
   }
 }
@@ -522,9 +552,10 @@ hipError_t sel_copy_words(const void* src, void* dst, size_t bytes, hipStream_t 
 }
 
 hipError_t sel_audit_compare(const float* err, const unsigned char* audit, int ntiles, float budget,
-                             unsigned char* sticky, unsigned long long* counters, hipStream_t s) {
+                             unsigned char* sticky, unsigned char* flags,
+                             unsigned long long* counters, hipStream_t s) {
   hipLaunchKernelGGL(sel::k_audit_compare, dim3((ntiles + 255) / 256), dim3(256), 0, s, err, audit,
-                     ntiles, budget, sticky, counters);
+                     ntiles, budget, sticky, flags, counters);
   return hipGetLastError();
 }
 

@@ -52,6 +52,7 @@ struct mhd_ctx {
   // first use.
   struct Selector {
     bool on = false;
+    long long since = 0;            // selected steps enqueued since configure
     int ntiles = 0, tx = 16, ty = 15;
     float threshold = 0.5f, audit_frac = 0.01f, audit_budget = 1e-6f;
     float* params = nullptr;        // kParams FP32 weights
@@ -219,20 +220,28 @@ int status_of(mhd_ctx* c, mhd_status* st) {
   return tmp.code;
 }
 
-// One step with the learned selector: tile features -> MLP -> FP64 flags;
-// the mixed kernels take the unflagged tiles and the FP64 kernels the
-// flagged ones; a random sample of the unflagged tiles is recomputed by the
-// FP64 kernel into scratch and compared (an audit miss makes the tile FP64
-// from then on).
+// Steps between refreshes of the selector's tile features and flags (a
+// tile's character changes slowly: information crosses less than one cell
+// per step).
+constexpr int kSelectorRefresh = 8;
+
+// One step with the learned selector: tile features -> MLP -> FP64 flags
+// (every kSelectorRefresh steps); the mixed kernels take the unflagged tiles
+// and the FP64 kernels the flagged ones; a random sample of the unflagged
+// tiles is recomputed by the FP64 kernel into scratch and compared (an audit
+// miss makes the tile FP64 from then on).
 hipError_t enqueue_step_selected(mhd_ctx* c) {
   auto& S = c->sel;
   const mhd::Launchers* F64 = mhd::launchers_exact(mhd::MODE_FP64);
   const char* ctrl = (const char*)c->ctrl;
-  hipError_t e = mhd::sel_tile_features((const double*)c->u, c->plane, c->cfg.nx, c->cfg.ny, S.tx,
-                                        S.ty, ctrl + offsetof(Ctrl<double>, ch), S.feats, c->stream);
-  if (e == hipSuccess)
-    e = mhd::sel_mlp_infer(S.feats, S.ntiles, S.params, S.mean, S.istd, S.threshold, S.sticky,
-                           nullptr, S.flags, c->stream);
+  hipError_t e = hipSuccess;
+  if (S.since++ % kSelectorRefresh == 0) {
+    e = mhd::sel_tile_features((const double*)c->u, c->plane, c->cfg.nx, c->cfg.ny, S.tx, S.ty,
+                               ctrl + offsetof(Ctrl<double>, ch), S.feats, c->stream);
+    if (e == hipSuccess)
+      e = mhd::sel_mlp_infer(S.feats, S.ntiles, S.params, S.mean, S.istd, S.threshold, S.sticky,
+                             nullptr, S.flags, c->stream);
+  }
   if (e == hipSuccess)
     e = mhd::sel_audit_pick(S.flags, S.ntiles, (const int*)(ctrl + offsetof(Ctrl<double>, step)),
                             S.audit_frac, S.audit, S.counters, c->stream);
@@ -251,8 +260,8 @@ hipError_t enqueue_step_selected(mhd_ctx* c) {
     e = mhd::sel_tile_errors((const double*)c->u1, (const double*)S.u1x, c->plane, c->cfg.nx,
                              c->cfg.ny, S.tx, S.ty, S.sumsq, S.audit, S.err, c->stream);
   if (e == hipSuccess)
-    e = mhd::sel_audit_compare(S.err, S.audit, S.ntiles, S.audit_budget, S.sticky, S.counters,
-                               c->stream);
+    e = mhd::sel_audit_compare(S.err, S.audit, S.ntiles, S.audit_budget, S.sticky, S.flags,
+                               S.counters, c->stream);
   if (e == hipSuccess) e = c->L->stage(gm, 2, c->u1, c->u, c->u, c->ctrl, c->stream);
   if (e == hipSuccess) e = F64->stage(g64, 2, c->u1, c->u, c->u, c->ctrl, c->stream);
   if (e == hipSuccess) e = c->L->finish(c->ctrl, c->stream);
@@ -1295,6 +1304,7 @@ int mhd_selector_configure(mhd_ctx* c, const float* params, const float* mean, c
   S.threshold = threshold;
   S.audit_frac = audit_frac;
   S.audit_budget = audit_budget;
+  S.since = 0;
   S.on = true;
   if (c->graph) {  // captured steps of the other path
     (void)hipGraphExecDestroy(c->graph);

@@ -42,8 +42,10 @@ hipError_t sel_adam(float* params, float* grad, float* m, float* v, int step, fl
 hipError_t sel_audit_pick(const unsigned char* flags, int ntiles, const int* step, float frac,
                           unsigned char* audit, unsigned long long* counters, hipStream_t s);
 hipError_t sel_copy_words(const void* src, void* dst, size_t bytes, hipStream_t s);
+// Audited tiles whose error exceeds the budget become sticky and flagged.
 hipError_t sel_audit_compare(const float* err, const unsigned char* audit, int ntiles, float budget,
-                             unsigned char* sticky, unsigned long long* counters, hipStream_t s);
+                             unsigned char* sticky, unsigned char* flags,
+                             unsigned long long* counters, hipStream_t s);
 
 }  // namespace mhd
 

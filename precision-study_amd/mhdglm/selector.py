@@ -8,7 +8,7 @@ exceeds the budget.  The MLP is trained on the device (mhd_mlp_train); the
 model (weights, feature standardisation, threshold, budget) is kept as an
 .npz file.
 
-    python -m mhdglm.selector --out model.npz [--n 512] [--steps 800] [--every 8]
+    python -m mhdglm.selector --out model.npz [--sizes 256,512,1024,2048] [--early-n 4096]
 """
 from __future__ import annotations
 
@@ -27,8 +27,8 @@ H = 64
 # difference this many times the label budget.  Misses at the budget itself
 # are round-off variation (every tile exceeds the budget now and then).
 AUDIT_FACTOR = 10.0
-# Model trained by `python -m mhdglm.selector` (Orszag-Tang and two variants,
-# 512^2), shipped with the package.
+# Model trained by `python -m mhdglm.selector` (Orszag-Tang at 256^2 to
+# 2048^2 and early 4096^2, two variants), shipped with the package.
 DEFAULT_MODEL = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))),
                              "selector_model.npz")
 
@@ -45,9 +45,11 @@ def init_params(seed: int = 0) -> np.ndarray:
     return p
 
 
-def collect(problem="ot", params=(), n=512, steps=600, every=4, log=print):
-    """Features and pair errors of every tile, every `every` steps of a MIXED
-    run of `problem` at n x n from t=0."""
+def collect(problem="ot", params=(), n=512, steps=600, every=4, t_end=None, snapshots=24,
+            log=print):
+    """Features and pair errors of every tile of a MIXED run of `problem` at
+    n x n from t=0: every `every` steps for `steps` steps, or (t_end given)
+    at `snapshots` evenly spaced times in [0, t_end)."""
     s = Solver(SolverConfig(nx=n, ny=n, precision=PREC_MIXED))
     if params:
         from . import problem_state_ex
@@ -55,10 +57,12 @@ def collect(problem="ot", params=(), n=512, steps=600, every=4, log=print):
     else:
         s.init_problem(problem)
     X, E = [], []
-    for k in range(0, steps, every):
+    marks = ([t_end * k / snapshots for k in range(1, snapshots + 1)] if t_end is not None
+             else [None] * len(range(0, steps, every)))
+    for mark in marks:
         X.append(s.tile_features())
         E.append(s.pair_tile_errors())
-        st = s.advance(10.0, every)
+        st = s.advance(mark, -1) if mark is not None else s.advance(10.0, every)
         if st.code != 0:
             break
     s.close()
@@ -115,26 +119,41 @@ def load(path):
 TRAIN_VARIANTS = ((0.9, 0.7, 2.1, 0.4, 1.3), (1.15, 3.0, 0.5, 2.2, 4.1))
 
 
+def training_set(sizes=(256, 512, 1024, 2048), early_n=4096, log=print):
+    """Tiles of Orszag-Tang runs to t=0.5 at each size (24 snapshots), of the
+    two variants at the middle size, and of the first 600 steps at early_n.
+    Returns (X, E, E_ot): features, errors, and the errors of the plain
+    Orszag-Tang runs (the budget's reference)."""
+    parts, ot = [], []
+    for n in sizes:
+        X, E = collect("ot", n=n, t_end=0.5, snapshots=24, log=log)
+        parts.append((X, E))
+        ot.append(E)
+    mid = sizes[len(sizes) // 2]
+    for v in TRAIN_VARIANTS:
+        parts.append(collect("otv", params=v, n=mid, t_end=0.5, snapshots=24, log=log))
+    if early_n:
+        X, E = collect("ot", n=early_n, steps=600, every=100, log=log)
+        parts.append((X, E))
+        ot.append(E)
+    return (np.concatenate([x for x, _ in parts]), np.concatenate([e for _, e in parts]),
+            np.concatenate(ot))
+
+
 def main(argv=None) -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--out", required=True)
-    ap.add_argument("--n", type=int, default=512)
-    ap.add_argument("--steps", type=int, default=800)
-    ap.add_argument("--every", type=int, default=8)
-    ap.add_argument("--epochs", type=int, default=30)
+    ap.add_argument("--sizes", default="256,512,1024,2048")
+    ap.add_argument("--early-n", type=int, default=4096)
+    ap.add_argument("--epochs", type=int, default=20)
     ap.add_argument("--unsafe-fraction", type=float, default=0.05)
     args = ap.parse_args(argv)
     log = lambda m: print(m, file=sys.stderr, flush=True)
-    Xo, Eo = collect("ot", n=args.n, steps=args.steps, every=args.every, log=log)
-    parts = [(Xo, Eo)]
-    for v in TRAIN_VARIANTS:
-        parts.append(collect("otv", params=v, n=args.n, steps=args.steps, every=args.every,
-                             log=log))
+    X, E, Eot = training_set(tuple(int(x) for x in args.sizes.split(",")), args.early_n, log)
     # The budget is a quantile of the Orszag-Tang errors (the benchmark
     # problem); labels elsewhere are relative to it.
-    budget = float(np.quantile(Eo, 1.0 - args.unsafe_fraction))
-    model = fit(np.concatenate([x for x, _ in parts]), np.concatenate([e for _, e in parts]),
-                budget=budget, epochs=args.epochs, log=log)
+    budget = float(np.quantile(Eot, 1.0 - args.unsafe_fraction))
+    model = fit(X, E, budget=budget, epochs=args.epochs, log=log)
     save(model, args.out)
     log(f"selector: model written to {args.out}")
     return 0
