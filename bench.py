@@ -18,6 +18,12 @@ reported on the same JSON line:
   accuracy     -- relative L2 against FP64 of rho, p, |B| and psi after whole
                   Orszag-Tang runs to t=0.5 at --accuracy-n (default 1024^2),
                   for each form, beside the FP64 noise floor (FMA vs no FMA);
+  kh           -- Kelvin-Helmholtz (the 1024^2 benchmark variant) to t=5,
+                  FP64 vs FP64-FMA (the noise floor) and mixed: steps, rates,
+                  relative L2;
+  selector_run -- Orszag-Tang 2048^2 to t=0.5, FP64 vs FP64-FMA, mixed, and
+                  mixed with the learned tile selector: rates, FP64 tile
+                  share, audits, relative L2;
   ensemble     -- the batched workload: 64 Orszag-Tang instances (seeded
                   sizes in {128..1024}^2, B amplitude and sine phases varied)
                   advanced together to t=0.5 by the persistent work-queue
@@ -81,12 +87,12 @@ def cpu_baseline(t_end: float):
     }
 
 
-def make_solver(mhdglm, form, n, tile=-1):
-    """OT n^2 solver in an arithmetic form; "selector" is the mixed form with
-    the committed selector model deciding FP64 tiles."""
+def make_solver(mhdglm, form, n, tile=-1, problem="ot"):
+    """n^2 solver of `problem` in an arithmetic form; "selector" is the mixed
+    form with the committed selector model deciding FP64 tiles."""
     prec = mhdglm.PREC_MIXED if form == "selector" else mhdglm.PRECISIONS[form]
     solver = mhdglm.Solver(mhdglm.SolverConfig(nx=n, ny=n, precision=prec, tile=tile))
-    solver.init_problem("ot")
+    solver.init_problem(problem)
     if form == "selector":
         from mhdglm import selector
         solver.configure_selector(selector.load(selector.DEFAULT_MODEL))
@@ -125,30 +131,45 @@ def time_form(mhdglm, torch, form, n, steps, warmup, tile=-1):
     return {"value": value, "wall": wall, "kernel_ms": ms, **extra}
 
 
-def accuracy_check(mhdglm, n, forms):
-    """Whole OT n^2 runs to t=0.5 in FP64 and each form: relative L2 vs FP64."""
+def whole_run(mhdglm, form, n, problem="ot", t_end=0.5):
+    """One whole run from t=0; returns (state, status, wall seconds, selector stats)."""
+    t0 = time.perf_counter()
+    s = make_solver(mhdglm, form, n, problem=problem)
+    while True:  # chunks, so that long runs report progress
+        st = s.advance(t_end, 4000)
+        if st.code != 0 or st.done:
+            break
+        log(f"bench: {problem} {n}^2 {form}: step {st.steps}, t={st.t:.4f}")
+    wall = time.perf_counter() - t0
+    u = s.get_state(0).astype("float64")
+    sel = None
+    if form == "selector":
+        sel = s.selector_stats()
+        sel["fp64_tile_fraction"] = round(sel["fp64_tile_steps"] / max(sel["steps"] * s.selector_tiles(), 1), 5)
+    s.close()
+    log(f"bench: {problem} {n}^2 {form}: {st.steps} steps, code {st.code}, {wall:.1f} s")
+    return u, st, wall, sel
+
+
+def accuracy_check(mhdglm, n, forms, problem="ot", t_end=0.5):
+    """Whole runs of `problem` at n^2 to t_end in FP64 and each form:
+    relative L2 against FP64 of rho, p, |B|, psi, steps, and the rate."""
     from mhdglm import compare
 
-    def run(form):
-        t0 = time.perf_counter()
-        s = make_solver(mhdglm, form, n)
-        while True:  # chunks, so that long runs report progress
-            st = s.advance(0.5, 2000)
-            if st.code != 0 or st.done:
-                break
-            log(f"bench: accuracy run {form} {n}^2: step {st.steps}, t={st.t:.4f}")
-        u = s.get_state(0).astype("float64")
-        s.close()
-        log(f"bench: accuracy run {form} {n}^2: {st.steps} steps, {time.perf_counter() - t0:.1f} s")
-        return u, st
-
-    u64, s64 = run("fp64")
-    out = {"n": n, "t_end": 0.5, "fp64_steps": s64.steps, "relL2_vs_fp64": {}}
+    u64, s64, w64, _ = whole_run(mhdglm, "fp64", n, problem, t_end)
+    out = {"problem": problem, "n": n, "t_end": t_end, "fp64_steps": s64.steps,
+           "fp64_cell_updates_per_s": round(n * n * s64.steps / w64, 1), "relL2_vs_fp64": {}}
     for form in forms:
-        u, st = run(form)
+        u, st, wall, sel = whole_run(mhdglm, form, n, problem, t_end)
         errs = compare.rel_l2_state(u64, u, ("rho", "p", "|B|", "psi"))
-        out["relL2_vs_fp64"][form] = {k: float(f"{v:.3e}") for k, v in errs.items()}
-        out["relL2_vs_fp64"][form]["steps"] = st.steps
+        row = {k: float(f"{v:.3e}") for k, v in errs.items()}
+        row["steps"] = st.steps
+        row["code"] = st.code
+        row["cell_updates_per_s"] = round(n * n * st.steps / wall, 1)
+        row["speedup_vs_fp64"] = round(row["cell_updates_per_s"] / out["fp64_cell_updates_per_s"], 3)
+        if sel:
+            row["selector"] = sel
+        out["relL2_vs_fp64"][form] = row
     return out
 
 
@@ -208,6 +229,11 @@ def main():
                     help="other forms timed beside --precision ('' for none)")
     ap.add_argument("--accuracy-n", type=int, default=1024,
                     help="grid of the whole-run accuracy check (0 to skip)")
+    ap.add_argument("--kh-n", type=int, default=1024,
+                    help="Kelvin-Helmholtz (khb) whole runs, FP64 vs mixed (0 to skip)")
+    ap.add_argument("--kh-t-end", type=float, default=5.0)
+    ap.add_argument("--selector-n", type=int, default=2048,
+                    help="Orszag-Tang whole runs to t=0.5, FP64 vs selector (0 to skip)")
     ap.add_argument("--ensemble", type=int, default=64,
                     help="instances of the batched-ensemble workload (0 to skip)")
     ap.add_argument("--ensemble-form", default="mixed")
@@ -246,13 +272,23 @@ def main():
     alg_bytes = STAGE_BYTES_PER_CELL[stage] * esize * n * n
     achieved = alg_bytes / (ms[stage - 1] * 1e-3) / 1e9
     traffic = None
+    valu = None
     tfile = os.path.join(ROOT, "profiles", f"traffic_stage_ot{n}_{args.precision}.json")
     if os.path.exists(tfile):
         with open(tfile) as f:
-            traffic = json.load(f).get(f"stage{stage}_bytes_per_launch")
+            prof = json.load(f)
+        traffic = prof.get(f"stage{stage}_bytes_per_launch")
+        v = prof.get(f"stage{stage}_valu")
+        if v:
+            # The stage kernels are bound by VALU issue, not HBM: wave64 VALU
+            # instructions per CU per cycle from rocprofv3 counters, against
+            # the FP64 issue limit of 1 (transcendentals counted 4x).
+            valu = {"insts_per_launch": round(v["insts"]), "issue_per_cu_cycle": round(v["issue_per_cu_cycle"], 3),
+                    "issue_incl_trans": round(v["issue_incl_trans_quarter_rate"], 3),
+                    "fp64_issue_limit": 1.0, "source": os.path.relpath(tfile, ROOT)}
     roofline = {
         "bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-        "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+        "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic, "valu": valu,
         "kernel": f"k_stage<{stage}> ({STAGE_BYTES_PER_CELL[stage] * esize} B/cell compulsory)",
         "kernel_ms": {"stage1": round(ms[0], 4), "stage2": round(ms[1], 4),
                       "finish": round(ms[2], 4)},
@@ -274,6 +310,13 @@ def main():
                                   [f for f in ("fp64_fma", "mixed", "selector", "mixed_kahan",
                                                "fp32")
                                    if f in forms or f == "fp64_fma"])
+    kh = None
+    if args.kh_n:
+        kh = accuracy_check(mhdglm, args.kh_n, ["fp64_fma", "mixed"], problem="khb",
+                            t_end=args.kh_t_end)
+    selector_run = None
+    if args.selector_n:
+        selector_run = accuracy_check(mhdglm, args.selector_n, ["fp64_fma", "mixed", "selector"])
     ensemble = None
     if args.ensemble:
         ensemble = run_ensemble(mhdglm, torch, args.ensemble, args.ensemble_form,
@@ -294,6 +337,8 @@ def main():
                   "mixed": "f32 flux / f64 state+update", "mixed_kahan": "f32 flux / f32x2 state"}[args.precision],
         "forms": forms,
         "accuracy": accuracy,
+        "kh": kh,
+        "selector_run": selector_run,
         "ensemble": ensemble,
         "data": "synthetic: Orszag-Tang initial condition evaluated on the host",
         "config": {

@@ -206,29 +206,32 @@ __global__ __launch_bounds__(256) void k_sumsq(const double* u, size_t ps, long 
 __global__ __launch_bounds__(256) void k_tile_errors(const double* a, const double* b, size_t ps,
                                                      int nx, int ny, int tx, int ty, int tiles_x,
                                                      int ntiles, const double* sumsq9,
-                                                     const unsigned char* mask, float* err) {
+                                                     const int* list, const int* count,
+                                                     float* err) {
   __shared__ double red[4];
-  const int tile = blockIdx.x;
-  if (tile >= ntiles) return;
-  if (mask && !mask[tile]) return;
-  const int j0 = (tile / tiles_x) * ty, i0 = (tile % tiles_x) * tx;
-  const int t = threadIdx.x, i = i0 + t % tx, j = j0 + t / tx;
-  double e = 0.0;
-  if (t < tx * ty && i < nx && j < ny) {
-    const size_t g = (size_t)j * nx + i;
-    const double cells = (double)nx * (double)ny;
-    const double s_rho = sqrt(sumsq9[0] / cells) + 1e-30;
-    const double s_m = sqrt((sumsq9[1] + sumsq9[2] + sumsq9[3]) / cells) + 1e-30;
-    const double s_e = sqrt(sumsq9[4] / cells) + 1e-30;
-    const double s_b = sqrt((sumsq9[5] + sumsq9[6] + sumsq9[7]) / cells) + 1e-30;
-    for (int k = 0; k < 8; k++) {
-      const double rms = k == 0 ? s_rho : (k < 4 ? s_m : (k == 4 ? s_e : s_b));
-      const double bv = b[k * ps + g];
-      e = fmax(e, fabs(a[k * ps + g] - bv) / (fabs(bv) + rms));
+  const int entries = list ? *count : ntiles;
+  const double cells = (double)nx * (double)ny;
+  const double s_rho = sqrt(sumsq9[0] / cells) + 1e-30;
+  const double s_m = sqrt((sumsq9[1] + sumsq9[2] + sumsq9[3]) / cells) + 1e-30;
+  const double s_e = sqrt(sumsq9[4] / cells) + 1e-30;
+  const double s_b = sqrt((sumsq9[5] + sumsq9[6] + sumsq9[7]) / cells) + 1e-30;
+  for (int x = (int)blockIdx.x; x < entries; x += (int)gridDim.x) {
+    const int tile = list ? list[x] : x;
+    const int j0 = (tile / tiles_x) * ty, i0 = (tile % tiles_x) * tx;
+    const int t = threadIdx.x, i = i0 + t % tx, j = j0 + t / tx;
+    double e = 0.0;
+    if (t < tx * ty && i < nx && j < ny) {
+      const size_t g = (size_t)j * nx + i;
+      for (int k = 0; k < 8; k++) {
+        const double rms = k == 0 ? s_rho : (k < 4 ? s_m : (k == 4 ? s_e : s_b));
+        const double bv = b[k * ps + g];
+        e = fmax(e, fabs(a[k * ps + g] - bv) / (fabs(bv) + rms));
+      }
     }
+    const double m = block_reduce(e, red, Max());
+    if (threadIdx.x == 0) err[tile] = (float)m;
+    __syncthreads();
   }
-  const double m = block_reduce(e, red, Max());
-  if (threadIdx.x == 0) err[tile] = (float)m;
 }
 
 // ---------------------------------------------------------------------------
@@ -441,38 +444,67 @@ __global__ void k_adam(float* P, float* grad, float* m, float* v, int step, floa
   grad[i] = 0.f;
 }
 
-// Audit: tiles flagged for audit (audit[t] = 1) were evaluated by both the
-// FP32 and the FP64 kernels; a scaled difference above `budget` marks the
-// tile sticky (FP64 from then on, starting with the next step's flags) and
+// Audit: the listed tiles (audit list, count counts[1]) were evaluated by
+// both the FP32-flux and the FP64 kernels; a scaled difference above
+// `budget` marks the tile sticky and flagged (FP64 from then on: its
+// stage 2 of this step already, through the FP64 list, counts[0]) and
 // counts a miss.
-__global__ void k_audit_compare(const float* err, const unsigned char* audit, int ntiles,
+__global__ void k_audit_compare(const float* err, const int* audit_list, int* counts,
                                 float budget, unsigned char* sticky, unsigned char* flags,
-                                unsigned long long* counters) {
-  const int t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= ntiles || !audit[t]) return;
+                                int* fp64_list, unsigned long long* counters) {
+  const int x = blockIdx.x * blockDim.x + threadIdx.x;
+  if (x >= counts[1]) return;
+  const int t = audit_list[x];
   atomicAdd(&counters[0], 1ull);
   if (err[t] > budget) {
     sticky[t] = 1;
     flags[t] = 1;
+    fp64_list[atomicAdd(&counts[0], 1)] = t;
     atomicAdd(&counters[1], 1ull);
   }
 }
 
-// Audit sample of a step (*step read on the device, so a replayed graph
-// samples new tiles): unflagged tiles whose hash falls under `frac`.  Also
-// counts the step's FP64-routed tiles into counters[2].
-__global__ void k_audit_pick(const unsigned char* flags, int ntiles, const int* step, float frac,
-                             unsigned char* audit, unsigned long long* counters) {
-  const int t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= ntiles) return;
+// Tile lists of a step, by one block: flagged tiles -> lists[0 .. counts[0])
+// (FP64 tiles), and an audit sample of the unflagged ones (hash of tile and
+// device step counter under `frac`, so a replayed graph samples new tiles)
+// -> lists[ntiles .. ntiles + counts[1]).  Counts the FP64 tile-steps into
+// counters[2] and the steps into counters[3].
+__global__ __launch_bounds__(1024) void k_audit_pick(const unsigned char* flags, int ntiles,
+                                                     const int* step, float frac, int* lists,
+                                                     int* counts, unsigned long long* counters) {
+  __shared__ int cnt[2];
+  if (threadIdx.x < 2) cnt[threadIdx.x] = 0;
+  __syncthreads();
   const unsigned st = (unsigned)*step;
-  unsigned h = (unsigned)t * 2654435761u ^ (st * 2246822519u + 0x9E3779B9u);
-  h ^= h >> 15;
-  h *= 2246822519u;
-  h ^= h >> 13;
-  audit[t] = (!flags[t] && (float)(h & 0xFFFFFF) < frac * 16777216.f) ? 1 : 0;
-  if (flags[t]) atomicAdd(&counters[2], 1ull);
-  if (t == 0) atomicAdd(&counters[3], 1ull);
+  // 8 tiles per thread and iteration (one 8-byte load; the flag array comes
+  // from hipMalloc, so 8-byte aligned), so the loop is a few iterations deep.
+  for (int t0 = 8 * (int)threadIdx.x; t0 < ntiles; t0 += 8 * (int)blockDim.x) {
+    unsigned long long f8 = 0;
+    if (t0 + 8 <= ntiles) {
+      f8 = *(const unsigned long long*)(flags + t0);
+    } else {
+      for (int q = 0; t0 + q < ntiles; q++) f8 |= (unsigned long long)flags[t0 + q] << (8 * q);
+    }
+    for (int q = 0; q < 8 && t0 + q < ntiles; q++) {
+      const int t = t0 + q;
+      if ((f8 >> (8 * q)) & 0xFFull) {
+        lists[atomicAdd(&cnt[0], 1)] = t;
+        continue;
+      }
+      unsigned h = (unsigned)t * 2654435761u ^ (st * 2246822519u + 0x9E3779B9u);
+      h ^= h >> 15;
+      h *= 2246822519u;
+      h ^= h >> 13;
+      if ((float)(h & 0xFFFFFF) < frac * 16777216.f) lists[ntiles + atomicAdd(&cnt[1], 1)] = t;
+    }
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    counts[0] = cnt[0];
+    counts[1] = cnt[1];
+    counters[2] += (unsigned long long)cnt[0];
+    counters[3] += 1ull;
+  }
 }
 
 // Word copy (control block snapshots inside captured steps).
@@ -504,11 +536,12 @@ hipError_t sel_sumsq(const double* u, size_t plane, long long cells, double* sum
 }
 
 hipError_t sel_tile_errors(const double* a, const double* b, size_t plane, int nx, int ny, int tx,
-                           int ty, const double* sumsq9, const unsigned char* mask, float* err,
-                           hipStream_t s) {
+                           int ty, const double* sumsq9, const int* list, const int* count,
+                           float* err, hipStream_t s) {
   const int tiles_x = (nx + tx - 1) / tx, ntiles = tiles_x * ((ny + ty - 1) / ty);
-  hipLaunchKernelGGL(sel::k_tile_errors, dim3(ntiles), dim3(256), 0, s, a, b, plane, nx, ny, tx, ty,
-                     tiles_x, ntiles, sumsq9, mask, err);
+  const int grid = list ? (ntiles < 1024 ? ntiles : 1024) : ntiles;
+  hipLaunchKernelGGL(sel::k_tile_errors, dim3(grid), dim3(256), 0, s, a, b, plane, nx, ny, tx, ty,
+                     tiles_x, ntiles, sumsq9, list, count, err);
   return hipGetLastError();
 }
 
@@ -538,9 +571,9 @@ hipError_t sel_adam(float* params, float* grad, float* m, float* v, int step, fl
 }
 
 hipError_t sel_audit_pick(const unsigned char* flags, int ntiles, const int* step, float frac,
-                          unsigned char* audit, unsigned long long* counters, hipStream_t s) {
-  hipLaunchKernelGGL(sel::k_audit_pick, dim3((ntiles + 255) / 256), dim3(256), 0, s, flags, ntiles,
-                     step, frac, audit, counters);
+                          int* lists, int* counts, unsigned long long* counters, hipStream_t s) {
+  hipLaunchKernelGGL(sel::k_audit_pick, dim3(1), dim3(1024), 0, s, flags, ntiles, step, frac,
+                     lists, counts, counters);
   return hipGetLastError();
 }
 
@@ -551,11 +584,11 @@ hipError_t sel_copy_words(const void* src, void* dst, size_t bytes, hipStream_t 
   return hipGetLastError();
 }
 
-hipError_t sel_audit_compare(const float* err, const unsigned char* audit, int ntiles, float budget,
+hipError_t sel_audit_compare(const float* err, int ntiles, int* lists, int* counts, float budget,
                              unsigned char* sticky, unsigned char* flags,
                              unsigned long long* counters, hipStream_t s) {
-  hipLaunchKernelGGL(sel::k_audit_compare, dim3((ntiles + 255) / 256), dim3(256), 0, s, err, audit,
-                     ntiles, budget, sticky, flags, counters);
+  hipLaunchKernelGGL(sel::k_audit_compare, dim3((ntiles + 255) / 256), dim3(256), 0, s, err,
+                     lists + ntiles, counts, budget, sticky, flags, lists, counters);
   return hipGetLastError();
 }
 

@@ -62,7 +62,8 @@ struct mhd_ctx {
     float* err = nullptr;           // ntiles
     unsigned char* flags = nullptr;   // FP64 routing of the step
     unsigned char* sticky = nullptr;  // tiles forced to FP64 by an audit miss
-    unsigned char* audit = nullptr;   // audited tiles of the step
+    int* lists = nullptr;           // 2 x ntiles: the step's FP64 tiles, audit sample
+    int* counts = nullptr;          // lengths of the two lists
     double* sumsq = nullptr;        // error scale (per-plane sum of squares)
     unsigned long long* counters = nullptr;  // audits, misses, FP64 tile-steps, steps
     void* u1x = nullptr;            // FP64 scratch stage output
@@ -242,28 +243,35 @@ hipError_t enqueue_step_selected(mhd_ctx* c) {
       e = mhd::sel_mlp_infer(S.feats, S.ntiles, S.params, S.mean, S.istd, S.threshold, S.sticky,
                              nullptr, S.flags, c->stream);
   }
+  // FP64 tiles and audit sample as device lists: the FP64 kernels run over
+  // the lists with a small fixed grid instead of a full-grid launch.
+  int* fp64_list = S.lists;
+  int* audit_list = S.lists + S.ntiles;
+  const int* fp64_count = S.counts;
+  const int* audit_count = S.counts + 1;
   if (e == hipSuccess)
     e = mhd::sel_audit_pick(S.flags, S.ntiles, (const int*)(ctrl + offsetof(Ctrl<double>, step)),
-                            S.audit_frac, S.audit, S.counters, c->stream);
-  mhd::GridDesc gm = c->grid, g64 = c->grid, ga = c->grid;
+                            S.audit_frac, S.lists, S.counts, S.counters, c->stream);
+  mhd::GridDesc gm = c->grid;
   gm.flags = S.flags;
   gm.flag_mode = 1;
-  g64.flags = S.flags;
-  g64.flag_mode = 2;
-  ga.flags = S.audit;
-  ga.flag_mode = 2;
   if (e == hipSuccess) e = c->L->stage(gm, 1, c->u, c->u, c->u1, c->ctrl, c->stream);
-  if (e == hipSuccess) e = F64->stage(g64, 1, c->u, c->u, c->u1, c->ctrl, c->stream);
+  if (e == hipSuccess)
+    e = F64->stage_list(c->grid, 1, c->u, c->u, c->u1, c->ctrl, fp64_list, fp64_count, c->stream);
   if (e == hipSuccess) e = mhd::sel_copy_words(c->ctrl, S.ctrl_scratch, c->L->ctrl_bytes, c->stream);
-  if (e == hipSuccess) e = F64->stage(ga, 1, c->u, c->u, S.u1x, S.ctrl_scratch, c->stream);
+  if (e == hipSuccess)
+    e = F64->stage_list(c->grid, 1, c->u, c->u, S.u1x, S.ctrl_scratch, audit_list, audit_count,
+                        c->stream);
   if (e == hipSuccess)
     e = mhd::sel_tile_errors((const double*)c->u1, (const double*)S.u1x, c->plane, c->cfg.nx,
-                             c->cfg.ny, S.tx, S.ty, S.sumsq, S.audit, S.err, c->stream);
+                             c->cfg.ny, S.tx, S.ty, S.sumsq, audit_list, audit_count, S.err,
+                             c->stream);
   if (e == hipSuccess)
-    e = mhd::sel_audit_compare(S.err, S.audit, S.ntiles, S.audit_budget, S.sticky, S.flags,
-                               S.counters, c->stream);
+    e = mhd::sel_audit_compare(S.err, S.ntiles, S.lists, S.counts, S.audit_budget, S.sticky,
+                               S.flags, S.counters, c->stream);
   if (e == hipSuccess) e = c->L->stage(gm, 2, c->u1, c->u, c->u, c->ctrl, c->stream);
-  if (e == hipSuccess) e = F64->stage(g64, 2, c->u1, c->u, c->u, c->ctrl, c->stream);
+  if (e == hipSuccess)
+    e = F64->stage_list(c->grid, 2, c->u1, c->u, c->u, c->ctrl, fp64_list, fp64_count, c->stream);
   if (e == hipSuccess) e = c->L->finish(c->ctrl, c->stream);
   return e;
 }
@@ -436,7 +444,7 @@ int mhd_create(const mhd_cfg* cfg, mhd_ctx** out) {
     }
     c->grid.redo_list[0] = c->redo_lists;
     c->grid.redo_list[1] = c->redo_lists + ntiles;
-    c->grid.redo = mhd::launchers_exact(mhd::MODE_FP64)->redo;
+    c->grid.redo = mhd::launchers_exact(mhd::MODE_FP64)->stage_list;
     c->grid.redo_all = cfg->fallback == 2 ? 1 : 0;
   }
   *out = c;
@@ -449,7 +457,8 @@ void mhd_destroy(mhd_ctx* c) {
   if (c->stream) (void)hipStreamSynchronize(c->stream);
   for (void* p : {(void*)c->sel.params, (void*)c->sel.mean, (void*)c->sel.istd,
                   (void*)c->sel.feats, (void*)c->sel.err, (void*)c->sel.flags,
-                  (void*)c->sel.sticky, (void*)c->sel.audit, (void*)c->sel.sumsq,
+                  (void*)c->sel.sticky, (void*)c->sel.lists, (void*)c->sel.counts,
+                  (void*)c->sel.sumsq,
                   (void*)c->sel.counters, c->sel.u1x, c->sel.ctrl_scratch})
     if (p) (void)hipFree(p);
   if (c->u) (void)hipFree(c->u);
@@ -1191,7 +1200,8 @@ int sel_alloc(mhd_ctx* c) {
   HIP_OR_FAIL(c, hipMalloc((void**)&S.err, nt * sizeof(float)));
   HIP_OR_FAIL(c, hipMalloc((void**)&S.flags, nt));
   HIP_OR_FAIL(c, hipMalloc((void**)&S.sticky, nt));
-  HIP_OR_FAIL(c, hipMalloc((void**)&S.audit, nt));
+  HIP_OR_FAIL(c, hipMalloc((void**)&S.lists, 2 * nt * sizeof(int)));
+  HIP_OR_FAIL(c, hipMalloc((void**)&S.counts, 2 * sizeof(int)));
   HIP_OR_FAIL(c, hipMalloc((void**)&S.sumsq, 9 * sizeof(double)));
   HIP_OR_FAIL(c, hipMalloc((void**)&S.counters, 4 * sizeof(unsigned long long)));
   HIP_OR_FAIL(c, hipMalloc(&S.u1x, 9 * c->plane * sizeof(double)));
@@ -1264,8 +1274,8 @@ int mhd_pair_tile_errors(mhd_ctx* c, float* err) {
   HIP_OR_FAIL(c, mhd::sel_sumsq((const double*)S.u1x, c->plane, (long long)c->cfg.nx * c->cfg.ny,
                                 sumsq, c->stream));
   HIP_OR_FAIL(c, mhd::sel_tile_errors((const double*)c->u1, (const double*)S.u1x, c->plane,
-                                      c->cfg.nx, c->cfg.ny, S.tx, S.ty, sumsq, nullptr, S.err,
-                                      c->stream));
+                                      c->cfg.nx, c->cfg.ny, S.tx, S.ty, sumsq, nullptr, nullptr,
+                                      S.err, c->stream));
   HIP_OR_FAIL(c, hipMemcpyAsync(err, S.err, (size_t)S.ntiles * sizeof(float),
                                 hipMemcpyDeviceToHost, c->stream));
   HIP_OR_FAIL(c, hipStreamSynchronize(c->stream));

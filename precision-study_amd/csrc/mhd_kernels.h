@@ -70,12 +70,13 @@ struct GridDesc {
   // FP64 fallback of the mixed form.  With redo_list[stage-1] non-null, a
   // mixed stage tile with an unusable cell (the positivity check) writes
   // nothing and appends its index to that list (count in the control
-  // block); `redo` then re-evaluates the listed tiles with the FP64 kernel,
-  // whose own check halts the run as the FP64 form would.  redo_all queues
-  // every tile (check mode: the step is then the FP64 step).
+  // block); `redo` (the FP64 launchers' stage_list) then re-evaluates the
+  // listed tiles with the FP64 kernel, whose own check halts the run as the
+  // FP64 form would.  redo_all queues every tile (check mode: the step is
+  // then the FP64 step).
   int* redo_list[2] = {nullptr, nullptr};
   hipError_t (*redo)(const GridDesc&, int stage, const void* src, const void* uold, void* dst,
-                     void* ctrl, hipStream_t) = nullptr;
+                     void* ctrl, const int* list, const int* count, hipStream_t) = nullptr;
   int redo_all = 0;
 };
 
@@ -137,10 +138,12 @@ struct Launchers {
   hipError_t (*stage)(const GridDesc&, int stage, const void* src, const void* uold, void* dst,
                       void* ctrl, hipStream_t);
   hipError_t (*finish)(void* ctrl, hipStream_t);
-  // FP64 re-evaluation of the tiles listed in g.redo_list[stage-1] (FP64
-  // launchers only; null elsewhere).
-  hipError_t (*redo)(const GridDesc&, int stage, const void* src, const void* uold, void* dst,
-                     void* ctrl, hipStream_t);
+  // One stage over the tiles list[0 .. *count) (device list and count, read
+  // by the kernel), a fixed grid striding over the list (FP64 launchers
+  // only; null elsewhere): the mixed form's fallback and the selector's
+  // FP64 tiles.
+  hipError_t (*stage_list)(const GridDesc&, int stage, const void* src, const void* uold,
+                           void* dst, void* ctrl, const int* list, const int* count, hipStream_t);
   // Ensembles.
   size_t desc_bytes;
   // Fills descriptor i of a host array; returns the instance's tile count.

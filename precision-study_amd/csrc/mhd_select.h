@@ -22,11 +22,11 @@ hipError_t sel_tile_features(const double* u, size_t plane, int nx, int ny, int 
 hipError_t sel_sumsq(const double* u, size_t plane, long long cells, double* sumsq9, hipStream_t s);
 // err[t]: max over tile t's cells and the 8 MHD variables (psi excluded) of
 // |a-b| / (|b| + s_k), s_k the grid RMS of rho, |momentum|, E or |B| (from
-// the per-plane sums of squares sumsq9); tiles with mask[t] == 0 are
-// skipped (mask may be null).
+// the per-plane sums of squares sumsq9); every tile, or (list non-null) the
+// tiles list[0 .. *count).
 hipError_t sel_tile_errors(const double* a, const double* b, size_t plane, int nx, int ny, int tx,
-                           int ty, const double* sumsq9, const unsigned char* mask, float* err,
-                           hipStream_t s);
+                           int ty, const double* sumsq9, const int* list, const int* count,
+                           float* err, hipStream_t s);
 // prob/flags of n samples (raw features standardised with mean/istd when
 // mean is non-null); flags[s] = prob > threshold || sticky[s].
 hipError_t sel_mlp_infer(const float* raw, int n, const float* params, const float* mean,
@@ -39,11 +39,14 @@ hipError_t sel_mlp_grad(const float* X, const float* Y, const float* W, const in
                         hipStream_t s);
 hipError_t sel_adam(float* params, float* grad, float* m, float* v, int step, float lr,
                     hipStream_t s);
+// Tile lists of a selected step: lists[0 .. counts[0]) the flagged (FP64)
+// tiles, lists[ntiles .. ntiles + counts[1]) the audit sample.
 hipError_t sel_audit_pick(const unsigned char* flags, int ntiles, const int* step, float frac,
-                          unsigned char* audit, unsigned long long* counters, hipStream_t s);
+                          int* lists, int* counts, unsigned long long* counters, hipStream_t s);
 hipError_t sel_copy_words(const void* src, void* dst, size_t bytes, hipStream_t s);
-// Audited tiles whose error exceeds the budget become sticky and flagged.
-hipError_t sel_audit_compare(const float* err, const unsigned char* audit, int ntiles, float budget,
+// Audited tiles whose error exceeds the budget become sticky and flagged and
+// join the FP64 list.
+hipError_t sel_audit_compare(const float* err, int ntiles, int* lists, int* counts, float budget,
                              unsigned char* sticky, unsigned char* flags,
                              unsigned long long* counters, hipStream_t s);
 

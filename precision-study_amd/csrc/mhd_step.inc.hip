@@ -692,16 +692,18 @@ __global__ __launch_bounds__((TX * TY + 63) / 64 * 64) void k_stage(StageArgs<Po
   stage_tile<Pol, MUSCL, HLLD, STAGE, TX, TY>(a, tile, sh);
 }
 
-// FP64 fallback: re-evaluates the tiles a mixed stage kernel queued in
-// `list` (count in the control block), a fixed grid striding over the list.
+// One stage over listed tiles (the mixed form's FP64 fallback queue, the
+// selector's FP64 and audit tiles): list[0 .. *count), a fixed grid striding
+// over the list.
 template <class Pol, bool MUSCL, bool HLLD, int STAGE, int TX, int TY>
-__global__ __launch_bounds__((TX * TY + 63) / 64 * 64) void k_stage_redo(StageArgs<Pol> a,
-                                                                          const int* list) {
+__global__ __launch_bounds__((TX * TY + 63) / 64 * 64) void k_stage_list(StageArgs<Pol> a,
+                                                                          const int* list,
+                                                                          const int* count_ptr) {
   __shared__ TileShared<typename Pol::R, TX, TY> sh;
   Ctrl<typename Pol::T>* ctrl = a.ctrl;
   if (ctrl->run != RUN_ACTIVE) return;
   if (STAGE == 2 && ctrl->bad[0] != INT_MAX) return;
-  const int count = ctrl->redo_count[STAGE - 1];
+  const int count = *count_ptr;
   for (int e = (int)blockIdx.x; e < count; e += (int)gridDim.x) {
     stage_tile<Pol, MUSCL, HLLD, STAGE, TX, TY>(a, list[e], sh);
     __syncthreads();
@@ -1041,7 +1043,7 @@ static hipError_t launch_step_t(const GridDesc& g, void* u, void* u1, void* ctrl
   hipLaunchKernelGGL((k_stage<Pol, MUSCL, HLLD, 1, TX, TY>), dim3(a.ntiles), dim3(nthr), 0, s,
                      a);
   if (ev) (void)hipEventRecord(ev[1], s);
-  if (redo) (void)g.redo(g, 1, u, u, u1, ctrl, s);
+  if (redo) (void)g.redo(g, 1, u, u, u1, ctrl, g.redo_list[0], redo_counter_of<T>(ctrl, 1), s);
   a.src = (const S*)u1;
   a.uold = (const S*)u;
   a.dst = (S*)u;
@@ -1049,7 +1051,7 @@ static hipError_t launch_step_t(const GridDesc& g, void* u, void* u1, void* ctrl
   a.redo_counter = redo_counter_of<T>(ctrl, 2);
   hipLaunchKernelGGL((k_stage<Pol, MUSCL, HLLD, 2, TX, TY>), dim3(a.ntiles), dim3(nthr), 0, s,
                      a);
-  if (redo) (void)g.redo(g, 2, u1, u, u, ctrl, s);
+  if (redo) (void)g.redo(g, 2, u1, u, u, ctrl, g.redo_list[1], redo_counter_of<T>(ctrl, 2), s);
   if (ev) (void)hipEventRecord(ev[2], s);
   hipLaunchKernelGGL(k_finish<T>, dim3(1), dim3(64), 0, s, (Ctrl<T>*)ctrl);
   if (ev) (void)hipEventRecord(ev[3], s);
@@ -1075,17 +1077,20 @@ static hipError_t launch_stage_t(const GridDesc& g, int stage, const void* src, 
     hipLaunchKernelGGL((k_stage<Pol, MUSCL, HLLD, 2, TX, TY>), dim3(a.ntiles), dim3(nthr), 0, s,
                        a);
   hipError_t e = hipGetLastError();
-  if (redo && e == hipSuccess) e = g.redo(g, stage, src, uold, dst, ctrl, s);
+  if (redo && e == hipSuccess)
+    e = g.redo(g, stage, src, uold, dst, ctrl, g.redo_list[stage - 1],
+               redo_counter_of<typename Pol::T>(ctrl, stage), s);
   return e;
 }
 
-// Grid of the FP64 fallback kernel: enough blocks for a few hundred queued
-// tiles at once; an empty queue costs one short launch.
+// Grid of the list-driven stage kernel: enough blocks for a few hundred
+// listed tiles at once; an empty list costs one short launch.
 constexpr int kRedoBlocks = 512;
 
 template <class Pol, bool MUSCL, bool HLLD, int TX, int TY>
-static hipError_t launch_redo_t(const GridDesc& g, int stage, const void* src, const void* uold,
-                                void* dst, void* ctrl, hipStream_t s) {
+static hipError_t launch_list_t(const GridDesc& g, int stage, const void* src, const void* uold,
+                                void* dst, void* ctrl, const int* list, const int* count,
+                                hipStream_t s) {
   using S = typename Pol::S;
   StageArgs<Pol> a = stage_args<Pol, TX, TY>(g, ctrl);
   a.flags = nullptr;
@@ -1094,35 +1099,40 @@ static hipError_t launch_redo_t(const GridDesc& g, int stage, const void* src, c
   a.src = (const S*)src;
   a.uold = (const S*)uold;
   a.dst = (S*)dst;
-  const int* list = g.redo_list[stage - 1];
   const int grid = a.ntiles < kRedoBlocks ? a.ntiles : kRedoBlocks;
   constexpr int nthr = (TX * TY + 63) / 64 * 64;
   if (stage == 1)
-    hipLaunchKernelGGL((k_stage_redo<Pol, MUSCL, HLLD, 1, TX, TY>), dim3(grid), dim3(nthr), 0, s,
-                       a, list);
+    hipLaunchKernelGGL((k_stage_list<Pol, MUSCL, HLLD, 1, TX, TY>), dim3(grid), dim3(nthr), 0, s,
+                       a, list, count);
   else
-    hipLaunchKernelGGL((k_stage_redo<Pol, MUSCL, HLLD, 2, TX, TY>), dim3(grid), dim3(nthr), 0, s,
-                       a, list);
+    hipLaunchKernelGGL((k_stage_list<Pol, MUSCL, HLLD, 2, TX, TY>), dim3(grid), dim3(nthr), 0, s,
+                       a, list, count);
   return hipGetLastError();
 }
 
 template <class Pol>
-static hipError_t launch_redo(const GridDesc& g, int stage, const void* src, const void* uold,
-                              void* dst, void* ctrl, hipStream_t s) {
-  if (!g.redo_list[stage - 1]) return hipErrorInvalidValue;
+static hipError_t launch_list(const GridDesc& g, int stage, const void* src, const void* uold,
+                              void* dst, void* ctrl, const int* list, const int* count,
+                              hipStream_t s) {
+  if (!list || !count) return hipErrorInvalidValue;
   if (!(g.muscl && g.hlld)) {
-    if (g.muscl) return launch_redo_t<Pol, true, false, 16, 15>(g, stage, src, uold, dst, ctrl, s);
-    if (g.hlld) return launch_redo_t<Pol, false, true, 16, 15>(g, stage, src, uold, dst, ctrl, s);
-    return launch_redo_t<Pol, false, false, 16, 15>(g, stage, src, uold, dst, ctrl, s);
+    if (g.muscl)
+      return launch_list_t<Pol, true, false, 16, 15>(g, stage, src, uold, dst, ctrl, list, count, s);
+    if (g.hlld)
+      return launch_list_t<Pol, false, true, 16, 15>(g, stage, src, uold, dst, ctrl, list, count, s);
+    return launch_list_t<Pol, false, false, 16, 15>(g, stage, src, uold, dst, ctrl, list, count, s);
   }
+#define MHD_LIST_TILE(ID, TX, TY) \
+  case ID: return launch_list_t<Pol, true, true, TX, TY>(g, stage, src, uold, dst, ctrl, list, count, s);
   switch (g.tile) {
-    case TILE_16x16: return launch_redo_t<Pol, true, true, 16, 16>(g, stage, src, uold, dst, ctrl, s);
-    case TILE_32x8: return launch_redo_t<Pol, true, true, 32, 8>(g, stage, src, uold, dst, ctrl, s);
-    case TILE_16x8: return launch_redo_t<Pol, true, true, 16, 8>(g, stage, src, uold, dst, ctrl, s);
-    case TILE_32x7: return launch_redo_t<Pol, true, true, 32, 7>(g, stage, src, uold, dst, ctrl, s);
-    case TILE_16x7: return launch_redo_t<Pol, true, true, 16, 7>(g, stage, src, uold, dst, ctrl, s);
-    default: return launch_redo_t<Pol, true, true, 16, 15>(g, stage, src, uold, dst, ctrl, s);
+    MHD_LIST_TILE(TILE_16x16, 16, 16)
+    MHD_LIST_TILE(TILE_32x8, 32, 8)
+    MHD_LIST_TILE(TILE_16x8, 16, 8)
+    MHD_LIST_TILE(TILE_32x7, 32, 7)
+    MHD_LIST_TILE(TILE_16x7, 16, 7)
+    default: return launch_list_t<Pol, true, true, 16, 15>(g, stage, src, uold, dst, ctrl, list, count, s);
   }
+#undef MHD_LIST_TILE
 }
 
 template <class Pol>
@@ -1346,10 +1356,10 @@ static Launchers make_launchers(int mode) {
   L.stage = &launch_stage<Pol>;
   L.finish = &launch_finish<typename Pol::T>;
   if constexpr (std::is_same<Pol, ExactPol<double>>::value) {
-    L.redo = &launch_redo<Pol>;
+    L.stage_list = &launch_list<Pol>;
     L.ens_redo = &ens_redo<Pol>;
   } else {
-    L.redo = nullptr;
+    L.stage_list = nullptr;
     L.ens_redo = nullptr;
   }
   L.desc_bytes = sizeof(InstDesc<Pol>);
