@@ -1,6 +1,6 @@
 """Comparator and md5 manifest for the solver's .dat output.
 
-    python -m mhdglm.compare REF.dat TEST.dat        # per-variable error table
+    python -m mhdglm.compare REF.dat TEST.dat [--floor FLOOR.dat]
     python -m mhdglm.compare --manifest DIR          # md5 of every .dat under DIR
 
 The error table keeps the columns and definitions of the reference's
@@ -8,7 +8,10 @@ analysis/mhd/compare_precision.py:22-66 for the 13-column 2D layout
 (x y rho vx vy vz p Bx By Bz E divB psi): with d = ref - test over N cells,
 L1 = mean|d|, L2 = sqrt(mean d^2), Linf = max|d|, relL2 = L2 / RMS(ref).
 It adds rows for |B| and psi, which that script does not compare, and the
-same divB summary.  The manifest mirrors scripts/make_manifest.sh:4-7
+same divB summary.  With --floor, a second run that differs from REF only
+by valid rounding (e.g. FP64 with FMA contraction) gives the noise floor:
+two more columns hold relL2(REF, FLOOR) and the ratio of TEST's relL2 to it
+(chaotic flows amplify any rounding difference to that level).  The manifest mirrors scripts/make_manifest.sh:4-7
 (`md5sum` of every .dat file), plus the md5 of the data rows alone
 (`grep -v '^#' | md5sum`), which is the reference's bitwise gate.
 """
@@ -69,12 +72,22 @@ def rel_l2_state(u_ref: np.ndarray, u_test: np.ndarray, names=("rho", "p", "|B|"
     return {n: norms(fr[n], ft[n])["relL2"] for n in names}
 
 
-def format_table(res: dict, ref_name: str, test_name: str, n: int, divb=None) -> str:
+def format_table(res: dict, ref_name: str, test_name: str, n: int, divb=None, floor=None,
+                 floor_name=None) -> str:
     out = [f"# layout = 2D OT-GLM   N = {n}   cols = 13",
-           f"# ref  = {ref_name}", f"# test = {test_name}",
-           f"{'var':>4} {'L1':>12} {'L2(RMS)':>12} {'Linf':>12} {'relL2':>10}", "-" * 56]
+           f"# ref  = {ref_name}", f"# test = {test_name}"]
+    head = f"{'var':>4} {'L1':>12} {'L2(RMS)':>12} {'Linf':>12} {'relL2':>10}"
+    if floor is not None:
+        out.append(f"# floor = {floor_name}")
+        head += f" {'floor':>10} {'x floor':>8}"
+    out += [head, "-" * len(head)]
     for name, r in res.items():
-        out.append(f"{name:>4} {r['L1']:12.4e} {r['L2']:12.4e} {r['Linf']:12.4e} {r['relL2']:10.2e}")
+        line = f"{name:>4} {r['L1']:12.4e} {r['L2']:12.4e} {r['Linf']:12.4e} {r['relL2']:10.2e}"
+        if floor is not None:
+            fl = floor[name]["relL2"]
+            ratio = r["relL2"] / fl if fl > 0 else float("inf") if r["relL2"] > 0 else 1.0
+            line += f" {fl:10.2e} {ratio:8.2f}"
+        out.append(line)
     if divb is not None:
         (l1a, lia), (l1b, lib) = divb
         out += ["-" * 56, "# divB control (col 12):",
@@ -82,14 +95,22 @@ def format_table(res: dict, ref_name: str, test_name: str, n: int, divb=None) ->
     return "\n".join(out)
 
 
-def compare_files(ref_path: str, test_path: str) -> str:
+def compare_files(ref_path: str, test_path: str, floor_path: str = None) -> str:
     a, b = load_dat(ref_path), load_dat(test_path)
     if a.shape != b.shape:
         raise ValueError(f"shape mismatch {a.shape} vs {b.shape}")
-    res = compare_fields(fields_from_dat(a), fields_from_dat(b))
+    fa = fields_from_dat(a)
+    res = compare_fields(fa, fields_from_dat(b))
+    floor = None
+    if floor_path:
+        c = load_dat(floor_path)
+        if c.shape != a.shape:
+            raise ValueError(f"shape mismatch {a.shape} vs {c.shape}")
+        floor = compare_fields(fa, fields_from_dat(c))
     divb = ((float(np.mean(np.abs(a[:, 11]))), float(np.max(np.abs(a[:, 11])))),
             (float(np.mean(np.abs(b[:, 11]))), float(np.max(np.abs(b[:, 11])))))
-    return format_table(res, os.path.basename(ref_path), os.path.basename(test_path), a.shape[0], divb)
+    return format_table(res, os.path.basename(ref_path), os.path.basename(test_path), a.shape[0],
+                        divb, floor, os.path.basename(floor_path) if floor_path else None)
 
 
 def data_md5(path: str) -> str:
@@ -124,10 +145,15 @@ def main(argv=None) -> int:
     if len(argv) == 2 and argv[0] == "--manifest":
         print(manifest(argv[1]))
         return 0
+    floor = None
+    if len(argv) == 4 and argv[2] == "--floor":
+        floor = argv[3]
+        argv = argv[:2]
     if len(argv) != 2:
-        print("Usage: python -m mhdglm.compare REF.dat TEST.dat | --manifest DIR", file=sys.stderr)
+        print("Usage: python -m mhdglm.compare REF.dat TEST.dat [--floor FLOOR.dat] | "
+              "--manifest DIR", file=sys.stderr)
         return 1
-    print(compare_files(argv[0], argv[1]))
+    print(compare_files(argv[0], argv[1], floor))
     return 0
 
 

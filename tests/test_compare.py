@@ -69,3 +69,24 @@ def test_cli_entry(tmp_path):
     r = subprocess.run(["python", "-m", "mhdglm.compare"], cwd=os.path.join(ROOT, "precision-study_amd"),
                        capture_output=True, text=True)
     assert r.returncode == 1 and "Usage" in r.stderr
+
+
+def test_noise_floor_columns(tmp_path):
+    """--floor: relL2(REF, FLOOR) and TEST's multiple of it, per variable."""
+    rows = []
+    rng = np.random.default_rng(1)
+    base = rng.standard_normal((64, 13)) + 2.0
+    for name, scale in (("ref", 0.0), ("test", 4e-6), ("floor", 2e-6)):
+        a = base.copy()
+        a[:, 2:11] += scale * rng.standard_normal((64, 9))
+        p = tmp_path / f"{name}.dat"
+        np.savetxt(p, a, header="layout test", fmt="%.17e")
+        rows.append(str(p))
+    table = compare.compare_files(rows[0], rows[1], rows[2])
+    assert "# floor = floor.dat" in table
+    rho = [l for l in table.splitlines() if l.strip().startswith("rho")][0].split()
+    ref_rows = compare.compare_fields(compare.fields_from_dat(compare.load_dat(rows[0])),
+                                      compare.fields_from_dat(compare.load_dat(rows[2])))
+    assert float(rho[5]) == pytest.approx(ref_rows["rho"]["relL2"], rel=1e-2)
+    assert float(rho[6]) == pytest.approx(float(rho[4]) / float(rho[5]), rel=2e-2)
+    assert 1.0 < float(rho[6]) < 4.0
