@@ -137,6 +137,27 @@ __device__ __forceinline__ T cell_speed(const Phys<T>& c, const T P[9], bool& va
   return (m > (T)0.0) ? m : (T)0.0;
 }
 
+// Wave-uniform copies (readfirstlane): values loaded from memory that the
+// compiler cannot prove unclobbered (inside the persistent kernels' loops)
+// would otherwise occupy vector registers.
+__device__ __forceinline__ int uni(int v) { return __builtin_amdgcn_readfirstlane(v); }
+__device__ __forceinline__ unsigned long long uni(unsigned long long v) {
+  const unsigned lo = (unsigned)__builtin_amdgcn_readfirstlane((int)(unsigned)v);
+  const unsigned hi = (unsigned)__builtin_amdgcn_readfirstlane((int)(unsigned)(v >> 32));
+  return ((unsigned long long)hi << 32) | lo;
+}
+__device__ __forceinline__ size_t uni_size(size_t v) { return (size_t)uni((unsigned long long)v); }
+__device__ __forceinline__ double uni(double v) {
+  return __longlong_as_double((long long)uni((unsigned long long)__double_as_longlong(v)));
+}
+__device__ __forceinline__ float uni(float v) {
+  return __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(v)));
+}
+template <class P>
+__device__ __forceinline__ P* uni_ptr(P* p) {
+  return (P*)uni((unsigned long long)(uintptr_t)p);
+}
+
 __device__ __forceinline__ int wrap(int i, int n) {
   i %= n;
   return i < 0 ? i + n : i;
@@ -494,7 +515,7 @@ __device__ __forceinline__ void stage_tile(const StageArgs<Pol>& a, int tile,
   unsigned long long* sRed = sh.sRed;
 
   Ctrl<T>* ctrl = a.ctrl;
-  const T ch_t = ctrl->ch, dt = ctrl->dt, dx = ctrl->dx, dy = ctrl->dy;
+  const T ch_t = uni(ctrl->ch), dt = uni(ctrl->dt), dx = uni(ctrl->dx), dy = uni(ctrl->dy);
   const R ch = (R)ch_t;
   const Phys<R> c = make_phys<R>(a.tol);
   const int nx = a.nx, ny = a.ny;
@@ -703,9 +724,9 @@ __global__ __launch_bounds__((TX * TY + 63) / 64 * 64) void k_stage_list(StageAr
   Ctrl<typename Pol::T>* ctrl = a.ctrl;
   if (ctrl->run != RUN_ACTIVE) return;
   if (STAGE == 2 && ctrl->bad[0] != INT_MAX) return;
-  const int count = *count_ptr;
+  const int count = __builtin_amdgcn_readfirstlane(*count_ptr);
   for (int e = (int)blockIdx.x; e < count; e += (int)gridDim.x) {
-    stage_tile<Pol, MUSCL, HLLD, STAGE, TX, TY>(a, list[e], sh);
+    stage_tile<Pol, MUSCL, HLLD, STAGE, TX, TY>(a, __builtin_amdgcn_readfirstlane(list[e]), sh);
     __syncthreads();
   }
 }
@@ -877,27 +898,28 @@ template <class Pol, int STAGE>
 __device__ __forceinline__ StageArgs<Pol> inst_args(const EnsArgs<Pol>& e, const InstDesc<Pol>& d,
                                                     int first_item) {
   StageArgs<Pol> a;
-  a.plane = d.plane;
-  a.nx = d.nx;
-  a.ny = d.ny;
-  a.tiles_x = d.tiles_x;
-  a.ntiles = d.ntiles;
-  a.pow2 = d.pow2;
-  a.inv_dx = d.inv_dx;
-  a.inv_dy = d.inv_dy;
+  a.plane = uni_size(d.plane);
+  a.nx = uni(d.nx);
+  a.ny = uni(d.ny);
+  a.tiles_x = uni(d.tiles_x);
+  a.ntiles = uni(d.ntiles);
+  a.pow2 = uni(d.pow2);
+  a.inv_dx = uni(d.inv_dx);
+  a.inv_dy = uni(d.inv_dy);
   a.tol = e.tol;
   a.alpha = e.alpha;
   a.lowbeta_ratio = e.lowbeta_ratio;
-  a.ctrl = d.ctrl;
+  a.ctrl = uni_ptr(d.ctrl);
   a.flags = nullptr;
   a.flag_mode = 0;
   a.redo_list = e.redo_items[STAGE - 1];
   a.redo_counter = (int*)&e.queue[16 + STAGE - 1];
   a.redo_base = first_item;
   a.redo_all = e.redo_all;
-  a.src = STAGE == 1 ? d.u : d.u1;
-  a.uold = d.u;
-  a.dst = STAGE == 1 ? d.u1 : d.u;
+  const auto u = uni_ptr(d.u), u1 = uni_ptr(d.u1);
+  a.src = STAGE == 1 ? u : u1;
+  a.uold = u;
+  a.dst = STAGE == 1 ? u1 : u;
   return a;
 }
 
@@ -910,40 +932,66 @@ __device__ __forceinline__ int find_instance(const int* prefix, int n, int item)
   return lo;
 }
 
+// Waves per SIMD the persistent kernels are register-budgeted for: the
+// single-grid kernels' occupancy, which LDS allows (FP64: 64 KB per 4-wave
+// block, 2 blocks per CU; FP32 flux: 32 KB, 4 blocks).  Without the target
+// the queue loop costs the FP64 kernel one register too many (257) and
+// half its occupancy.
+template <class Pol> struct QueueWaves { static constexpr int value = sizeof(typename Pol::R) == 8 ? 2 : 4; };
+
 template <class Pol, bool MUSCL, bool HLLD, int STAGE, int TX, int TY>
-__global__ __launch_bounds__((TX * TY + 63) / 64 * 64) void k_stage_queue(EnsArgs<Pol> e) {
+__global__ __launch_bounds__((TX * TY + 63) / 64 * 64)
+__attribute__((amdgpu_waves_per_eu(QueueWaves<Pol>::value))) void k_stage_queue(EnsArgs<Pol> e) {
   __shared__ TileShared<typename Pol::R, TX, TY> sh;
-  __shared__ int s_item;
+  __shared__ int s_item, s_count;
+  // Items are claimed kClaim at a time (consecutive tiles, nearly always of
+  // one instance): fewer queue atomics per tile at the cost of at most
+  // kClaim - 1 tiles of imbalance at the end of a stage.
+  constexpr int kClaim = 4;
   unsigned* q = e.queue + (STAGE - 1) * 8;
   const int home = (int)(blockIdx.x & 7);
   int chunk = 0;
+  // Item range of the instance of the previous item (block-uniform), so the
+  // binary search over instances runs only when an item leaves it.
+  int cur = 0, cur_lo = 0, cur_hi = 0;
   for (;;) {
     if (threadIdx.x == 0) {
-      int item = -1;
+      int item = -1, cnt = 0;
       while (chunk < 8) {
         const int x = (home + chunk) & 7;
         const int lo = (int)((long long)e.total * x / 8);
         const int hi = (int)((long long)e.total * (x + 1) / 8);
-        const unsigned k = atomicAdd(&q[x], 1u);
+        const unsigned k = atomicAdd(&q[x], (unsigned)kClaim);
         if ((long long)lo + k < (long long)hi) {
           item = lo + (int)k;
+          cnt = hi - item < kClaim ? hi - item : kClaim;
           break;
         }
         chunk++;
       }
       s_item = item;
+      s_count = cnt;
     }
     __syncthreads();
-    const int item = s_item;
+    // Block-uniform: readfirstlane keeps the item, the instance lookup and
+    // the descriptor in scalar registers.
+    const int first_item = uni(s_item), count = uni(s_count);
     __syncthreads();
-    if (item < 0) break;  // every chunk drained: every block gets here
-    const int inst = find_instance(e.prefix, e.n, item);
-    const InstDesc<Pol>& d = e.desc[inst];
-    if (d.ctrl->run != RUN_ACTIVE) continue;
-    if (STAGE == 2 && d.ctrl->bad[0] != INT_MAX) continue;
-    const int first = e.prefix[inst];
-    stage_tile<Pol, MUSCL, HLLD, STAGE, TX, TY>(inst_args<Pol, STAGE>(e, d, first), item - first,
-                                                sh);
+    if (first_item < 0) break;  // every chunk drained: every block gets here
+    for (int c = 0; c < count; c++) {
+      const int item = first_item + c;
+      if (item < cur_lo || item >= cur_hi) {
+        cur = uni(find_instance(e.prefix, e.n, item));
+        cur_lo = uni(e.prefix[cur]);
+        cur_hi = uni(e.prefix[cur + 1]);
+      }
+      const InstDesc<Pol>& d = e.desc[cur];
+      if (uni(d.ctrl->run) != RUN_ACTIVE) continue;
+      if (STAGE == 2 && uni(d.ctrl->bad[0]) != INT_MAX) continue;
+      stage_tile<Pol, MUSCL, HLLD, STAGE, TX, TY>(inst_args<Pol, STAGE>(e, d, cur_lo),
+                                                  item - cur_lo, sh);
+      __syncthreads();  // LDS is reused by the next item
+    }
   }
 }
 
@@ -953,9 +1001,9 @@ template <class Pol, bool MUSCL, bool HLLD, int STAGE, int TX, int TY>
 __global__ __launch_bounds__((TX * TY + 63) / 64 * 64) void k_queue_redo(EnsArgs<Pol> e,
                                                                           const int* list) {
   __shared__ TileShared<typename Pol::R, TX, TY> sh;
-  const int count = (int)e.queue[16 + STAGE - 1];
+  const int count = __builtin_amdgcn_readfirstlane((int)e.queue[16 + STAGE - 1]);
   for (int x = (int)blockIdx.x; x < count; x += (int)gridDim.x) {
-    const int item = list[x];
+    const int item = __builtin_amdgcn_readfirstlane(list[x]);
     const int inst = find_instance(e.prefix, e.n, item);
     const InstDesc<Pol>& d = e.desc[inst];
     if (d.ctrl->run != RUN_ACTIVE) continue;

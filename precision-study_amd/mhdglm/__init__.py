@@ -22,7 +22,8 @@ from dataclasses import dataclass
 import numpy as np
 
 PKG_ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
-LIB_PATH = os.path.join(PKG_ROOT, "lib", "libmhdglm.so")
+# The in-tree build; MHDGLM_LIB names another build of the same library.
+LIB_PATH = os.environ.get("MHDGLM_LIB") or os.path.join(PKG_ROOT, "lib", "libmhdglm.so")
 CLI_PATH = os.path.join(PKG_ROOT, "bin", "mhd2d_glm_amd")
 
 PREC_FP64 = 0
