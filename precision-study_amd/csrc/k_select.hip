@@ -203,11 +203,24 @@ __global__ __launch_bounds__(256) void k_sumsq(const double* u, size_t ps, long 
 // zero) is measured against its vector.  The GLM potential psi (plane 8) is
 // left out: it has no scale of its own (it starts at zero and stays near the
 // round-off level of div B).
+// With `audit` non-null the listed tiles are audits: a tile whose error
+// exceeds the budget becomes sticky and flagged (FP64 from then on: its
+// stage 2 of this step already, through the FP64 list audit->fp64_list,
+// count audit->fp64_count) and counts a miss; every audit is counted.
+struct AuditOut {
+  float budget;
+  unsigned char* sticky;
+  unsigned char* flags;
+  int* fp64_list;
+  int* fp64_count;
+  unsigned long long* counters;
+};
+
 __global__ __launch_bounds__(256) void k_tile_errors(const double* a, const double* b, size_t ps,
                                                      int nx, int ny, int tx, int ty, int tiles_x,
                                                      int ntiles, const double* sumsq9,
                                                      const int* list, const int* count,
-                                                     float* err) {
+                                                     float* err, AuditOut audit) {
   __shared__ double red[4];
   const int entries = list ? *count : ntiles;
   const double cells = (double)nx * (double)ny;
@@ -229,7 +242,18 @@ __global__ __launch_bounds__(256) void k_tile_errors(const double* a, const doub
       }
     }
     const double m = block_reduce(e, red, Max());
-    if (threadIdx.x == 0) err[tile] = (float)m;
+    if (threadIdx.x == 0) {
+      err[tile] = (float)m;
+      if (audit.counters) {
+        atomicAdd(&audit.counters[0], 1ull);
+        if ((float)m > audit.budget) {
+          audit.sticky[tile] = 1;
+          audit.flags[tile] = 1;
+          audit.fp64_list[atomicAdd(audit.fp64_count, 1)] = tile;
+          atomicAdd(&audit.counters[1], 1ull);
+        }
+      }
+    }
     __syncthreads();
   }
 }
@@ -444,36 +468,21 @@ __global__ void k_adam(float* P, float* grad, float* m, float* v, int step, floa
   grad[i] = 0.f;
 }
 
-// Audit: the listed tiles (audit list, count counts[1]) were evaluated by
-// both the FP32-flux and the FP64 kernels; a scaled difference above
-// `budget` marks the tile sticky and flagged (FP64 from then on: its
-// stage 2 of this step already, through the FP64 list, counts[0]) and
-// counts a miss.
-__global__ void k_audit_compare(const float* err, const int* audit_list, int* counts,
-                                float budget, unsigned char* sticky, unsigned char* flags,
-                                int* fp64_list, unsigned long long* counters) {
-  const int x = blockIdx.x * blockDim.x + threadIdx.x;
-  if (x >= counts[1]) return;
-  const int t = audit_list[x];
-  atomicAdd(&counters[0], 1ull);
-  if (err[t] > budget) {
-    sticky[t] = 1;
-    flags[t] = 1;
-    fp64_list[atomicAdd(&counts[0], 1)] = t;
-    atomicAdd(&counters[1], 1ull);
-  }
-}
-
 // Tile lists of a step, by one block: flagged tiles -> lists[0 .. counts[0])
 // (FP64 tiles), and an audit sample of the unflagged ones (hash of tile and
 // device step counter under `frac`, so a replayed graph samples new tiles)
 // -> lists[ntiles .. ntiles + counts[1]).  Counts the FP64 tile-steps into
 // counters[2] and the steps into counters[3].
+// Also copies the control block (ctrl_words words) into ctrl_copy: the
+// audits' snapshot.
 __global__ __launch_bounds__(1024) void k_audit_pick(const unsigned char* flags, int ntiles,
                                                      const int* step, float frac, int* lists,
-                                                     int* counts, unsigned long long* counters) {
+                                                     int* counts, unsigned long long* counters,
+                                                     const unsigned* ctrl, unsigned* ctrl_copy,
+                                                     int ctrl_words) {
   __shared__ int cnt[2];
   if (threadIdx.x < 2) cnt[threadIdx.x] = 0;
+  for (int w = threadIdx.x; w < ctrl_words; w += blockDim.x) ctrl_copy[w] = ctrl[w];
   __syncthreads();
   const unsigned st = (unsigned)*step;
   // 8 tiles per thread and iteration (one 8-byte load; the flag array comes
@@ -541,7 +550,19 @@ hipError_t sel_tile_errors(const double* a, const double* b, size_t plane, int n
   const int tiles_x = (nx + tx - 1) / tx, ntiles = tiles_x * ((ny + ty - 1) / ty);
   const int grid = list ? (ntiles < 1024 ? ntiles : 1024) : ntiles;
   hipLaunchKernelGGL(sel::k_tile_errors, dim3(grid), dim3(256), 0, s, a, b, plane, nx, ny, tx, ty,
-                     tiles_x, ntiles, sumsq9, list, count, err);
+                     tiles_x, ntiles, sumsq9, list, count, err, sel::AuditOut{});
+  return hipGetLastError();
+}
+
+hipError_t sel_audit(const double* a, const double* b, size_t plane, int nx, int ny, int tx, int ty,
+                     const double* sumsq9, int* lists, int* counts, float budget,
+                     unsigned char* sticky, unsigned char* flags, float* err,
+                     unsigned long long* counters, hipStream_t s) {
+  const int tiles_x = (nx + tx - 1) / tx, ntiles = tiles_x * ((ny + ty - 1) / ty);
+  const int grid = ntiles < 1024 ? ntiles : 1024;
+  sel::AuditOut o{budget, sticky, flags, lists, counts, counters};
+  hipLaunchKernelGGL(sel::k_tile_errors, dim3(grid), dim3(256), 0, s, a, b, plane, nx, ny, tx, ty,
+                     tiles_x, ntiles, sumsq9, lists + ntiles, counts + 1, err, o);
   return hipGetLastError();
 }
 
@@ -571,9 +592,11 @@ hipError_t sel_adam(float* params, float* grad, float* m, float* v, int step, fl
 }
 
 hipError_t sel_audit_pick(const unsigned char* flags, int ntiles, const int* step, float frac,
-                          int* lists, int* counts, unsigned long long* counters, hipStream_t s) {
+                          int* lists, int* counts, unsigned long long* counters, const void* ctrl,
+                          void* ctrl_copy, size_t ctrl_bytes, hipStream_t s) {
   hipLaunchKernelGGL(sel::k_audit_pick, dim3(1), dim3(1024), 0, s, flags, ntiles, step, frac,
-                     lists, counts, counters);
+                     lists, counts, counters, (const unsigned*)ctrl, (unsigned*)ctrl_copy,
+                     (int)((ctrl_bytes + 3) / 4));
   return hipGetLastError();
 }
 
@@ -584,12 +607,6 @@ hipError_t sel_copy_words(const void* src, void* dst, size_t bytes, hipStream_t 
   return hipGetLastError();
 }
 
-hipError_t sel_audit_compare(const float* err, int ntiles, int* lists, int* counts, float budget,
-                             unsigned char* sticky, unsigned char* flags,
-                             unsigned long long* counters, hipStream_t s) {
-  hipLaunchKernelGGL(sel::k_audit_compare, dim3((ntiles + 255) / 256), dim3(256), 0, s, err,
-                     lists + ntiles, counts, budget, sticky, flags, lists, counters);
-  return hipGetLastError();
-}
+
 
 }  // namespace mhd

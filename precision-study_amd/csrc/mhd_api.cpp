@@ -243,35 +243,46 @@ hipError_t enqueue_step_selected(mhd_ctx* c) {
       e = mhd::sel_mlp_infer(S.feats, S.ntiles, S.params, S.mean, S.istd, S.threshold, S.sticky,
                              nullptr, S.flags, c->stream);
   }
-  // FP64 tiles and audit sample as device lists: the FP64 kernels run over
-  // the lists with a small fixed grid instead of a full-grid launch.
+  // FP64 tiles and audit sample as device lists (from the pick, which also
+  // snapshots the control block for the audits).  One list launch per stage
+  // evaluates in FP64, over a small fixed grid: the flagged tiles, the tiles
+  // the mixed stage queued for its fallback, and (stage 1) the audit tiles
+  // into scratch.  The mixed stages only queue their fallback tiles here.
   int* fp64_list = S.lists;
   int* audit_list = S.lists + S.ntiles;
   const int* fp64_count = S.counts;
   const int* audit_count = S.counts + 1;
-  if (e == hipSuccess)
-    e = mhd::sel_audit_pick(S.flags, S.ntiles, (const int*)(ctrl + offsetof(Ctrl<double>, step)),
-                            S.audit_frac, S.lists, S.counts, S.counters, c->stream);
   mhd::GridDesc gm = c->grid;
   gm.flags = S.flags;
   gm.flag_mode = 1;
+  gm.redo = nullptr;
+  auto redo_count = [&](int stage) {
+    return (const int*)((char*)c->ctrl + offsetof(Ctrl<double>, redo_count) +
+                        (stage - 1) * sizeof(int));
+  };
+  if (e == hipSuccess)
+    e = mhd::sel_audit_pick(S.flags, S.ntiles, (const int*)(ctrl + offsetof(Ctrl<double>, step)),
+                            S.audit_frac, S.lists, S.counts, S.counters, c->ctrl, S.ctrl_scratch,
+                            c->L->ctrl_bytes, c->stream);
   if (e == hipSuccess) e = c->L->stage(gm, 1, c->u, c->u, c->u1, c->ctrl, c->stream);
+  if (e == hipSuccess) {
+    const mhd::ListJob jobs[3] = {
+        {c->u, c->u, c->u1, c->ctrl, fp64_list, fp64_count},
+        {c->u, c->u, S.u1x, S.ctrl_scratch, audit_list, audit_count},
+        {c->u, c->u, c->u1, c->ctrl, c->grid.redo_list[0], redo_count(1)}};
+    e = F64->stage_listn(c->grid, 1, jobs, c->grid.redo_list[0] ? 3 : 2, c->stream);
+  }
   if (e == hipSuccess)
-    e = F64->stage_list(c->grid, 1, c->u, c->u, c->u1, c->ctrl, fp64_list, fp64_count, c->stream);
-  if (e == hipSuccess) e = mhd::sel_copy_words(c->ctrl, S.ctrl_scratch, c->L->ctrl_bytes, c->stream);
-  if (e == hipSuccess)
-    e = F64->stage_list(c->grid, 1, c->u, c->u, S.u1x, S.ctrl_scratch, audit_list, audit_count,
-                        c->stream);
-  if (e == hipSuccess)
-    e = mhd::sel_tile_errors((const double*)c->u1, (const double*)S.u1x, c->plane, c->cfg.nx,
-                             c->cfg.ny, S.tx, S.ty, S.sumsq, audit_list, audit_count, S.err,
-                             c->stream);
-  if (e == hipSuccess)
-    e = mhd::sel_audit_compare(S.err, S.ntiles, S.lists, S.counts, S.audit_budget, S.sticky,
-                               S.flags, S.counters, c->stream);
+    e = mhd::sel_audit((const double*)c->u1, (const double*)S.u1x, c->plane, c->cfg.nx, c->cfg.ny,
+                       S.tx, S.ty, S.sumsq, S.lists, S.counts, S.audit_budget, S.sticky, S.flags,
+                       S.err, S.counters, c->stream);
   if (e == hipSuccess) e = c->L->stage(gm, 2, c->u1, c->u, c->u, c->ctrl, c->stream);
-  if (e == hipSuccess)
-    e = F64->stage_list(c->grid, 2, c->u1, c->u, c->u, c->ctrl, fp64_list, fp64_count, c->stream);
+  if (e == hipSuccess) {
+    const mhd::ListJob jobs[2] = {
+        {c->u1, c->u, c->u, c->ctrl, fp64_list, fp64_count},
+        {c->u1, c->u, c->u, c->ctrl, c->grid.redo_list[1], redo_count(2)}};
+    e = F64->stage_listn(c->grid, 2, jobs, c->grid.redo_list[1] ? 2 : 1, c->stream);
+  }
   if (e == hipSuccess) e = c->L->finish(c->ctrl, c->stream);
   return e;
 }

@@ -117,6 +117,17 @@ struct EnsLaunch {
 // fallback counts.
 constexpr int kEnsQueueWords = 18;
 
+// One segment of a list-driven stage launch: buffers, control block, and a
+// device list of tile indices with its device count (list null: empty).
+struct ListJob {
+  const void* src;
+  const void* uold;
+  void* dst;
+  void* ctrl;
+  const int* list;
+  const int* count;
+};
+
 // Launchers of one arithmetic form.  All launches go to `stream`; none
 // synchronises.  State buffers hold `state_planes` planes of `plane`
 // elements of `state_elem` bytes (SoA), cell (i,j) at j*nx+i.
@@ -144,6 +155,9 @@ struct Launchers {
   // FP64 tiles.
   hipError_t (*stage_list)(const GridDesc&, int stage, const void* src, const void* uold,
                            void* dst, void* ctrl, const int* list, const int* count, hipStream_t);
+  // The same over 1..3 segments in one launch (FP64 launchers only).
+  hipError_t (*stage_listn)(const GridDesc&, int stage, const ListJob* jobs, int njobs,
+                            hipStream_t);
   // Ensembles.
   size_t desc_bytes;
   // Fills descriptor i of a host array; returns the instance's tile count.

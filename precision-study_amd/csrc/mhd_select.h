@@ -40,15 +40,20 @@ hipError_t sel_mlp_grad(const float* X, const float* Y, const float* W, const in
 hipError_t sel_adam(float* params, float* grad, float* m, float* v, int step, float lr,
                     hipStream_t s);
 // Tile lists of a selected step: lists[0 .. counts[0]) the flagged (FP64)
-// tiles, lists[ntiles .. ntiles + counts[1]) the audit sample.
+// tiles, lists[ntiles .. ntiles + counts[1]) the audit sample; also copies
+// the control block into ctrl_copy (the audits' snapshot).
 hipError_t sel_audit_pick(const unsigned char* flags, int ntiles, const int* step, float frac,
-                          int* lists, int* counts, unsigned long long* counters, hipStream_t s);
+                          int* lists, int* counts, unsigned long long* counters, const void* ctrl,
+                          void* ctrl_copy, size_t ctrl_bytes, hipStream_t s);
+// Errors of the audit list (a: FP32-flux, b: FP64 stage output) as
+// sel_tile_errors; tiles above `budget` become sticky and flagged and join
+// the FP64 list (lists[0 .. counts[0])).
+hipError_t sel_audit(const double* a, const double* b, size_t plane, int nx, int ny, int tx, int ty,
+                     const double* sumsq9, int* lists, int* counts, float budget,
+                     unsigned char* sticky, unsigned char* flags, float* err,
+                     unsigned long long* counters, hipStream_t s);
 hipError_t sel_copy_words(const void* src, void* dst, size_t bytes, hipStream_t s);
-// Audited tiles whose error exceeds the budget become sticky and flagged and
-// join the FP64 list.
-hipError_t sel_audit_compare(const float* err, int ntiles, int* lists, int* counts, float budget,
-                             unsigned char* sticky, unsigned char* flags,
-                             unsigned long long* counters, hipStream_t s);
+
 
 }  // namespace mhd
 

@@ -714,19 +714,45 @@ __global__ __launch_bounds__((TX * TY + 63) / 64 * 64) void k_stage(StageArgs<Po
 }
 
 // One stage over listed tiles (the mixed form's FP64 fallback queue, the
-// selector's FP64 and audit tiles): list[0 .. *count), a fixed grid striding
-// over the list.
+// selector's FP64 and audit tiles): up to kListSegs segments, each with its
+// own arguments (destination, control block) and tile list
+// list[0 .. *count); a fixed grid strides over all of them, so a selected
+// step's FP64 tiles, fallback queue and audit tiles cost one launch.
+constexpr int kListSegs = 3;
+template <class Pol>
+struct ListSeg {
+  StageArgs<Pol> a;
+  const int* list;   // null: empty segment
+  const int* count;
+};
+
+template <class Pol, int STAGE>
+__device__ __forceinline__ int seg_count(const ListSeg<Pol>& g) {
+  if (!g.list) return 0;
+  const Ctrl<typename Pol::T>* ctrl = g.a.ctrl;
+  if (uni(ctrl->run) != RUN_ACTIVE) return 0;
+  if (STAGE == 2 && uni(ctrl->bad[0]) != INT_MAX) return 0;
+  return uni(*g.count);
+}
+
+template <class Pol>
+struct ListSegs {
+  ListSeg<Pol> seg[kListSegs];
+};
+
 template <class Pol, bool MUSCL, bool HLLD, int STAGE, int TX, int TY>
-__global__ __launch_bounds__((TX * TY + 63) / 64 * 64) void k_stage_list(StageArgs<Pol> a,
-                                                                          const int* list,
-                                                                          const int* count_ptr) {
+__global__ __launch_bounds__((TX * TY + 63) / 64 * 64) void k_stage_list(ListSegs<Pol> g) {
   __shared__ TileShared<typename Pol::R, TX, TY> sh;
-  Ctrl<typename Pol::T>* ctrl = a.ctrl;
-  if (ctrl->run != RUN_ACTIVE) return;
-  if (STAGE == 2 && ctrl->bad[0] != INT_MAX) return;
-  const int count = __builtin_amdgcn_readfirstlane(*count_ptr);
-  for (int e = (int)blockIdx.x; e < count; e += (int)gridDim.x) {
-    stage_tile<Pol, MUSCL, HLLD, STAGE, TX, TY>(a, __builtin_amdgcn_readfirstlane(list[e]), sh);
+  int n[kListSegs], total = 0;
+#pragma unroll
+  for (int k = 0; k < kListSegs; k++) {
+    n[k] = seg_count<Pol, STAGE>(g.seg[k]);
+    total += n[k];
+  }
+  for (int e = (int)blockIdx.x; e < total; e += (int)gridDim.x) {
+    int k = 0, x = e;
+    while (x >= n[k]) x -= n[k++];
+    stage_tile<Pol, MUSCL, HLLD, STAGE, TX, TY>(g.seg[k].a, uni(g.seg[k].list[x]), sh);
     __syncthreads();
   }
 }
@@ -1111,11 +1137,14 @@ static hipError_t launch_stage_t(const GridDesc& g, int stage, const void* src, 
                                  void* dst, void* ctrl, hipStream_t s) {
   using S = typename Pol::S;
   StageArgs<Pol> a = stage_args<Pol, TX, TY>(g, ctrl);
-  const bool redo = Pol::kRedo && g.redo && g.redo_list[stage - 1];
+  // With a queue and no `redo` launcher the caller re-evaluates the queue
+  // itself (the selector folds it into its FP64 list launch).
+  const bool queue = Pol::kRedo && g.redo_list[stage - 1];
+  const bool redo = queue && g.redo;
   a.src = (const S*)src;
   a.uold = (const S*)uold;
   a.dst = (S*)dst;
-  if (redo) a.redo_list = g.redo_list[stage - 1];
+  if (queue) a.redo_list = g.redo_list[stage - 1];
   a.redo_counter = redo_counter_of<typename Pol::T>(ctrl, stage);
   constexpr int nthr = (TX * TY + 63) / 64 * 64;
   if (stage == 1)
@@ -1136,26 +1165,54 @@ static hipError_t launch_stage_t(const GridDesc& g, int stage, const void* src, 
 constexpr int kRedoBlocks = 512;
 
 template <class Pol, bool MUSCL, bool HLLD, int TX, int TY>
-static hipError_t launch_list_t(const GridDesc& g, int stage, const void* src, const void* uold,
-                                void* dst, void* ctrl, const int* list, const int* count,
+static hipError_t launch_list_t(const GridDesc& g, int stage, const ListJob* jobs, int njobs,
                                 hipStream_t s) {
   using S = typename Pol::S;
-  StageArgs<Pol> a = stage_args<Pol, TX, TY>(g, ctrl);
-  a.flags = nullptr;
-  a.flag_mode = 0;
-  a.redo_all = 0;
-  a.src = (const S*)src;
-  a.uold = (const S*)uold;
-  a.dst = (S*)dst;
-  const int grid = a.ntiles < kRedoBlocks ? a.ntiles : kRedoBlocks;
+  auto seg = [&](const ListJob& j) {
+    ListSeg<Pol> q;
+    q.a = stage_args<Pol, TX, TY>(g, j.ctrl);
+    q.a.flags = nullptr;
+    q.a.flag_mode = 0;
+    q.a.redo_all = 0;
+    q.a.src = (const S*)j.src;
+    q.a.uold = (const S*)j.uold;
+    q.a.dst = (S*)j.dst;
+    q.list = j.list;
+    q.count = j.count;
+    return q;
+  };
+  const ListJob none{nullptr, nullptr, nullptr, jobs[0].ctrl, nullptr, nullptr};
+  ListSegs<Pol> segs;
+  for (int k = 0; k < kListSegs; k++) segs.seg[k] = seg(k < njobs ? jobs[k] : none);
+  const int ntiles = segs.seg[0].a.ntiles;
+  const int grid = ntiles < kRedoBlocks ? ntiles : kRedoBlocks;
   constexpr int nthr = (TX * TY + 63) / 64 * 64;
   if (stage == 1)
     hipLaunchKernelGGL((k_stage_list<Pol, MUSCL, HLLD, 1, TX, TY>), dim3(grid), dim3(nthr), 0, s,
-                       a, list, count);
+                       segs);
   else
     hipLaunchKernelGGL((k_stage_list<Pol, MUSCL, HLLD, 2, TX, TY>), dim3(grid), dim3(nthr), 0, s,
-                       a, list, count);
+                       segs);
   return hipGetLastError();
+}
+
+template <class Pol>
+static hipError_t launch_listn(const GridDesc& g, int stage, const ListJob* jobs, int njobs,
+                               hipStream_t s) {
+  if (njobs < 1 || njobs > kListSegs) return hipErrorInvalidValue;
+  if (!(g.muscl && g.hlld)) {
+    if (g.muscl) return launch_list_t<Pol, true, false, 16, 15>(g, stage, jobs, njobs, s);
+    if (g.hlld) return launch_list_t<Pol, false, true, 16, 15>(g, stage, jobs, njobs, s);
+    return launch_list_t<Pol, false, false, 16, 15>(g, stage, jobs, njobs, s);
+  }
+  switch (g.tile) {
+    case TILE_16x16: return launch_list_t<Pol, true, true, 16, 16>(g, stage, jobs, njobs, s);
+    case TILE_32x8: return launch_list_t<Pol, true, true, 32, 8>(g, stage, jobs, njobs, s);
+    case TILE_16x8: return launch_list_t<Pol, true, true, 16, 8>(g, stage, jobs, njobs, s);
+    case TILE_32x7: return launch_list_t<Pol, true, true, 32, 7>(g, stage, jobs, njobs, s);
+    case TILE_16x7: return launch_list_t<Pol, true, true, 16, 7>(g, stage, jobs, njobs, s);
+    default: return launch_list_t<Pol, true, true, 16, 15>(g, stage, jobs, njobs, s);
+  }
 }
 
 template <class Pol>
@@ -1163,24 +1220,8 @@ static hipError_t launch_list(const GridDesc& g, int stage, const void* src, con
                               void* dst, void* ctrl, const int* list, const int* count,
                               hipStream_t s) {
   if (!list || !count) return hipErrorInvalidValue;
-  if (!(g.muscl && g.hlld)) {
-    if (g.muscl)
-      return launch_list_t<Pol, true, false, 16, 15>(g, stage, src, uold, dst, ctrl, list, count, s);
-    if (g.hlld)
-      return launch_list_t<Pol, false, true, 16, 15>(g, stage, src, uold, dst, ctrl, list, count, s);
-    return launch_list_t<Pol, false, false, 16, 15>(g, stage, src, uold, dst, ctrl, list, count, s);
-  }
-#define MHD_LIST_TILE(ID, TX, TY) \
-  case ID: return launch_list_t<Pol, true, true, TX, TY>(g, stage, src, uold, dst, ctrl, list, count, s);
-  switch (g.tile) {
-    MHD_LIST_TILE(TILE_16x16, 16, 16)
-    MHD_LIST_TILE(TILE_32x8, 32, 8)
-    MHD_LIST_TILE(TILE_16x8, 16, 8)
-    MHD_LIST_TILE(TILE_32x7, 32, 7)
-    MHD_LIST_TILE(TILE_16x7, 16, 7)
-    default: return launch_list_t<Pol, true, true, 16, 15>(g, stage, src, uold, dst, ctrl, list, count, s);
-  }
-#undef MHD_LIST_TILE
+  const ListJob j{src, uold, dst, ctrl, list, count};
+  return launch_listn<Pol>(g, stage, &j, 1, s);
 }
 
 template <class Pol>
@@ -1405,9 +1446,11 @@ static Launchers make_launchers(int mode) {
   L.finish = &launch_finish<typename Pol::T>;
   if constexpr (std::is_same<Pol, ExactPol<double>>::value) {
     L.stage_list = &launch_list<Pol>;
+    L.stage_listn = &launch_listn<Pol>;
     L.ens_redo = &ens_redo<Pol>;
   } else {
     L.stage_list = nullptr;
+    L.stage_listn = nullptr;
     L.ens_redo = nullptr;
   }
   L.desc_bytes = sizeof(InstDesc<Pol>);
