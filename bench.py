@@ -232,6 +232,8 @@ def main():
     ap.add_argument("--kh-n", type=int, default=1024,
                     help="Kelvin-Helmholtz (khb) whole runs, FP64 vs mixed (0 to skip)")
     ap.add_argument("--kh-t-end", type=float, default=5.0)
+    ap.add_argument("--kh-forms", default="fp64_fma,mixed",
+                    help="forms of the Kelvin-Helmholtz whole runs, against FP64")
     ap.add_argument("--selector-n", type=int, default=2048,
                     help="Orszag-Tang whole runs to t=0.5, FP64 vs selector (0 to skip)")
     ap.add_argument("--ensemble", type=int, default=64,
@@ -312,8 +314,8 @@ def main():
                                    if f in forms or f == "fp64_fma"])
     kh = None
     if args.kh_n:
-        kh = accuracy_check(mhdglm, args.kh_n, ["fp64_fma", "mixed"], problem="khb",
-                            t_end=args.kh_t_end)
+        kh = accuracy_check(mhdglm, args.kh_n, [f for f in args.kh_forms.split(",") if f],
+                            problem="khb", t_end=args.kh_t_end)
     selector_run = None
     if args.selector_n:
         selector_run = accuracy_check(mhdglm, args.selector_n, ["fp64_fma", "mixed", "selector"])
