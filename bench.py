@@ -31,8 +31,10 @@ reported on the same JSON line:
   roofline     -- the dominant (stage) kernel's algorithmic HBM bytes per
                   launch over its average duration measured here with HIP
                   events on the solver stream, against 8 TB/s;
+  large        -- FP64 and mixed steps at 8192^2 (4.8 GB per FP64 state);
   cpu_baseline -- the reference CPU build (oracle/_ref, serial, one core) on
-                  a bounded sample of the same workload at 128^2.
+                  a bounded sample of Orszag-Tang at 512^2 (the first ~50
+                  steps).
 """
 import argparse
 import json
@@ -57,9 +59,9 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-def cpu_baseline(t_end: float):
+def cpu_baseline(n: int, t_end: float):
     """Reference CPU build (or the oracle if the reference build is absent),
-    OT 128^2 FP64 to t_end, one thread."""
+    OT n^2 FP64 to t_end (a bounded sample of the whole run), one thread."""
     ref = os.path.join(ROOT, "oracle", "_ref", "mhd2d_glm_cpu_fp64")
     kind = "reference"
     if not os.path.exists(ref):
@@ -69,7 +71,7 @@ def cpu_baseline(t_end: float):
         return None
     with tempfile.TemporaryDirectory() as d:
         t0 = time.perf_counter()
-        r = subprocess.run([ref, "128", "128", str(t_end), "0.25", "muscl", "hlld", "0.18"],
+        r = subprocess.run([ref, str(n), str(n), str(t_end), "0.25", "muscl", "hlld", "0.18"],
                            env=dict(os.environ, OUTDIR=d), capture_output=True, text=True,
                            timeout=600)
         wall = time.perf_counter() - t0
@@ -82,7 +84,7 @@ def cpu_baseline(t_end: float):
         "unit": "cell-updates/s",
         "cores": 1,
         "kind": kind,
-        "sample": f"Orszag-Tang 128x128 FP64 MUSCL+HLLD to t={t_end} ({steps} SSPRK2 steps, "
+        "sample": f"Orszag-Tang {n}x{n} FP64 MUSCL+HLLD to t={t_end} ({steps} SSPRK2 steps, "
                   f"{float(m.group(1)):.1f} s loop, {wall:.1f} s process), serial g++ -O2 build",
     }
 
@@ -223,7 +225,10 @@ def main():
     ap.add_argument("--precision", choices=("fp64", "fp32", "mixed", "mixed_kahan", "fp64_fma"),
                     default="fp64")
     ap.add_argument("--tile", type=int, default=-1)
-    ap.add_argument("--cpu-t-end", type=float, default=0.2)
+    ap.add_argument("--cpu-n", type=int, default=512)
+    ap.add_argument("--cpu-t-end", type=float, default=0.006)
+    ap.add_argument("--large-n", type=int, default=8192,
+                    help="also time FP64 and mixed steps at this size (0 to skip)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--forms", default="fp64_fma,mixed,selector,mixed_kahan,fp32",
                     help="other forms timed beside --precision ('' for none)")
@@ -323,7 +328,13 @@ def main():
     if args.ensemble:
         ensemble = run_ensemble(mhdglm, torch, args.ensemble, args.ensemble_form,
                                 args.ensemble_t_end)
-    cpu = None if args.no_cpu else cpu_baseline(args.cpu_t_end)
+    large = None
+    if args.large_n:
+        large = {"n": args.large_n, "steps": 20, "warmup": 5}
+        for form in ("fp64", "mixed"):
+            large[form] = round(time_form(mhdglm, torch, form, args.large_n, 20, 5)["value"], 1)
+        large["mixed_vs_fp64_speedup"] = round(large["mixed"] / large["fp64"], 3)
+    cpu = None if args.no_cpu else cpu_baseline(args.cpu_n, args.cpu_t_end)
     out = {
         "metric": METRIC,
         "value": value,
@@ -339,6 +350,7 @@ def main():
                   "mixed": "f32 flux / f64 state+update", "mixed_kahan": "f32 flux / f32x2 state"}[args.precision],
         "forms": forms,
         "accuracy": accuracy,
+        "large": large,
         "kh": kh,
         "selector_run": selector_run,
         "ensemble": ensemble,
