@@ -13,8 +13,9 @@ the reference's "Mcell-steps/s" unit (mhd2d_glm_cpu.cpp:589-591) times 1e6.
 reported on the same JSON line:
   forms        -- throughput of the other arithmetic forms on the same
                   workload (FP64 with FMA, mixed, mixed with the learned
-                  tile selector, mixed + compensated, FP32), the
-                  mixed-vs-FP64 speedup and the selector's FP64 tile share;
+                  tile selector, mixed + compensated, FP32, and the
+                  reference's stage-level mixed form), the mixed-vs-FP64
+                  speedup and the selector's FP64 tile share;
   accuracy     -- relative L2 against FP64 of rho, p, |B| and psi after whole
                   Orszag-Tang runs to t=0.5 at --accuracy-n (default 1024^2),
                   for each form, beside the FP64 noise floor (FMA vs no FMA);
@@ -222,7 +223,8 @@ def main():
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--n", type=int, default=4096)
-    ap.add_argument("--precision", choices=("fp64", "fp32", "mixed", "mixed_kahan", "fp64_fma"),
+    ap.add_argument("--precision",
+                    choices=("fp64", "fp32", "mixed", "mixed_kahan", "fp64_fma", "ref_mixed"),
                     default="fp64")
     ap.add_argument("--tile", type=int, default=-1)
     ap.add_argument("--cpu-n", type=int, default=512)
@@ -230,7 +232,7 @@ def main():
     ap.add_argument("--large-n", type=int, default=8192,
                     help="also time FP64 and mixed steps at this size (0 to skip)")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--forms", default="fp64_fma,mixed,selector,mixed_kahan,fp32",
+    ap.add_argument("--forms", default="fp64_fma,mixed,selector,mixed_kahan,fp32,ref_mixed",
                     help="other forms timed beside --precision ('' for none)")
     ap.add_argument("--accuracy-n", type=int, default=1024,
                     help="grid of the whole-run accuracy check (0 to skip)")
@@ -311,11 +313,13 @@ def main():
         forms["mixed_vs_fp64_speedup"] = round(forms["mixed"] / forms.get("fp64", value), 3)
     if "mixed" in forms and "fp64_fma" in forms:
         forms["mixed_vs_fp64_fma_speedup"] = round(forms["mixed"] / forms["fp64_fma"], 3)
+    if "ref_mixed" in forms and args.precision == "fp64":
+        forms["ref_mixed_vs_fp64_speedup"] = round(forms["ref_mixed"] / value, 3)
     accuracy = None
     if args.accuracy_n:
         accuracy = accuracy_check(mhdglm, args.accuracy_n,
                                   [f for f in ("fp64_fma", "mixed", "selector", "mixed_kahan",
-                                               "fp32")
+                                               "fp32", "ref_mixed")
                                    if f in forms or f == "fp64_fma"])
     kh = None
     if args.kh_n:
@@ -347,7 +351,8 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": {"fp64": "f64", "fp32": "f32", "fp64_fma": "f64",
-                  "mixed": "f32 flux / f64 state+update", "mixed_kahan": "f32 flux / f32x2 state"}[args.precision],
+                  "mixed": "f32 flux / f64 state+update", "mixed_kahan": "f32 flux / f32x2 state",
+                  "ref_mixed": "f32 recovery+recon+rhs / f64 flux+state"}[args.precision],
         "forms": forms,
         "accuracy": accuracy,
         "large": large,

@@ -36,11 +36,16 @@ extern "C" {
  *                update accumulation, dt and time.
  *   MIXED_KAHAN  as MIXED but the state is FP32 + FP32 compensation.
  *   FP64_FMA     FP64 with FMA contraction (relative L2 <= 1e-12 of FP64).
- * No reference counterpart exists for the last three except the reference's
- * own mixed build with the opposite allocation (mhd2d_glm_gpu_mixed.cu:4-8). */
+ *   REF_MIXED    the reference's stage-level mixed build
+ *                (mhd2d_glm_gpu_mixed.cu:1-10): state, Riemann fluxes and
+ *                SSPRK2 updates in FP64; recovery, CFL reductions, face
+ *                reconstruction, rhs differences and psi damping in FP32.
+ *                MUSCL only (first order keeps FP64 face states there).
+ * MIXED, MIXED_KAHAN and FP64_FMA have no reference counterpart; MIXED has
+ * the opposite allocation to REF_MIXED. */
 enum mhd_precision {
   MHD_PREC_FP64 = 0, MHD_PREC_FP32 = 1, MHD_PREC_MIXED = 2,
-  MHD_PREC_MIXED_KAHAN = 3, MHD_PREC_FP64_FMA = 4
+  MHD_PREC_MIXED_KAHAN = 3, MHD_PREC_FP64_FMA = 4, MHD_PREC_REF_MIXED = 5
 };
 
 /* Run configuration.  Mirrors the reference CLI

@@ -4,6 +4,7 @@
 // oracle on the same inputs.  Arrays are AoS: state n, variable k at n*9+k.
 // Each function exists for double (_f64) and float (_f32).
 #include "mhd_oracle.hpp"
+#include "mhd_refmixed.hpp"
 #include "problems.hpp"
 
 #include <cstdint>
@@ -114,3 +115,57 @@ using namespace oracle;
 
 ORACLE_CAPI(double, f64)
 ORACLE_CAPI(float, f32)
+
+// FP32 stage functions of the reference-mixed restatement, batched (AoS 9).
+extern "C" void orc_refmixed_cons_to_prim_batch(int n, const float* U, float* W) {
+  for (int s = 0; s < n; s++) refmixed::cons_to_prim(U + (size_t)s * NV, W + (size_t)s * NV);
+}
+extern "C" void orc_refmixed_prim_to_cons_batch(int n, const float* W, float* U) {
+  for (int s = 0; s < n; s++) refmixed::prim_to_cons(W + (size_t)s * NV, U + (size_t)s * NV);
+}
+extern "C" void orc_refmixed_fast_speed_batch(int n, const float* W, int dir, float* out) {
+  for (int s = 0; s < n; s++) {
+    const float* w = W + (size_t)s * NV;
+    out[s] = refmixed::fast_speed(w, dir ? w[BY] : w[BX]);
+  }
+}
+extern "C" void orc_refmixed_recon_batch(int n, const float* Pm, const float* P0,
+                                         const float* Pp, float* Pl, float* Pr) {
+  for (int s = 0; s < n; s++) {
+    const size_t o = (size_t)s * NV;
+    refmixed::recon(Pm + o, P0 + o, Pp + o, Pl + o, Pr + o);
+  }
+}
+
+// The reference's stage-level mixed build (mhd_refmixed.hpp), MUSCL: the
+// same arguments as orc_advance_f64.
+extern "C" int orc_refmixed_advance(int nx, int ny, double* u, int max_steps, double t_end,
+                                    double cfl, double alpha, int hlld, double tol,
+                                    double* t_io, int* steps_out, uint64_t* counters4,
+                                    int* bad_ij, double* last_ch) {
+  const Consts<double> c(tol);
+  Grid<double> U(nx, ny), U1(nx, ny);
+  for (size_t q = 0; q < U.u.size(); q++) U.u[q] = u[q];
+  const double dx = (1.0 - 0.0) / (double)nx, dy = (1.0 - 0.0) / (double)ny;
+  FallbackStats st;
+  double t = *t_io;
+  int steps = 0, code = 0;
+  while (t < t_end && steps < max_steps) {
+    StepResult r = refmixed::ssprk2_step(c, U, U1, dx, dy, cfl, alpha, hlld != 0, t, t_end, &st,
+                                         last_ch);
+    if (r.code != 0) {
+      code = r.code;
+      if (bad_ij) { bad_ij[0] = r.bad_i; bad_ij[1] = r.bad_j; }
+      break;
+    }
+    steps++;
+  }
+  for (size_t q = 0; q < U.u.size(); q++) u[q] = U.u[q];
+  *t_io = t;
+  if (steps_out) *steps_out = steps;
+  if (counters4) {
+    counters4[0] = st.x_evals; counters4[1] = st.y_evals;
+    counters4[2] = st.degenerate; counters4[3] = st.nonphysical;
+  }
+  return code;
+}

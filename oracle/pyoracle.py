@@ -133,6 +133,50 @@ class Oracle:
                    "bad": (int(bad[0]), int(bad[1])), "ch": ch[0]}
 
 
+    def refmixed_f32(self, name, *arrays, direction=None):
+        """FP32 stage function of the reference-mixed restatement over a
+        batch: cons_to_prim, prim_to_cons, fast_speed (direction) or recon
+        (Pm, P0, Pp -> Pl, Pr)."""
+        P, I = ctypes.c_void_p, ctypes.c_int
+        a = [np.ascontiguousarray(x, dtype=np.float32) for x in arrays]
+        n = a[0].shape[0]
+        f = getattr(self.lib, f"orc_refmixed_{name}_batch")
+        if name == "fast_speed":
+            out = np.empty(n, dtype=np.float32)
+            f.argtypes = [I, P, I, P]
+            f(n, _p(a[0]), int(direction), _p(out))
+            return out
+        if name == "recon":
+            Pl, Pr = np.empty_like(a[0]), np.empty_like(a[0])
+            f.argtypes = [I, P, P, P, P, P]
+            f(n, _p(a[0]), _p(a[1]), _p(a[2]), _p(Pl), _p(Pr))
+            return Pl, Pr
+        out = np.empty_like(a[0])
+        f.argtypes = [I, P, P]
+        f(n, _p(a[0]), _p(out))
+        return out
+
+    def refmixed_advance(self, u_aos, max_steps, t_end=0.5, cfl=0.25, alpha=0.18, hlld=True,
+                         tol=1e-4, t0=0.0):
+        """The reference's stage-level mixed build (MUSCL), FP64 state:
+        advances a copy of u_aos; returns (u, info dict) as advance()."""
+        ny, nx, _ = u_aos.shape
+        a = np.array(u_aos, dtype=np.float64, copy=True, order="C")
+        t = np.array([t0], dtype=np.float64)
+        steps = np.zeros(1, dtype=np.int32)
+        cnt = np.zeros(4, dtype=np.uint64)
+        bad = np.full(2, -1, dtype=np.int32)
+        ch = np.zeros(1, dtype=np.float64)
+        f = self.lib.orc_refmixed_advance
+        D, I, P = ctypes.c_double, ctypes.c_int, ctypes.c_void_p
+        f.argtypes = [I, I, P, I, D, D, D, I, D, P, P, P, P, P]
+        f.restype = I
+        code = f(nx, ny, _p(a), max_steps, t_end, cfl, alpha, int(hlld), tol, _p(t), _p(steps),
+                 _p(cnt), _p(bad), _p(ch))
+        return a, {"code": code, "t": t[0], "steps": int(steps[0]), "counters": cnt,
+                   "bad": (int(bad[0]), int(bad[1])), "ch": ch[0]}
+
+
 class RefHeaders:
     """The reference's own physics functions (oracle/_ref, when built)."""
 

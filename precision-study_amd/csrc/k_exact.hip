@@ -1,6 +1,7 @@
 // k_exact.hip -- FP64 and FP32 step kernels with FP contraction off and IEEE
 // division/square root: bitwise the reference CPU build of the same
-// precision.  Also hosts the operator-level batch kernels.
+// precision; the reference's stage-level mixed form under the same flags.
+// Also hosts the operator-level batch kernels.
 #define MHD_NS exact
 #include "mhd_step.inc.hip"
 
@@ -9,7 +10,10 @@ namespace mhd {
 const Launchers* launchers_exact(int mode) {
   static const Launchers fp64 = exact::make_launchers<exact::ExactPol<double>>(MODE_FP64);
   static const Launchers fp32 = exact::make_launchers<exact::ExactPol<float>>(MODE_FP32);
-  return mode == MODE_FP32 ? &fp32 : &fp64;
+  static const Launchers ref_mixed = exact::make_launchers<exact::RefMixPol>(MODE_REF_MIXED);
+  if (mode == MODE_FP32) return &fp32;
+  if (mode == MODE_REF_MIXED) return &ref_mixed;
+  return &fp64;
 }
 
 // Operator-level batch kernels (one state pair per thread).

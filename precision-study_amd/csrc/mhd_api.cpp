@@ -85,6 +85,7 @@ const mhd::Launchers* launchers_for(int prec) {
     case MHD_PREC_MIXED: return mhd::launchers_mixed(mhd::MODE_MIXED);
     case MHD_PREC_MIXED_KAHAN: return mhd::launchers_mixed(mhd::MODE_MIXED_KAHAN);
     case MHD_PREC_FP64_FMA: return mhd::launchers_fma();
+    case MHD_PREC_REF_MIXED: return mhd::launchers_exact(mhd::MODE_REF_MIXED);
     default: return nullptr;
   }
 }
@@ -118,6 +119,7 @@ void init_ctrl(Ctrl<R>* h, const mhd_cfg& cfg) {
   h->bad[0] = INT_MAX;
   h->bad[1] = INT_MAX;
   h->fail_cell = -1;
+  h->rate_f32 = cfg.precision == MHD_PREC_REF_MIXED ? 1 : 0;
 }
 
 template <class R>
@@ -407,6 +409,7 @@ int mhd_create(const mhd_cfg* cfg, mhd_ctx** out) {
   if (!(cfg->cfl > 0.0 && cfg->cfl < 1.0)) return 1;
   if (!(cfg->alpha > 0.0)) return 1;
   if (!launchers_for(cfg->precision)) return 1;
+  if (cfg->precision == MHD_PREC_REF_MIXED && !cfg->muscl) return 1;
   if ((long long)cfg->nx * cfg->ny > (long long)INT_MAX) return 1;
   mhd_ctx* c = new mhd_ctx();
   c->cfg = *cfg;
@@ -960,6 +963,7 @@ int mhd_ensemble_create(const mhd_cfg* cfg, int n, const int* nx, const int* ny,
   if (!cfg || n < 1 || !nx || !ny) return 1;
   if (!(cfg->cfl > 0.0 && cfg->cfl < 1.0) || !(cfg->alpha > 0.0)) return 1;
   if (!launchers_for(cfg->precision)) return 1;
+  if (cfg->precision == MHD_PREC_REF_MIXED && !cfg->muscl) return 1;
   for (int i = 0; i < n; i++)
     if (nx[i] < 2 || ny[i] < 2 || (long long)nx[i] * ny[i] > (long long)INT_MAX) return 1;
   mhd_ensemble* e = new mhd_ensemble();

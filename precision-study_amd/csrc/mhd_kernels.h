@@ -18,7 +18,8 @@ enum Mode {
   MODE_MIXED = 2,        // FP32 flux/reconstruction/GLM, FP64 state + update + time
   MODE_MIXED_KAHAN = 3,  // as MIXED with an FP32 + FP32 compensated state
   MODE_FP64_FMA = 4,     // FP64 with FMA contraction
-  MODE_COUNT = 5
+  MODE_REF_MIXED = 5,    // the reference's stage-level mixed build (FP64 flux + update)
+  MODE_COUNT = 6
 };
 enum RunState { RUN_ACTIVE = 0, RUN_DONE = 1, RUN_HALTED = 2 };
 
@@ -45,6 +46,7 @@ struct Ctrl {
   // over the run.
   int redo_count[2];
   unsigned long long redo_tiles;
+  int rate_f32;  // CFL rate in float (the reference-mixed form)
   // Per-block partial maxima of the signal speed, spread over kSpeedSlots
   // addresses (block b uses slot b % kSpeedSlots) so that tens of thousands
   // of blocks do not serialise on one atomic; folded into speed_acc by the
@@ -170,7 +172,7 @@ struct Launchers {
   hipError_t (*ens_redo)(const EnsLaunch&, int stage, hipStream_t);
 };
 
-const Launchers* launchers_exact(int mode);  // MODE_FP64, MODE_FP32
+const Launchers* launchers_exact(int mode);  // MODE_FP64, MODE_FP32, MODE_REF_MIXED
 const Launchers* launchers_fma();            // MODE_FP64_FMA
 const Launchers* launchers_mixed(int mode);  // MODE_MIXED, MODE_MIXED_KAHAN
 

@@ -137,6 +137,81 @@ __device__ __forceinline__ T cell_speed(const Phys<T>& c, const T P[9], bool& va
   return (m > (T)0.0) ? m : (T)0.0;
 }
 
+// FP32 stage functions of the reference's stage-level mixed build
+// (mhd2d_glm_gpu_mixed.cu:54-125), with its fmaxf floors.
+namespace refmix {
+constexpr float kRhoFloor = 1.0e-12f, kPFloor = 1.0e-12f;
+constexpr float kGamma = (float)(5.0 / 3.0);
+
+// f32ConsToPrim9 (mhd2d_glm_gpu_mixed.cu:61-74)
+__device__ __forceinline__ void cons_to_prim(const float U[9], float P[9]) {
+  const float rho = fmaxf(U[RHO], kRhoFloor);
+  const float vx = U[MX] / rho, vy = U[MY] / rho, vz = U[MZ] / rho;
+  const float bx = U[BX], by = U[BY], bz = U[BZ];
+  const float v2 = vx * vx + vy * vy + vz * vz;
+  const float b2 = bx * bx + by * by + bz * bz;
+  const float p = (kGamma - 1.0f) * (U[EN] - 0.5f * rho * v2 - 0.5f * b2);
+  P[0] = rho;
+  P[1] = vx;
+  P[2] = vy;
+  P[3] = vz;
+  P[4] = p;
+  P[5] = bx;
+  P[6] = by;
+  P[7] = bz;
+  P[8] = U[PSI];
+}
+
+// f32PrimToCons9 (mhd2d_glm_gpu_mixed.cu:75-87)
+__device__ __forceinline__ void prim_to_cons(const float P[9], float U[9]) {
+  const float rho = P[0], vx = P[1], vy = P[2], vz = P[3];
+  const float bx = P[5], by = P[6], bz = P[7];
+  U[RHO] = rho;
+  U[MX] = rho * vx;
+  U[MY] = rho * vy;
+  U[MZ] = rho * vz;
+  U[EN] = P[4] / (kGamma - 1.0f) + 0.5f * rho * (vx * vx + vy * vy + vz * vz) +
+          0.5f * (bx * bx + by * by + bz * bz);
+  U[BX] = bx;
+  U[BY] = by;
+  U[BZ] = bz;
+  U[PSI] = P[8];
+}
+
+// f32FastSpeed (mhd2d_glm_gpu_mixed.cu:88-99)
+__device__ __forceinline__ float fast_speed(const float P[9], float bn) {
+  const float rho = fmaxf(P[0], kRhoFloor);
+  const float p = fmaxf(P[4], kPFloor);
+  const float bx = P[5], by = P[6], bz = P[7];
+  const float a2 = kGamma * p / rho;
+  const float b2 = (bx * bx + by * by + bz * bz) / rho;
+  const float can2 = bn * bn / rho;
+  const float s = a2 + b2;
+  const float disc = fmaxf(s * s - 4.0f * a2 * can2, 0.0f);
+  return sqrtf(0.5f * (s + sqrtf(disc)));
+}
+
+// f32LoadPrim (mhd2d_glm_gpu_mixed.cu:121-125)
+__device__ __forceinline__ void load_prim(const double U9[9], float P[9]) {
+  float Uf[9];
+#pragma unroll
+  for (int k = 0; k < 9; k++) Uf[k] = (float)U9[k];
+  cons_to_prim(Uf, P);
+}
+
+// Signal speed of one cell as speedKernel takes it
+// (mhd2d_glm_gpu_mixed.cu:256-280): (double)fmaxf(sx, sy).
+__device__ __forceinline__ double cell_speed(const double U9[9], bool& valid) {
+  float P[9];
+  load_prim(U9, P);
+  const float sx = fabsf(P[1]) + fast_speed(P, P[5]);
+  const float sy = fabsf(P[2]) + fast_speed(P, P[6]);
+  const float m = fmaxf(sx, sy);
+  valid = (m == m);
+  return (m > 0.0f) ? (double)m : 0.0;
+}
+}  // namespace refmix
+
 // Wave-uniform copies (readfirstlane): values loaded from memory that the
 // compiler cannot prove unclobbered (inside the persistent kernels' loops)
 // would otherwise occupy vector registers.
@@ -268,30 +343,54 @@ __device__ __forceinline__ void ff_add(float ah, float al, float bh, float bl, f
 // Plain FP64 / FP32: everything in R, state stored in R.
 template <class R_>
 struct ExactPol {
-  using R = R_;
-  using T = R_;
-  using S = R_;
+  using R = R_;   // Riemann flux
+  using RP = R_;  // primitives / reconstruction (LDS)
+  using RS = R_;  // face fluxes as stored in LDS
+  using T = R_;   // time, dt, c_h, update
+  using S = R_;   // state in HBM
   static constexpr int kPlanes = 9;
   static constexpr int kMode = sizeof(R_) == 8 ? 0 : 1;
   static constexpr bool kRedo = false;
+  static constexpr bool kRefMixed = false;
+};
+// The reference's stage-level mixed build (mhd2d_glm_gpu_mixed.cu:1-10):
+// state, Riemann (GLM-HLLD/HLL) fluxes and the SSPRK2 updates in FP64;
+// recovery, CFL reductions, face reconstruction, the rhs flux differences
+// and the psi damping in FP32 (each stage's inputs cast to float, results
+// cast back).
+struct RefMixPol {
+  using R = double;
+  using RP = float;
+  using RS = float;  // the rhs reads (float)F (mhd2d_glm_gpu_mixed.cu:355-375)
+  using T = double;
+  using S = double;
+  static constexpr int kPlanes = 9;
+  static constexpr bool kRedo = false;
+  static constexpr bool kRefMixed = true;
 };
 // Mixed: FP32 reconstruction / HLLD / GLM; FP64 state, update accumulation,
 // time; FP64 pressure recovery where FP32 would cancel.
 struct MixedPol {
   using R = float;
+  using RP = float;
+  using RS = float;
   using T = double;
   using S = double;
   static constexpr int kPlanes = 9;
   static constexpr bool kRedo = true;  // FP64 fallback of unusable tiles
+  static constexpr bool kRefMixed = false;
 };
 // Compensated: as Mixed, but the state is FP32 hi + FP32 lo (planes 0-8 and
 // 9-17), accumulated with float-float additions.
 struct KahanPol {
   using R = float;
   using T = double;
+  using RP = float;
+  using RS = float;
   using S = float;
   static constexpr int kPlanes = 18;
   static constexpr bool kRedo = false;
+  static constexpr bool kRefMixed = false;
 };
 
 template <class Pol>
@@ -329,6 +428,16 @@ struct CellIO<ExactPol<R>> {
     for (int k = 0; k < 9; k++) v[k] = p[k * ps + g];
   }
   __device__ __forceinline__ R psi() const { return v[PSI]; }
+};
+
+template <>
+struct CellIO<RefMixPol> {
+  double v[9];
+  __device__ __forceinline__ void load(const double* p, size_t ps, size_t g) {
+#pragma unroll
+    for (int k = 0; k < 9; k++) v[k] = p[k * ps + g];
+  }
+  __device__ __forceinline__ double psi() const { return v[PSI]; }
 };
 
 template <>
@@ -398,6 +507,28 @@ __device__ __forceinline__ void recover_mixed(const Phys<float>& c, const CellIO
   }
 }
 
+// Reference-mixed recovery: the FP64 state cast to float, f32ConsToPrim9
+// (mhd2d_glm_gpu_mixed.cu:121-125).  First order is not offered by this
+// policy (the reference keeps FP64 face states there).
+template <bool MUSCL>
+__device__ __forceinline__ void recover(const Phys<float>&, const CellIO<RefMixPol>& u, float,
+                                        float P[9]) {
+  refmix::load_prim(u.v, P);
+}
+
+// Face state from a reconstructed primitive: prim->cons in RP, widened to
+// the flux type.
+template <class Pol, class RP, class R>
+__device__ __forceinline__ void face_cons(const Phys<RP>& c, const RP Pf[9], R U[9]) {
+  RP Uf[9];
+  if constexpr (Pol::kRefMixed)
+    refmix::prim_to_cons(Pf, Uf);  // f32PrimToCons9, then (Real) (:194-200)
+  else
+    prim_to_cons(c, Pf, Uf);
+#pragma unroll
+  for (int k = 0; k < 9; k++) U[k] = (R)Uf[k];
+}
+
 template <bool MUSCL>
 __device__ __forceinline__ void recover(const Phys<float>& c, const CellIO<MixedPol>& u,
                                         float ratio, float P[9]) {
@@ -434,6 +565,29 @@ __device__ __forceinline__ void update_cell(const StageArgs<ExactPol<R>>& a,
       out[k] = src.v[k] + dt * rhs;
     else
       out[k] = (R)0.5 * old.v[k] + (R)0.5 * (src.v[k] + dt * rhs);
+  }
+#pragma unroll
+  for (int k = 0; k < 9; k++) a.dst[k * a.plane + g] = out[k];
+}
+
+// Reference-mixed policy: rhs in FP32 with the psi damping in FP32
+// (rhsKernel, mhd2d_glm_gpu_mixed.cu:355-375), the stage update in FP64
+// (stage1Kernel/stage2Kernel, :377-410).
+template <int STAGE>
+__device__ __forceinline__ void update_cell(const StageArgs<RefMixPol>& a,
+                                            const CellIO<RefMixPol>& src, size_t g,
+                                            const float rhs_flux[9], double dt, double damping,
+                                            double out[9]) {
+  CellIO<RefMixPol> old;
+  if (STAGE == 2) old.load(a.uold, a.plane, g);
+#pragma unroll
+  for (int k = 0; k < 9; k++) {
+    float rhs = rhs_flux[k];
+    if (k == PSI) rhs = rhs - (float)damping * (float)src.psi();
+    if (STAGE == 1)
+      out[k] = src.v[k] + dt * (double)rhs;
+    else
+      out[k] = 0.5 * old.v[k] + 0.5 * (src.v[k] + dt * (double)rhs);
   }
 #pragma unroll
   for (int k = 0; k < 9; k++) a.dst[k * a.plane + g] = out[k];
@@ -484,40 +638,47 @@ __device__ __forceinline__ void update_cell(const StageArgs<KahanPol>& a,
 // rhs type of each policy.
 template <class Pol> struct RhsType { using type = typename Pol::T; };
 template <> struct RhsType<KahanPol> { using type = float; };
+template <> struct RhsType<RefMixPol> { using type = float; };
 
-// LDS footprint of one tile.
-template <class R, int TX, int TY>
+// LDS footprint of one tile: primitives (tile + halo) in RP, face fluxes in
+// RS.
+template <class RP, class RS, int TX, int TY>
 struct TileShared {
   static constexpr int NT = TX * TY;
   static constexpr int HX = TX + 4, HY = TY + 4, HN = HX * HY;
   static constexpr int NXF = (TX + 1) * TY, NYF = TX * (TY + 1), NF = NXF + NYF;
   static constexpr int NW = (NT + 63) / 64;
   static constexpr int NTHR = NW * 64;  // threads per block: whole waves
-  R sP[9 * HN];
-  R sF[9 * NF];
+  RP sP[9 * HN];
+  RS sF[9 * NF];
   unsigned long long sRed[3 * NW];
 };
+template <class Pol, int TX, int TY>
+using TileSharedOf = TileShared<typename Pol::RP, typename Pol::RS, TX, TY>;
 
 // One stage on one tile (tile index `tile` of the instance described by
 // `a`), by one workgroup.  The caller has checked that the instance is
 // running; every thread of the block must call this.
 template <class Pol, bool MUSCL, bool HLLD, int STAGE, int TX, int TY>
 __device__ __forceinline__ void stage_tile(const StageArgs<Pol>& a, int tile,
-                                           TileShared<typename Pol::R, TX, TY>& sh) {
+                                           TileSharedOf<Pol, TX, TY>& sh) {
   using R = typename Pol::R;
+  using RP = typename Pol::RP;
+  using RS = typename Pol::RS;
   using T = typename Pol::T;
   using A = typename RhsType<Pol>::type;
-  using Sh = TileShared<R, TX, TY>;
+  using Sh = TileSharedOf<Pol, TX, TY>;
   constexpr int NT = Sh::NT, HX = Sh::HX, HN = Sh::HN;
   constexpr int NXF = Sh::NXF, NF = Sh::NF, NW = Sh::NW, NTHR = Sh::NTHR;
-  R* sP = sh.sP;
-  R* sF = sh.sF;
+  RP* sP = sh.sP;
+  RS* sF = sh.sF;
   unsigned long long* sRed = sh.sRed;
 
   Ctrl<T>* ctrl = a.ctrl;
   const T ch_t = uni(ctrl->ch), dt = uni(ctrl->dt), dx = uni(ctrl->dx), dy = uni(ctrl->dy);
   const R ch = (R)ch_t;
   const Phys<R> c = make_phys<R>(a.tol);
+  const Phys<RP> cp = make_phys<RP>(a.tol);  // recovery / reconstruction
   const int nx = a.nx, ny = a.ny;
   const size_t ps = a.plane;
   const int ty = tile / a.tiles_x, tx = tile - ty * a.tiles_x;
@@ -531,8 +692,8 @@ __device__ __forceinline__ void stage_tile(const StageArgs<Pol>& a, int tile,
     const size_t g = (size_t)wrap(j0 + r - 2, ny) * nx + wrap(i0 + cc - 2, nx);
     CellIO<Pol> u;
     u.load(a.src, ps, g);
-    R P[9];
-    recover<MUSCL>(c, u, a.lowbeta_ratio, P);
+    RP P[9];
+    recover<MUSCL>(cp, u, a.lowbeta_ratio, P);
 #pragma unroll
     for (int k = 0; k < 9; k++) sP[k * HN + h] = P[k];
   }
@@ -560,7 +721,7 @@ __device__ __forceinline__ void stage_tile(const StageArgs<Pol>& a, int tile,
     }
     const int q1 = (ydir ? 2 : 1) * HN, q2 = (ydir ? 1 : 2) * HN;
     const int q5 = (ydir ? 6 : 5) * HN, q6 = (ydir ? 5 : 6) * HN;
-    auto load = [&](int cell, R out[9]) {
+    auto load = [&](int cell, RP out[9]) {
       out[0] = sP[cell];
       out[1] = sP[q1 + cell];
       out[2] = sP[q2 + cell];
@@ -571,20 +732,28 @@ __device__ __forceinline__ void stage_tile(const StageArgs<Pol>& a, int tile,
       out[7] = sP[7 * HN + cell];
       out[8] = sP[8 * HN + cell];
     };
+    // Face states in RP, widened to the flux type R (a no-op but for the
+    // reference-mixed policy, whose face states are FP32 and flux FP64).
     R UL[9], UR[9];
     if (MUSCL) {
-      R Pa[9], Pb[9], Pc[9], Pf[9];
+      RP Pa[9], Pb[9], Pc[9], Pf[9];
       load(base, Pa);
       load(base + stride, Pb);
       load(base + 2 * stride, Pc);
-      reconstruct_face<R, true>(c, Pa, Pb, Pc, Pf);
-      prim_to_cons(c, Pf, UL);
+      reconstruct_face<RP, true>(cp, Pa, Pb, Pc, Pf);
+      face_cons<Pol>(cp, Pf, UL);
       load(base + 3 * stride, Pa);
-      reconstruct_face<R, false>(c, Pb, Pc, Pa, Pf);
-      prim_to_cons(c, Pf, UR);
+      reconstruct_face<RP, false>(cp, Pb, Pc, Pa, Pf);
+      face_cons<Pol>(cp, Pf, UR);
     } else {
-      load(base + stride, UL);
-      load(base + 2 * stride, UR);
+      RP Ua[9], Ub[9];
+      load(base + stride, Ua);
+      load(base + 2 * stride, Ub);
+#pragma unroll
+      for (int k = 0; k < 9; k++) {
+        UL[k] = (R)Ua[k];
+        UR[k] = (R)Ub[k];
+      }
     }
     R F[9];
     const int st = glm_flux_x<R, HLLD>(c, UL, UR, ch, F);
@@ -594,15 +763,15 @@ __device__ __forceinline__ void stage_tile(const StageArgs<Pol>& a, int tile,
     }
     const int p1 = (ydir ? 2 : 1) * NF, p2 = (ydir ? 1 : 2) * NF;
     const int p5 = (ydir ? 6 : 5) * NF, p6 = (ydir ? 5 : 6) * NF;
-    sF[f] = F[0];
-    sF[p1 + f] = F[1];
-    sF[p2 + f] = F[2];
-    sF[3 * NF + f] = F[3];
-    sF[4 * NF + f] = F[4];
-    sF[p5 + f] = F[5];
-    sF[p6 + f] = F[6];
-    sF[7 * NF + f] = F[7];
-    sF[8 * NF + f] = F[8];
+    sF[f] = (RS)F[0];
+    sF[p1 + f] = (RS)F[1];
+    sF[p2 + f] = (RS)F[2];
+    sF[3 * NF + f] = (RS)F[3];
+    sF[4 * NF + f] = (RS)F[4];
+    sF[p5 + f] = (RS)F[5];
+    sF[p6 + f] = (RS)F[6];
+    sF[7 * NF + f] = (RS)F[7];
+    sF[8 * NF + f] = (RS)F[8];
   }
   __syncthreads();
 
@@ -624,7 +793,7 @@ __device__ __forceinline__ void stage_tile(const StageArgs<Pol>& a, int tile,
     A rhs[9];
 #pragma unroll
     for (int k = 0; k < 9; k++)
-      rhs[k] = flux_div<A, R>(sF[k * NF + xr], sF[k * NF + xl], sF[k * NF + yt],
+      rhs[k] = flux_div<A, RS>(sF[k * NF + xr], sF[k * NF + xl], sF[k * NF + yt],
                               sF[k * NF + yb], pow2, adx, ady, aidx, aidy);
     CellIO<Pol> src;
     src.load(a.src, ps, g);
@@ -632,13 +801,18 @@ __device__ __forceinline__ void stage_tile(const StageArgs<Pol>& a, int tile,
     update_cell<STAGE>(a, src, g, rhs, dt, damping, out);
     if constexpr (sizeof(R) == sizeof(T)) {
       // Exact forms: the reference's check (mhd2d_glm_cpu.cpp:328-345) and
-      // wave speeds (289-305), bitwise.
+      // wave speeds (289-305), bitwise.  Reference-mixed: the same FP64
+      // check (mhd2d_glm_gpu_mixed.cu:377-410), FP32 speeds (256-280).
       const Phys<T> ct = make_phys<T>(a.tol);
       T P[9];
       if (!state_usable(ct, out, P)) atomicMin(&ctrl->bad[STAGE - 1], (int)g);
       if (STAGE == 2) {
         bool valid;
-        const T s = cell_speed(ct, P, valid);
+        T s;
+        if constexpr (Pol::kRefMixed)
+          s = refmix::cell_speed(out, valid);
+        else
+          s = cell_speed(ct, P, valid);
         sbits = speed_bits(s);
         svalid = valid ? 1u : 0u;
       }
@@ -704,7 +878,7 @@ __device__ __forceinline__ void stage_tile(const StageArgs<Pol>& a, int tile,
 // One instance, one block per tile (XCD-banded tile order).
 template <class Pol, bool MUSCL, bool HLLD, int STAGE, int TX, int TY>
 __global__ __launch_bounds__((TX * TY + 63) / 64 * 64) void k_stage(StageArgs<Pol> a) {
-  __shared__ TileShared<typename Pol::R, TX, TY> sh;
+  __shared__ TileSharedOf<Pol, TX, TY> sh;
   Ctrl<typename Pol::T>* ctrl = a.ctrl;
   if (ctrl->run != RUN_ACTIVE) return;
   if (STAGE == 2 && ctrl->bad[0] != INT_MAX) return;
@@ -742,7 +916,7 @@ struct ListSegs {
 
 template <class Pol, bool MUSCL, bool HLLD, int STAGE, int TX, int TY>
 __global__ __launch_bounds__((TX * TY + 63) / 64 * 64) void k_stage_list(ListSegs<Pol> g) {
-  __shared__ TileShared<typename Pol::R, TX, TY> sh;
+  __shared__ TileSharedOf<Pol, TX, TY> sh;
   int n[kListSegs], total = 0;
 #pragma unroll
   for (int k = 0; k < kListSegs; k++) {
@@ -785,9 +959,14 @@ __global__ __launch_bounds__(256) void k_speed(const typename Pol::S* src, size_
        q += (long long)gridDim.x * 256) {
     T U[9], P[9];
     load_wide<Pol>(src, plane, (size_t)q, U);
-    cons_to_prim<T, 9>(c, U, P);
     bool valid;
-    const T s = cell_speed(c, P, valid);
+    T s;
+    if constexpr (Pol::kRefMixed) {
+      s = refmix::cell_speed(U, valid);
+    } else {
+      cons_to_prim<T, 9>(c, U, P);
+      s = cell_speed(c, P, valid);
+    }
     best = u64max(best, speed_bits(s));
     valid_any |= valid ? 1u : 0u;
   }
@@ -820,7 +999,11 @@ __device__ void prepare_step(Ctrl<T>* ctrl) {
   }
   const T speed = bits_speed<T>(ctrl->speed_cur);
   const T ch = smax(speed, (T)1.0e-12);
-  const T r = ch / ctrl->dx + ch / ctrl->dy;
+  // The reference-mixed build takes the rate in float (rateKernel,
+  // mhd2d_glm_gpu_mixed.cu:283-308); in closed form as for the exact forms.
+  const T r = ctrl->rate_f32
+                  ? (T)((float)ch / (float)ctrl->dx + (float)ch / (float)ctrl->dy)
+                  : ch / ctrl->dx + ch / ctrl->dy;
   const T rate = ctrl->valid_cur ? smax((T)0.0, r) : (T)0.0;
   ctrl->ch = ch;
   if (rate <= (T)0.0) {
@@ -968,7 +1151,7 @@ template <class Pol> struct QueueWaves { static constexpr int value = sizeof(typ
 template <class Pol, bool MUSCL, bool HLLD, int STAGE, int TX, int TY>
 __global__ __launch_bounds__((TX * TY + 63) / 64 * 64)
 __attribute__((amdgpu_waves_per_eu(QueueWaves<Pol>::value))) void k_stage_queue(EnsArgs<Pol> e) {
-  __shared__ TileShared<typename Pol::R, TX, TY> sh;
+  __shared__ TileSharedOf<Pol, TX, TY> sh;
   __shared__ int s_item, s_count;
   // Items are claimed kClaim at a time (consecutive tiles, nearly always of
   // one instance): fewer queue atomics per tile at the cost of at most
@@ -1026,7 +1209,7 @@ __attribute__((amdgpu_waves_per_eu(QueueWaves<Pol>::value))) void k_stage_queue(
 template <class Pol, bool MUSCL, bool HLLD, int STAGE, int TX, int TY>
 __global__ __launch_bounds__((TX * TY + 63) / 64 * 64) void k_queue_redo(EnsArgs<Pol> e,
                                                                           const int* list) {
-  __shared__ TileShared<typename Pol::R, TX, TY> sh;
+  __shared__ TileSharedOf<Pol, TX, TY> sh;
   const int count = __builtin_amdgcn_readfirstlane((int)e.queue[16 + STAGE - 1]);
   for (int x = (int)blockIdx.x; x < count; x += (int)gridDim.x) {
     const int item = __builtin_amdgcn_readfirstlane(list[x]);

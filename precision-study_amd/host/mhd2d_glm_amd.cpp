@@ -10,8 +10,9 @@
 // .dat name and 13-column layout, and exit codes (1 usage, 2 rate, 3/4
 // non-physical state, 5 output file, 9 device).  Precision comes from the
 // program name (a name ending in "fp32" runs FP32, as the reference's
-// separate -DUSE_FLOAT binary does) or from MHD_PRECISION (fp64, fp32,
-// mixed, mixed_kahan, fp64_fma).  MHD_HLLD_TOLERANCE sets the HLLD tolerance
+// separate -DUSE_FLOAT binary does; a name ending in "_gpu_mixed" runs the
+// reference's stage-level mixed form, as mhd2d_glm_gpu_mixed does) or from
+// MHD_PRECISION (fp64, fp32, mixed, mixed_kahan, fp64_fma, ref_mixed).  MHD_HLLD_TOLERANCE sets the HLLD tolerance
 // (default 1e-4, the value of every reference timing and checksum run);
 // MHD_TILE the stage tile shape, MHD_LOWBETA_RATIO the mixed form's FP64
 // recovery switch and MHD_FALLBACK its FP64 tile fallback (0 on, 1 off).
@@ -118,6 +119,10 @@ int run(const Args& a, int precision, const char* prec_name, double tol) {
     return 1;
   }
   const bool use_muscl = a.order == "muscl", use_hlld = a.solver == "hlld";
+  if (precision == MHD_PREC_REF_MIXED && !use_muscl) {
+    std::cerr << "Error: the reference-mixed form needs 'muscl'." << std::endl;
+    return 1;
+  }
   const int nx = a.nx, ny = a.ny;
   const R dx = ((R)1.0 - (R)0.0) / (R)nx, dy = ((R)1.0 - (R)0.0) / (R)ny;
   const R gamma = (R)(5.0 / 3.0);
@@ -299,17 +304,28 @@ int main(int argc, char* argv[]) {
   if (argc > 8) a.problem = argv[8];
   static const struct { const char* name; int prec; } kForms[] = {
       {"fp64", MHD_PREC_FP64}, {"fp32", MHD_PREC_FP32}, {"mixed", MHD_PREC_MIXED},
-      {"mixed_kahan", MHD_PREC_MIXED_KAHAN}, {"fp64_fma", MHD_PREC_FP64_FMA}};
+      {"mixed_kahan", MHD_PREC_MIXED_KAHAN}, {"fp64_fma", MHD_PREC_FP64_FMA},
+      {"ref_mixed", MHD_PREC_REF_MIXED}};
+  constexpr int kNumForms = 6;
   int form = 0;
   const std::string prog = argv[0];
-  for (int q = 0; q < 5; q++)
+  for (int q = 0; q < kNumForms; q++)
     if (ends_with(prog, std::string("_") + kForms[q].name)) form = q;
+  // A program named as the reference's mixed build (mhd2d_glm_gpu_mixed)
+  // runs that build's form and writes its file tag, "mixed".
+  const char* tag = nullptr;
+  if (ends_with(prog, "_gpu_mixed")) {
+    form = 5;
+    tag = "mixed";
+  }
   if (const char* s = std::getenv("MHD_PRECISION")) {
     form = -1;
-    for (int q = 0; q < 5; q++)
+    tag = nullptr;
+    for (int q = 0; q < kNumForms; q++)
       if (!std::strcmp(s, kForms[q].name)) form = q;
     if (form < 0) {
-      std::cerr << "Error: MHD_PRECISION must be fp64, fp32, mixed, mixed_kahan or fp64_fma."
+      std::cerr << "Error: MHD_PRECISION must be fp64, fp32, mixed, mixed_kahan, fp64_fma or "
+                   "ref_mixed."
                 << std::endl;
       return 1;
     }
@@ -317,5 +333,5 @@ int main(int argc, char* argv[]) {
   double tol = 1.0e-4;
   if (const char* s = std::getenv("MHD_HLLD_TOLERANCE")) tol = std::atof(s);
   if (kForms[form].prec == MHD_PREC_FP32) return run<float>(a, kForms[form].prec, "fp32", tol);
-  return run<double>(a, kForms[form].prec, kForms[form].name, tol);
+  return run<double>(a, kForms[form].prec, tag ? tag : kForms[form].name, tol);
 }

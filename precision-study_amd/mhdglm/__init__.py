@@ -31,11 +31,13 @@ PREC_FP32 = 1
 PREC_MIXED = 2
 PREC_MIXED_KAHAN = 3
 PREC_FP64_FMA = 4
+PREC_REF_MIXED = 5
 PRECISIONS = {"fp64": PREC_FP64, "fp32": PREC_FP32, "mixed": PREC_MIXED,
-              "mixed_kahan": PREC_MIXED_KAHAN, "fp64_fma": PREC_FP64_FMA}
+              "mixed_kahan": PREC_MIXED_KAHAN, "fp64_fma": PREC_FP64_FMA,
+              "ref_mixed": PREC_REF_MIXED}
 # Element type of host state buffers.
 DTYPES = {PREC_FP64: np.float64, PREC_FP32: np.float32, PREC_MIXED: np.float64,
-          PREC_MIXED_KAHAN: np.float64, PREC_FP64_FMA: np.float64}
+          PREC_MIXED_KAHAN: np.float64, PREC_FP64_FMA: np.float64, PREC_REF_MIXED: np.float64}
 
 # Public symbols of include/mhd_glm.h.
 EXPORTED = (
