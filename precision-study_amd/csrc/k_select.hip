@@ -468,51 +468,99 @@ __global__ void k_adam(float* P, float* grad, float* m, float* v, int step, floa
   grad[i] = 0.f;
 }
 
-// Tile lists of a step, by one block: flagged tiles -> lists[0 .. counts[0])
-// (FP64 tiles), and an audit sample of the unflagged ones (hash of tile and
-// device step counter under `frac`, so a replayed graph samples new tiles)
-// -> lists[ntiles .. ntiles + counts[1]).  Counts the FP64 tile-steps into
-// counters[2] and the steps into counters[3].
-// Also copies the control block (ctrl_words words) into ctrl_copy: the
-// audits' snapshot.
-__global__ __launch_bounds__(1024) void k_audit_pick(const unsigned char* flags, int ntiles,
-                                                     const int* step, float frac, int* lists,
-                                                     int* counts, unsigned long long* counters,
-                                                     const unsigned* ctrl, unsigned* ctrl_copy,
-                                                     int ctrl_words) {
-  __shared__ int cnt[2];
-  if (threadIdx.x < 2) cnt[threadIdx.x] = 0;
-  for (int w = threadIdx.x; w < ctrl_words; w += blockDim.x) ctrl_copy[w] = ctrl[w];
+// Tile lists of the current flags, by one block of kRouteThreads, each in
+// tile order: flagged tiles -> lists[0 .. counts[0]) (FP64 tiles), the
+// others -> lists[2 ntiles .. 2 ntiles + counts[2]) (mixed tiles).  Each
+// thread takes a contiguous chunk of tiles: it counts, a block-wide
+// exclusive scan gives its offsets, it writes.  Runs when the flags are
+// refreshed; an audit miss later appends its tile to the FP64 list and sets
+// its flag (the mixed stages skip flagged tiles).
+constexpr int kRouteThreads = 1024;
+__global__ __launch_bounds__(kRouteThreads) void k_route(const unsigned char* flags, int ntiles,
+                                                         int* lists, int* counts) {
+  constexpr int kWaves = kRouteThreads / 64;
+  __shared__ int wsum[kWaves];
+  const int tid = (int)threadIdx.x, lane = tid & 63, w = tid >> 6;
+  const int chunk = (ntiles + kRouteThreads - 1) / kRouteThreads;
+  const int t0 = min(tid * chunk, ntiles), t1 = min(t0 + chunk, ntiles);
+  int nf = 0;
+  for (int t = t0; t < t1; t++) nf += flags[t] ? 1 : 0;
+  int sf = nf;  // inclusive scan within the wave
+#pragma unroll
+  for (int d = 1; d < 64; d <<= 1) {
+    const int u = __shfl_up(sf, d);
+    if (lane >= d) sf += u;
+  }
+  if (lane == 63) wsum[w] = sf;
   __syncthreads();
-  const unsigned st = (unsigned)*step;
-  // 8 tiles per thread and iteration (one 8-byte load; the flag array comes
-  // from hipMalloc, so 8-byte aligned), so the loop is a few iterations deep.
-  for (int t0 = 8 * (int)threadIdx.x; t0 < ntiles; t0 += 8 * (int)blockDim.x) {
-    unsigned long long f8 = 0;
-    if (t0 + 8 <= ntiles) {
-      f8 = *(const unsigned long long*)(flags + t0);
-    } else {
-      for (int q = 0; t0 + q < ntiles; q++) f8 |= (unsigned long long)flags[t0 + q] << (8 * q);
-    }
-    for (int q = 0; q < 8 && t0 + q < ntiles; q++) {
-      const int t = t0 + q;
-      if ((f8 >> (8 * q)) & 0xFFull) {
-        lists[atomicAdd(&cnt[0], 1)] = t;
-        continue;
-      }
-      unsigned h = (unsigned)t * 2654435761u ^ (st * 2246822519u + 0x9E3779B9u);
-      h ^= h >> 15;
-      h *= 2246822519u;
-      h ^= h >> 13;
-      if ((float)(h & 0xFFFFFF) < frac * 16777216.f) lists[ntiles + atomicAdd(&cnt[1], 1)] = t;
+  int bf = 0, tf = 0;
+#pragma unroll
+  for (int k = 0; k < kWaves; k++) {
+    bf += (k < w) ? wsum[k] : 0;
+    tf += wsum[k];
+  }
+  int pf = bf + sf - nf;  // FP64 tiles before the chunk
+  int pm = t0 - pf;       // mixed tiles before the chunk
+  for (int t = t0; t < t1; t++) {
+    if (flags[t])
+      lists[pf++] = t;
+    else
+      lists[2 * ntiles + pm++] = t;
+  }
+  if (tid == 0) {
+    counts[0] = tf;
+    counts[2] = ntiles - tf;
+  }
+}
+
+// Audit sample of a step, by one block: systematic sampling of the mixed
+// list, entries floor((k + phi) / frac) for k = 0, 1, ... below its length,
+// with the phase phi in [0, 1) a hash of the device step counter (a replayed
+// graph samples anew).  Distinct entries (the stride 1/frac is >= 1), a
+// fraction `frac` of the mixed tiles -> lists[ntiles .. ntiles + counts[1]).
+// Counts the FP64 tile-steps into counters[2] and the steps into
+// counters[3].  Also copies the control block (ctrl_words words) into
+// ctrl_copy: the audits' snapshot.  A step of a run that has finished or
+// halted (*run != 0: the step kernels do nothing, as in the tail of a
+// replayed batch) samples and counts nothing, so its audits cannot compare
+// stale buffers.
+constexpr int kPickThreads = 256;
+__global__ __launch_bounds__(kPickThreads) void k_audit_pick(int ntiles, const int* step,
+                                                             const int* run, float frac,
+                                                             int* lists, int* counts,
+                                                             unsigned long long* counters,
+                                                             const unsigned* ctrl,
+                                                             unsigned* ctrl_copy, int ctrl_words) {
+  __shared__ int picked;
+  const int tid = (int)threadIdx.x;
+  if (tid == 0) picked = 0;
+  for (int q = tid; q < ctrl_words; q += kPickThreads) ctrl_copy[q] = ctrl[q];
+  __syncthreads();
+  const bool active = *run == 0;
+  const int n = counts[2];
+  int mine = 0;
+  if (active && frac > 0.f && n > 0) {
+    unsigned h = (unsigned)*step * 2246822519u + 0x9E3779B9u;
+    h ^= h >> 15;
+    h *= 2654435761u;
+    h ^= h >> 13;
+    const double phi = (double)(h & 0xFFFFFF) * (1.0 / 16777216.0);
+    const double stride = frac >= 1.f ? 1.0 : 1.0 / (double)frac;
+    for (int k = tid;; k += kPickThreads) {
+      const double x = ((double)k + phi) * stride;
+      if (x >= (double)n) break;
+      lists[ntiles + k] = lists[2 * ntiles + (int)x];
+      mine++;
     }
   }
+  if (mine) atomicAdd(&picked, mine);
   __syncthreads();
-  if (threadIdx.x == 0) {
-    counts[0] = cnt[0];
-    counts[1] = cnt[1];
-    counters[2] += (unsigned long long)cnt[0];
-    counters[3] += 1ull;
+  if (tid == 0) {
+    counts[1] = picked;
+    if (active) {
+      counters[2] += (unsigned long long)counts[0];
+      counters[3] += 1ull;
+    }
   }
 }
 
@@ -591,12 +639,19 @@ hipError_t sel_adam(float* params, float* grad, float* m, float* v, int step, fl
   return hipGetLastError();
 }
 
-hipError_t sel_audit_pick(const unsigned char* flags, int ntiles, const int* step, float frac,
-                          int* lists, int* counts, unsigned long long* counters, const void* ctrl,
+hipError_t sel_route(const unsigned char* flags, int ntiles, int* lists, int* counts,
+                     hipStream_t s) {
+  hipLaunchKernelGGL(sel::k_route, dim3(1), dim3(sel::kRouteThreads), 0, s, flags, ntiles, lists,
+                     counts);
+  return hipGetLastError();
+}
+
+hipError_t sel_audit_pick(int ntiles, const int* step, const int* run, float frac, int* lists,
+                          int* counts, unsigned long long* counters, const void* ctrl,
                           void* ctrl_copy, size_t ctrl_bytes, hipStream_t s) {
-  hipLaunchKernelGGL(sel::k_audit_pick, dim3(1), dim3(1024), 0, s, flags, ntiles, step, frac,
-                     lists, counts, counters, (const unsigned*)ctrl, (unsigned*)ctrl_copy,
-                     (int)((ctrl_bytes + 3) / 4));
+  hipLaunchKernelGGL(sel::k_audit_pick, dim3(1), dim3(sel::kPickThreads), 0, s, ntiles, step,
+                     run, frac, lists, counts, counters, (const unsigned*)ctrl,
+                     (unsigned*)ctrl_copy, (int)((ctrl_bytes + 3) / 4));
   return hipGetLastError();
 }
 

@@ -62,8 +62,8 @@ struct mhd_ctx {
     float* err = nullptr;           // ntiles
     unsigned char* flags = nullptr;   // FP64 routing of the step
     unsigned char* sticky = nullptr;  // tiles forced to FP64 by an audit miss
-    int* lists = nullptr;           // 2 x ntiles: the step's FP64 tiles, audit sample
-    int* counts = nullptr;          // lengths of the two lists
+    int* lists = nullptr;           // 3 x ntiles: the step's FP64 tiles, audit sample, mixed tiles
+    int* counts = nullptr;          // lengths of the three lists
     double* sumsq = nullptr;        // error scale (per-plane sum of squares)
     unsigned long long* counters = nullptr;  // audits, misses, FP64 tile-steps, steps
     void* u1x = nullptr;            // FP64 scratch stage output
@@ -227,6 +227,7 @@ int status_of(mhd_ctx* c, mhd_status* st) {
 // tile's character changes slowly: information crosses less than one cell
 // per step).
 constexpr int kSelectorRefresh = 8;
+static_assert(mhd::RUN_ACTIVE == 0, "sel_audit_pick treats run state 0 as active");
 
 // One step with the learned selector: tile features -> MLP -> FP64 flags
 // (every kSelectorRefresh steps); the mixed kernels take the unflagged tiles
@@ -244,27 +245,32 @@ hipError_t enqueue_step_selected(mhd_ctx* c) {
     if (e == hipSuccess)
       e = mhd::sel_mlp_infer(S.feats, S.ntiles, S.params, S.mean, S.istd, S.threshold, S.sticky,
                              nullptr, S.flags, c->stream);
+    if (e == hipSuccess) e = mhd::sel_route(S.flags, S.ntiles, S.lists, S.counts, c->stream);
   }
-  // FP64 tiles and audit sample as device lists (from the pick, which also
-  // snapshots the control block for the audits).  One list launch per stage
-  // evaluates in FP64, over a small fixed grid: the flagged tiles, the tiles
-  // the mixed stage queued for its fallback, and (stage 1) the audit tiles
-  // into scratch.  The mixed stages only queue their fallback tiles here.
+  // FP64 tiles and mixed tiles as device lists (rebuilt with the flags), and
+  // the step's audit sample (from the pick, which also snapshots the control
+  // block for the audits).  The mixed stages walk the mixed list, each XCD
+  // an equal part of it, so flagged tiles cost the mixed launch nothing
+  // wherever they cluster; they only queue their fallback tiles here.  One
+  // list launch per stage evaluates in FP64, over a small fixed grid: the
+  // flagged tiles, the tiles the mixed stage queued for its fallback, and
+  // (stage 1) the audit tiles into scratch.
   int* fp64_list = S.lists;
   int* audit_list = S.lists + S.ntiles;
   const int* fp64_count = S.counts;
   const int* audit_count = S.counts + 1;
   mhd::GridDesc gm = c->grid;
   gm.flags = S.flags;
-  gm.flag_mode = 1;
+  gm.tiles = S.lists + 2 * S.ntiles;
+  gm.tile_count = S.counts + 2;
   gm.redo = nullptr;
   auto redo_count = [&](int stage) {
     return (const int*)((char*)c->ctrl + offsetof(Ctrl<double>, redo_count) +
                         (stage - 1) * sizeof(int));
   };
   if (e == hipSuccess)
-    e = mhd::sel_audit_pick(S.flags, S.ntiles, (const int*)(ctrl + offsetof(Ctrl<double>, step)),
-                            S.audit_frac, S.lists, S.counts, S.counters, c->ctrl, S.ctrl_scratch,
+    e = mhd::sel_audit_pick(S.ntiles, (const int*)(ctrl + offsetof(Ctrl<double>, step)),
+                            (const int*)(ctrl + offsetof(Ctrl<double>, run)), S.audit_frac, S.lists, S.counts, S.counters, c->ctrl, S.ctrl_scratch,
                             c->L->ctrl_bytes, c->stream);
   if (e == hipSuccess) e = c->L->stage(gm, 1, c->u, c->u, c->u1, c->ctrl, c->stream);
   if (e == hipSuccess) {
@@ -1215,8 +1221,8 @@ int sel_alloc(mhd_ctx* c) {
   HIP_OR_FAIL(c, hipMalloc((void**)&S.err, nt * sizeof(float)));
   HIP_OR_FAIL(c, hipMalloc((void**)&S.flags, nt));
   HIP_OR_FAIL(c, hipMalloc((void**)&S.sticky, nt));
-  HIP_OR_FAIL(c, hipMalloc((void**)&S.lists, 2 * nt * sizeof(int)));
-  HIP_OR_FAIL(c, hipMalloc((void**)&S.counts, 2 * sizeof(int)));
+  HIP_OR_FAIL(c, hipMalloc((void**)&S.lists, 3 * nt * sizeof(int)));
+  HIP_OR_FAIL(c, hipMalloc((void**)&S.counts, 3 * sizeof(int)));
   HIP_OR_FAIL(c, hipMalloc((void**)&S.sumsq, 9 * sizeof(double)));
   HIP_OR_FAIL(c, hipMalloc((void**)&S.counters, 4 * sizeof(unsigned long long)));
   HIP_OR_FAIL(c, hipMalloc(&S.u1x, 9 * c->plane * sizeof(double)));
@@ -1275,7 +1281,8 @@ int mhd_pair_tile_errors(mhd_ctx* c, float* err) {
   const mhd::Launchers* F64 = mhd::launchers_exact(mhd::MODE_FP64);
   mhd::GridDesc g = c->grid;
   g.flags = nullptr;
-  g.flag_mode = 0;
+  g.tiles = nullptr;
+  g.tile_count = nullptr;
   g.redo_list[0] = g.redo_list[1] = nullptr;  // both updates of every tile
   // Both stage-1 updates of the current state, against a prepared snapshot
   // of the control block (the run's own counters and verdicts are

@@ -64,11 +64,14 @@ struct GridDesc {
   double tol;            // HLLD degeneracy tolerance
   double lowbeta_ratio;  // mixed forms: FP64 recovery where E > ratio * e_int
   int tile;              // tile-shape id
-  // Per-tile precision routing (learned selector): with flags non-null a
-  // stage kernel evaluates only the tiles with flags[t] == 0 (flag_mode 1)
-  // or only those with flags[t] != 0 (flag_mode 2).
+  // Per-tile precision routing (learned selector).  With `tiles` non-null a
+  // stage kernel evaluates the listed tiles tiles[0 .. *tile_count) (each
+  // XCD a contiguous part of the list) instead of the whole grid; with
+  // `flags` non-null it skips a tile whose flag is set (a tile the stage-1
+  // audit moved to FP64 after the list was made).
   const unsigned char* flags = nullptr;
-  int flag_mode = 0;
+  const int* tiles = nullptr;
+  const int* tile_count = nullptr;
   // FP64 fallback of the mixed form.  With redo_list[stage-1] non-null, a
   // mixed stage tile with an unusable cell (the positivity check) writes
   // nothing and appends its index to that list (count in the control

@@ -39,11 +39,18 @@ hipError_t sel_mlp_grad(const float* X, const float* Y, const float* W, const in
                         hipStream_t s);
 hipError_t sel_adam(float* params, float* grad, float* m, float* v, int step, float lr,
                     hipStream_t s);
-// Tile lists of a selected step: lists[0 .. counts[0]) the flagged (FP64)
-// tiles, lists[ntiles .. ntiles + counts[1]) the audit sample; also copies
-// the control block into ctrl_copy (the audits' snapshot).
-hipError_t sel_audit_pick(const unsigned char* flags, int ntiles, const int* step, float frac,
-                          int* lists, int* counts, unsigned long long* counters, const void* ctrl,
+// Tile lists of the current flags, each in tile order: lists[0 .. counts[0])
+// the flagged (FP64) tiles, lists[2 ntiles .. 2 ntiles + counts[2]) the
+// unflagged (mixed) tiles.
+hipError_t sel_route(const unsigned char* flags, int ntiles, int* lists, int* counts,
+                     hipStream_t s);
+// The step's audit sample, a fraction `frac` of the mixed list taken at a
+// stride of 1/frac from a phase hashed from the device step counter ->
+// lists[ntiles .. ntiles + counts[1]); also copies the control block into
+// ctrl_copy (the audits' snapshot).  Nothing is sampled or counted while
+// *run (the control block's run state) is not 0 (active).
+hipError_t sel_audit_pick(int ntiles, const int* step, const int* run, float frac, int* lists,
+                          int* counts, unsigned long long* counters, const void* ctrl,
                           void* ctrl_copy, size_t ctrl_bytes, hipStream_t s);
 // Errors of the audit list (a: FP32-flux, b: FP64 stage output) as
 // sel_tile_errors; tiles above `budget` become sticky and flagged and join

@@ -241,6 +241,38 @@ def test_cli_whole_run_goldens(mhd, goldens, tmp_path, n, prec):
     assert data_md5(f) == g["md5"]
 
 
+def test_cli_fp32_vs_fp64_table_192(mhd, tmp_path):
+    """The comparator over the FP64 and FP32 .dat files of OT 192^2 (bitwise
+    the reference CPU builds' files) prints the reference's own table
+    (logs/mhd/raptor_scan/ot2d_cmp_nativefp32.txt) digit for digit: L1, L2,
+    Linf, relL2 per variable and the divB lines."""
+    import json
+
+    from mhdglm import compare
+
+    g = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "cmp_ot192.json")))
+    n = g["n"]
+    files = {}
+    for prec in ("fp64", "fp32"):
+        d = tmp_path / prec
+        d.mkdir()
+        say(f"cli OT {n} {prec} for the comparison table")
+        r = mhd.run_cli([n, n, g["t_end"], 0.25, "muscl", "hlld", 0.18], str(d), precision=prec)
+        assert r.returncode == 0, r.stderr
+        (files[prec],) = list(d.iterdir())
+    table = compare.compare_files(str(files["fp64"]), str(files["fp32"]))
+    rows = {}
+    for line in table.splitlines():
+        f = line.split()
+        if len(f) == 5 and f[0] in g["native_fp32"]["rows"]:
+            rows[f[0]] = f[1:]
+    for var, want in g["native_fp32"]["rows"].items():
+        assert rows[var] == want["text"], (var, rows[var], want["text"])
+    divb = re.findall(r"L1=(\S+)\s+Linf=(\S+)", table)
+    assert divb == [(g["native_fp32"]["divB"][k]["L1"], g["native_fp32"]["divB"][k]["Linf"])
+                    for k in ("fp64", "fp32")]
+
+
 # --------------------------------------------------------------------------
 # full size: size-independent properties
 # --------------------------------------------------------------------------

@@ -181,6 +181,34 @@ def test_ot512_distance_from_fp64_as_reference_logged(mhd):
         assert ref / 4 <= errs[k] <= 4 * ref, k
 
 
+def test_ot192_distance_from_fp64_as_truncation_scan(mhd):
+    """192^2 Orszag-Tang to t=0.5: the form against FP64, per field, within a
+    factor 5 of the reference's scan that truncates the same regions
+    (recovery, CFL, reconstruction, rhs, GLM source) to FP32 mantissas while
+    the flux and update stay FP64 (ot2d_table.txt row keepFU); and far below
+    the native FP32 build's distance at the same size."""
+    g = json.load(open(os.path.join(HERE, "golden", "cmp_ot192.json")))
+    n = g["n"]
+    say("ref_mixed vs fp64 OT 192^2 to t=0.5")
+    out = {}
+    for name, prec in (("fp64", mhd.PREC_FP64), ("ref_mixed", mhd.PREC_REF_MIXED)):
+        s = mhd.Solver(mhd.SolverConfig(nx=n, ny=n, precision=prec))
+        s.init_problem("ot")
+        st = s.advance(g["t_end"])
+        out[name] = s.get_state(0)
+        s.close()
+        assert st.code == 0 and st.done == 1
+    from mhdglm import compare
+
+    names = ("rho", "vx", "vy", "p", "Bx", "By", "E")
+    errs = compare.rel_l2_state(out["fp64"], out["ref_mixed"], names)
+    for k in names:
+        keep, native = g["keepFU"]["rows"][k]["relL2"], g["native_fp32"]["rows"][k]["relL2"]
+        say(f"  {k}: relL2 {errs[k]:.3e} (truncation scan {keep:.3e}, native FP32 {native:.3e})")
+        assert keep / 5 <= errs[k] <= 5 * keep, k
+        assert errs[k] < native / 10, k
+
+
 def test_cli_program_name_and_env(mhd, tmp_path):
     """A program named as the reference's mixed build runs this form and
     writes the reference's file tag; MHD_PRECISION=ref_mixed selects it too

@@ -195,6 +195,32 @@ def test_selector_none_flagged_is_mixed(mhd, audit_frac):
         assert st["audits"] == 0
 
 
+def test_selector_counts_only_live_steps(mhd):
+    """A run that ends inside a replayed batch of steps: the batch's tail
+    steps do nothing, and the selector samples, audits and counts nothing in
+    them (audits there would compare stale buffers); advancing a finished
+    run again changes nothing."""
+    say("selector to t_end inside a batch")
+    s = mhd.Solver(mhd.SolverConfig(nx=96, ny=80, precision=mhd.PREC_MIXED))
+    s.init_problem("ot")
+    steps = s.advance(0.05).steps
+    s.close()
+    batch = 32 if steps % 32 else 31
+    s = mhd.Solver(mhd.SolverConfig(nx=96, ny=80, precision=mhd.PREC_MIXED, batch_steps=batch))
+    s.init_problem("ot")
+    s.configure_selector(random_model(), threshold=2.0, audit_frac=0.3, audit_budget=1e30)
+    st = s.advance(0.05)
+    a = s.selector_stats()
+    st2 = s.advance(0.05)
+    b = s.selector_stats()
+    s.close()
+    say(f"  steps {st.steps}, stats {a}")
+    assert st.done == 1 and st.steps == steps and steps % batch != 0
+    assert a["steps"] == st.steps and a["misses"] == 0
+    assert 0.2 * 36 * st.steps < a["audits"] < 0.4 * 36 * st.steps
+    assert st2.steps == st.steps and b == a
+
+
 def test_selector_partial_routing(mhd):
     """Stage 1 of one step: each tile's update comes from the FP64 kernel if
     flagged, else from the mixed kernel (flags predicted on the host from
