@@ -45,6 +45,70 @@ template <class R> MHD_DEV R sqrt_(R a) { return __builtin_sqrt(a); }
 template <> MHD_DEV float sqrt_<float>(float a) { return __builtin_sqrtf(a); }
 template <class R> MHD_DEV bool finite_(R a) { return __builtin_isfinite(a); }
 
+// IEEE division n / d with the denominator's reciprocal shared among several
+// divisions.  gfx950 divides in FP64 as: ds = div_scale(d; n) (the
+// denominator, rescaled only in extreme exponent ranges or when n is 0),
+// ns = div_scale(n; d) with a rescale flag, y = rcp(ds) refined by two
+// Newton steps, q = ns*y, r = fma(-ds, q, ns), and the quotient
+// div_fixup(div_fmas(r, y, q, flag), d, n).  Only y takes the quarter-rate
+// rcp and four FMAs, and it depends on ds alone.
+//   Div<true, double>::rec  performs that sequence and records (ds, y);
+//   Div<true, double>::use  performs it for a later (n, d) with the recorded
+//       y, without a branch.  Its quotient is the `/` quotient when its own
+//       ds is the recorded one bit for bit, or when n is zero, infinite or
+//       NaN (div_fixup then returns a value that does not depend on the
+//       quotient it is given).  Otherwise (a denominator that div_scale
+//       rescales for this numerator: extreme exponents only) it sets `miss`
+//       and the caller evaluates again with `/`: glm_flux_x re-evaluates the
+//       whole face flux, for the lanes that missed.
+//   Div<false, R> and every FP32 Div use `/`.
+// rr() is use() when `reuse`, else rec(), as one division expression for
+// FP32: two alike `/` in the arms of a conditional get merged by the
+// optimiser, which then drops their accuracy annotation (the mixed forms'
+// fast FP32 division would silently become the IEEE one).
+template <class R> struct Den {};
+template <> struct Den<double> { double ds, y; };
+
+template <bool S, class R>
+struct Div {
+  static MHD_DEV R rec(R n, R d, Den<R>&) { return n / d; }
+  static MHD_DEV R use(R n, R d, const Den<R>&, bool&) { return n / d; }
+  static MHD_DEV R rr(R n, R d, Den<R>&, bool, bool&) { return n / d; }
+};
+template <>
+struct Div<true, double> {
+  static MHD_DEV double refine(double ds) {
+    const double y0 = __builtin_amdgcn_rcp(ds);
+    const double e0 = __builtin_fma(-ds, y0, 1.0);
+    const double y1 = __builtin_fma(y0, e0, y0);
+    const double e1 = __builtin_fma(-ds, y1, 1.0);
+    return __builtin_fma(y1, e1, y1);
+  }
+  static MHD_DEV double finish(double n, double d, double ds, double ns, bool scaled, double y) {
+    const double q = ns * y;
+    const double r = __builtin_fma(-ds, q, ns);
+    return __builtin_amdgcn_div_fixup(__builtin_amdgcn_div_fmas(r, y, q, scaled), d, n);
+  }
+  static MHD_DEV double rec(double n, double d, Den<double>& D) {
+    bool unused, scaled;
+    D.ds = __builtin_amdgcn_div_scale(n, d, false, &unused);
+    const double ns = __builtin_amdgcn_div_scale(n, d, true, &scaled);
+    D.y = refine(D.ds);
+    return finish(n, d, D.ds, ns, scaled, D.y);
+  }
+  static MHD_DEV double use(double n, double d, const Den<double>& D, bool& miss) {
+    bool unused, scaled;
+    const double ds = __builtin_amdgcn_div_scale(n, d, false, &unused);
+    const double ns = __builtin_amdgcn_div_scale(n, d, true, &scaled);
+    const bool counts = __builtin_isfinite(n) && n != 0.0;
+    miss = miss || (counts && __double_as_longlong(ds) != __double_as_longlong(D.ds));
+    return finish(n, d, ds, ns, scaled, D.y);
+  }
+  static MHD_DEV double rr(double n, double d, Den<double>& D, bool reuse, bool& miss) {
+    return reuse ? use(n, d, D, miss) : rec(n, d, D);
+  }
+};
+
 // minmod (mhd_common.hpp:140-143)
 template <class R> MHD_DEV R minmod(R a, R b) {
   if (a * b <= (R)0.0) return (R)0.0;
@@ -71,13 +135,18 @@ MHD_DEV void prim_to_cons(const Phys<R>& c, const R W[9], R U[9]) {
 }
 
 // Conserved -> primitive, pressure recovery (mhd_common.hpp:178-201).
-// Writes rho, v, p, B; psi passes through when NPSI is 9.
-template <class R, int NPSI = 9>
-MHD_DEV void cons_to_prim(const Phys<R>& c, const R U[], R W[]) {
+// Writes rho, v, p, B; psi passes through when NPSI is 9.  With S the
+// density's reciprocal is recorded in D (for the wave speeds of the same
+// state), or with `reuse` taken from D (Div).  (References and flags rather
+// than optional pointers: a private-memory pointer cannot be proven
+// non-null, and a select between two of them would put D on the stack.)
+template <class R, int NPSI, bool S>
+MHD_DEV void cons_to_prim(const Phys<R>& c, const R U[], R W[], Den<R>& D, bool reuse,
+                          bool& miss) {
   const R rho = smax(U[RHO], c.rho_floor);
-  const R vx = U[MX] / rho;
-  const R vy = U[MY] / rho;
-  const R vz = U[MZ] / rho;
+  const R vx = Div<S, R>::rr(U[MX], rho, D, reuse, miss);
+  const R vy = Div<S, R>::use(U[MY], rho, D, miss);
+  const R vz = Div<S, R>::use(U[MZ], rho, D, miss);
   const R bx = U[BX], by = U[BY], bz = U[BZ];
   const R v2 = vx * vx + vy * vy + vz * vz;
   const R b2 = bx * bx + by * by + bz * bz;
@@ -86,21 +155,35 @@ MHD_DEV void cons_to_prim(const Phys<R>& c, const R U[], R W[]) {
   W[5] = bx; W[6] = by; W[7] = bz;
   if (NPSI == 9) W[8] = U[PSI];
 }
+template <class R, int NPSI = 9>
+MHD_DEV void cons_to_prim(const Phys<R>& c, const R U[], R W[]) {
+  Den<R> D;
+  bool miss = false;
+  cons_to_prim<R, NPSI, false>(c, U, W, D, false, miss);
+}
 
 // Fast magnetosonic speed along the axis whose field component is bn
 // (mhd_common.hpp:206-238; bn = Bx for x, By for y).
-template <class R>
-MHD_DEV R fast_speed(const Phys<R>& c, const R W[], R bn) {
+// With S the density's reciprocal is recorded in D, or (reuse) taken from
+// it (recorded by the recovery of the same state).
+template <bool S, class R>
+MHD_DEV R fast_speed(const Phys<R>& c, const R W[], R bn, Den<R>& D, bool reuse, bool& miss) {
   const R rho = smax(W[0], c.rho_floor);
   const R p = smax(W[4], c.p_floor);
   const R bx = W[5], by = W[6], bz = W[7];
-  const R a2 = c.gamma * p / rho;
-  const R b2 = (bx * bx + by * by + bz * bz) / rho;
-  const R can2 = bn * bn / rho;
+  const R a2 = Div<S, R>::rr(c.gamma * p, rho, D, reuse, miss);
+  const R b2 = Div<S, R>::use(bx * bx + by * by + bz * bz, rho, D, miss);
+  const R can2 = Div<S, R>::use(bn * bn, rho, D, miss);
   R disc = (a2 + b2) * (a2 + b2) - (R)4.0 * a2 * can2;
   disc = smax(disc, (R)0.0);
   const R cf2 = (R)0.5 * (a2 + b2 + sqrt_(disc));
   return sqrt_(smax(cf2, (R)0.0));
+}
+template <class R>
+MHD_DEV R fast_speed(const Phys<R>& c, const R W[], R bn) {
+  Den<R> D;
+  bool miss = false;
+  return fast_speed<false>(c, W, bn, D, false, miss);
 }
 
 // x-normal physical flux from primitives and total energy
@@ -123,12 +206,14 @@ MHD_DEV void phys_flux_x(const R W[], R e, R F[8]) {
 }
 
 // Positive density and pressure of a conserved state; NaN fails
-// (mhd_common.hpp:426-431).
-template <class R>
-MHD_DEV bool positive_pressure(const Phys<R>& c, const R U[8]) {
+// (mhd_common.hpp:426-431).  With S the recovery's density reciprocal is
+// recorded in D, or (reuse) taken from it.
+template <bool S, class R>
+MHD_DEV bool positive_pressure(const Phys<R>& c, const R U[8], Den<R>& D, bool reuse,
+                               bool& miss) {
   if (!(U[RHO] > c.rho_floor)) return false;
   R W[8];
-  cons_to_prim<R, 8>(c, U, W);
+  cons_to_prim<R, 8, S>(c, U, W, D, reuse, miss);
   return (W[0] > c.rho_floor && W[4] > c.p_floor);
 }
 
@@ -139,9 +224,11 @@ MHD_DEV bool hlld_small(const Phys<R>& c, R value, R scale) {
 }
 
 // HLLD star state next to an outer fast wave (mhd_common.hpp:445-511).
-template <class R>
+// With S, drs receives the reciprocal of the star density (recorded by the
+// positivity check's recovery).
+template <bool S, class R>
 MHD_DEV int hlld_star(const Phys<R>& c, R e, const R W[8], R s, R sm,
-                      R pt_star, R bx, R Us[8]) {
+                      R pt_star, R bx, R Us[8], Den<R>& drs, bool& miss) {
   const R rho = smax(W[0], c.rho_floor);
   const R vx = W[1], vy = W[2], vz = W[3];
   const R p = smax(W[4], c.p_floor);
@@ -150,23 +237,23 @@ MHD_DEV int hlld_star(const Phys<R>& c, R e, const R W[8], R s, R sm,
   const R s_minus_sm = s - sm;
   const R wave_scale = smax(fabs_(s), fabs_(sm));
   if (hlld_small(c, s_minus_sm, wave_scale)) return HLLD_DEGENERATE;
-  const R rho_star = rho * s_minus_vx / s_minus_sm;
+  Den<R> dsm, dd;
+  const R rho_star = Div<S, R>::rec(rho * s_minus_vx, s_minus_sm, dsm);
   if (rho_star <= c.rho_floor) return HLLD_NONPHYSICAL;
   const R denom = rho * s_minus_vx * s_minus_sm - bx * bx;
   const R denom_scale = fabs_(rho * s_minus_vx * s_minus_sm) + bx * bx;
   if (hlld_small(c, denom, denom_scale)) return HLLD_DEGENERATE;
   const R common = sm - vx;
-  const R vy_star = vy - bx * by * common / denom;
-  const R vz_star = vz - bx * bz * common / denom;
-  const R by_star = by * (rho * s_minus_vx * s_minus_vx - bx * bx) / denom;
-  const R bz_star = bz * (rho * s_minus_vx * s_minus_vx - bx * bx) / denom;
+  const R vy_star = vy - Div<S, R>::rec(bx * by * common, denom, dd);
+  const R vz_star = vz - Div<S, R>::use(bx * bz * common, denom, dd, miss);
+  const R by_star = Div<S, R>::use(by * (rho * s_minus_vx * s_minus_vx - bx * bx), denom, dd, miss);
+  const R bz_star = Div<S, R>::use(bz * (rho * s_minus_vx * s_minus_vx - bx * bx), denom, dd, miss);
   const R b2 = bx * bx + by * by + bz * bz;
   const R pt = p + (R)0.5 * b2;
   const R vdotb = vx * bx + vy * by + vz * bz;
   const R vsdotbs = sm * bx + vy_star * by_star + vz_star * bz_star;
-  const R e_star =
-      (s_minus_vx * e - pt * vx + pt_star * sm + bx * (vdotb - vsdotbs)) /
-      s_minus_sm;
+  const R e_star = Div<S, R>::use(
+      s_minus_vx * e - pt * vx + pt_star * sm + bx * (vdotb - vsdotbs), s_minus_sm, dsm, miss);
   Us[RHO] = rho_star;
   Us[MX] = rho_star * sm;
   Us[MY] = rho_star * vy_star;
@@ -175,13 +262,13 @@ MHD_DEV int hlld_star(const Phys<R>& c, R e, const R W[8], R s, R sm,
   Us[BX] = bx;
   Us[BY] = by_star;
   Us[BZ] = bz_star;
-  return positive_pressure(c, Us) ? HLLD_OK : HLLD_NONPHYSICAL;
+  return positive_pressure<S>(c, Us, drs, false, miss) ? HLLD_OK : HLLD_NONPHYSICAL;
 }
 
 // HLL average of the outer states (mhd_common.hpp:338-347).
-template <class R>
+template <bool S, class R>
 MHD_DEV void hll_average(const R UL[], const R UR[], const R FL[8],
-                         const R FR[8], R sL, R sR, R F[8]) {
+                         const R FR[8], R sL, R sR, R F[8], bool& miss) {
   if (sL >= (R)0.0) {
 #pragma unroll
     for (int k = 0; k < 8; k++) F[k] = FL[k];
@@ -190,28 +277,34 @@ MHD_DEV void hll_average(const R UL[], const R UR[], const R FL[8],
     for (int k = 0; k < 8; k++) F[k] = FR[k];
   } else {
     const R denom = sR - sL;
+    Den<R> dd;
+    F[0] = Div<S, R>::rec(sR * FL[0] - sL * FR[0] + sL * sR * (UR[0] - UL[0]), denom, dd);
 #pragma unroll
-    for (int k = 0; k < 8; k++)
-      F[k] = (sR * FL[k] - sL * FR[k] + sL * sR * (UR[k] - UL[k])) / denom;
+    for (int k = 1; k < 8; k++)
+      F[k] = Div<S, R>::use(sR * FL[k] - sL * FR[k] + sL * sR * (UR[k] - UL[k]), denom, dd,
+                            miss);
   }
 }
 
 // x-normal HLL or HLLD flux of the 8 MHD slots of UL/UR (UL[BX], UR[BX]
 // already replaced by the GLM normal field).  HLLD follows
-// mhd_common.hpp:518-654; HLL follows mhd_common.hpp:323-348.
-template <class R, bool HLLD>
-MHD_DEV int mhd_flux_x(const Phys<R>& c, const R UL[], const R UR[], R F[8]) {
+// mhd_common.hpp:518-654; HLL follows mhd_common.hpp:323-348.  With S the
+// divisions by one denominator share its reciprocal (Div); `miss` marks a
+// lane whose result must be evaluated again with S = false.
+template <class R, bool HLLD, bool S>
+MHD_DEV int mhd_flux_x(const Phys<R>& c, const R UL[], const R UR[], R F[8], bool& miss) {
   R WL[8], WR[8], FL[8], FR[8];
-  cons_to_prim<R, 8>(c, UL, WL);
-  cons_to_prim<R, 8>(c, UR, WR);
+  Den<R> drL, drR;  // reciprocals of the outer densities
+  cons_to_prim<R, 8, S>(c, UL, WL, drL, false, miss);
+  cons_to_prim<R, 8, S>(c, UR, WR, drR, false, miss);
   phys_flux_x(WL, UL[EN], FL);
   phys_flux_x(WR, UR[EN], FR);
-  const R cfL = fast_speed(c, WL, WL[5]);
-  const R cfR = fast_speed(c, WR, WR[5]);
+  const R cfL = fast_speed<S>(c, WL, WL[5], drL, true, miss);
+  const R cfR = fast_speed<S>(c, WR, WR[5], drR, true, miss);
   const R sL = smin(WL[1] - cfL, WR[1] - cfR);
   const R sR = smax(WL[1] + cfL, WR[1] + cfR);
   if (!HLLD || sL >= (R)0.0 || sR <= (R)0.0) {
-    hll_average(UL, UR, FL, FR, sL, sR, F);
+    hll_average<S>(UL, UR, FL, FR, sL, sR, F, miss);
     return HLLD_OK;
   }
   const R rhoL = smax(WL[0], c.rho_floor);
@@ -227,6 +320,7 @@ MHD_DEV int mhd_flux_x(const Phys<R>& c, const R UL[], const R UR[], R F[8]) {
   const R b_scale = sqrt_(smax(b2L, b2R));
   int status = HLLD_OK;
   R ULs[8], URs[8];
+  Den<R> drLs, drRs;  // reciprocals of the star densities
   R sLs = (R)0, sRs = (R)0, sm = (R)0, sqrt_rhoLs = (R)0, sqrt_rhoRs = (R)0;
   R sgn_bx = (R)1;
   if (hlld_small(c, denom_sm, denom_sm_scale) || hlld_small(c, bx, b_scale)) {
@@ -237,8 +331,8 @@ MHD_DEV int mhd_flux_x(const Phys<R>& c, const R UL[], const R UR[], R F[8]) {
     const R pt_star_L = ptL + rhoL * (sL - vxL) * (sm - vxL);
     const R pt_star_R = ptR + rhoR * (sR - vxR) * (sm - vxR);
     const R pt_star = (R)0.5 * (pt_star_L + pt_star_R);
-    const int ls = hlld_star(c, UL[EN], WL, sL, sm, pt_star, bx, ULs);
-    const int rs = hlld_star(c, UR[EN], WR, sR, sm, pt_star, bx, URs);
+    const int ls = hlld_star<S>(c, UL[EN], WL, sL, sm, pt_star, bx, ULs, drLs, miss);
+    const int rs = hlld_star<S>(c, UR[EN], WR, sR, sm, pt_star, bx, URs, drRs, miss);
     if (ls != HLLD_OK || rs != HLLD_OK) {
       status = (ls != HLLD_OK) ? ls : rs;
     } else {
@@ -251,24 +345,28 @@ MHD_DEV int mhd_flux_x(const Phys<R>& c, const R UL[], const R UR[], R F[8]) {
     }
   }
   if (status != HLLD_OK) {
-    hll_average(UL, UR, FL, FR, sL, sR, F);
+    hll_average<S>(UL, UR, FL, FR, sL, sR, F, miss);
     return status;
   }
   const R denom = sqrt_rhoLs + sqrt_rhoRs;
-  const R vyLs = ULs[MY] / ULs[RHO];
-  const R vzLs = ULs[MZ] / ULs[RHO];
+  const R vyLs = Div<S, R>::use(ULs[MY], ULs[RHO], drLs, miss);
+  const R vzLs = Div<S, R>::use(ULs[MZ], ULs[RHO], drLs, miss);
   const R byLs = ULs[BY], bzLs = ULs[BZ];
-  const R vyRs = URs[MY] / URs[RHO];
-  const R vzRs = URs[MZ] / URs[RHO];
+  const R vyRs = Div<S, R>::use(URs[MY], URs[RHO], drRs, miss);
+  const R vzRs = Div<S, R>::use(URs[MZ], URs[RHO], drRs, miss);
   const R byRs = URs[BY], bzRs = URs[BZ];
+  Den<R> dd;
   const R vyD =
-      (sqrt_rhoLs * vyLs + sqrt_rhoRs * vyRs + sgn_bx * (byRs - byLs)) / denom;
+      Div<S, R>::rec(sqrt_rhoLs * vyLs + sqrt_rhoRs * vyRs + sgn_bx * (byRs - byLs), denom, dd);
   const R vzD =
-      (sqrt_rhoLs * vzLs + sqrt_rhoRs * vzRs + sgn_bx * (bzRs - bzLs)) / denom;
-  const R byD = (sqrt_rhoLs * byRs + sqrt_rhoRs * byLs +
-                 sgn_bx * sqrt_rhoLs * sqrt_rhoRs * (vyRs - vyLs)) / denom;
-  const R bzD = (sqrt_rhoLs * bzRs + sqrt_rhoRs * bzLs +
-                 sgn_bx * sqrt_rhoLs * sqrt_rhoRs * (vzRs - vzLs)) / denom;
+      Div<S, R>::use(sqrt_rhoLs * vzLs + sqrt_rhoRs * vzRs + sgn_bx * (bzRs - bzLs), denom, dd,
+                     miss);
+  const R byD = Div<S, R>::use(sqrt_rhoLs * byRs + sqrt_rhoRs * byLs +
+                                   sgn_bx * sqrt_rhoLs * sqrt_rhoRs * (vyRs - vyLs),
+                               denom, dd, miss);
+  const R bzD = Div<S, R>::use(sqrt_rhoLs * bzRs + sqrt_rhoRs * bzLs +
+                                   sgn_bx * sqrt_rhoLs * sqrt_rhoRs * (vzRs - vzLs),
+                               denom, dd, miss);
   R ULD[8], URD[8];
 #pragma unroll
   for (int k = 0; k < 8; k++) { ULD[k] = ULs[k]; URD[k] = URs[k]; }
@@ -285,8 +383,9 @@ MHD_DEV int mhd_flux_x(const Phys<R>& c, const R UL[], const R UR[], R F[8]) {
   const R vdotbRs = sm * bx + vyRs * byRs + vzRs * bzRs;
   ULD[EN] = ULs[EN] - sgn_bx * sqrt_rhoLs * (vdotbLs - vdotbD);
   URD[EN] = URs[EN] + sgn_bx * sqrt_rhoRs * (vdotbRs - vdotbD);
-  if (!positive_pressure(c, ULD) || !positive_pressure(c, URD)) {
-    hll_average(UL, UR, FL, FR, sL, sR, F);
+  if (!positive_pressure<S>(c, ULD, drLs, true, miss) ||
+      !positive_pressure<S>(c, URD, drRs, true, miss)) {
+    hll_average<S>(UL, UR, FL, FR, sL, sR, F, miss);
     return HLLD_NONPHYSICAL;
   }
   if (sLs >= (R)0.0) {
@@ -311,8 +410,10 @@ MHD_DEV int mhd_flux_x(const Phys<R>& c, const R UL[], const R UR[], R F[8]) {
 // state of the (Bn, psi) subsystem, Bn replaced on both sides, then the MHD
 // flux; F[Bn] = psi*, F[psi] = ch^2 Bn*.  For a y face the caller passes
 // swapped states and swaps the flux back (mhd_glm_common.hpp:99-126).
-template <class R, bool HLLD>
-MHD_DEV int glm_flux_x(const Phys<R>& c, R UL[9], R UR[9], R ch, R F[9]) {
+// With S the divisions share reciprocals and `miss` marks a lane to be
+// evaluated again with S = false (Div).
+template <class R, bool HLLD, bool S = false>
+MHD_DEV int glm_flux_x(const Phys<R>& c, R UL[9], R UR[9], R ch, R F[9], bool& miss) {
   const R ch_safe = smax(ch, (R)1.0e-12);
   const R bnStar =
       (R)0.5 * (UL[BX] + UR[BX]) - (R)0.5 * (UR[PSI] - UL[PSI]) / ch_safe;
@@ -320,10 +421,15 @@ MHD_DEV int glm_flux_x(const Phys<R>& c, R UL[9], R UR[9], R ch, R F[9]) {
       (R)0.5 * (UL[PSI] + UR[PSI]) - (R)0.5 * ch_safe * (UR[BX] - UL[BX]);
   UL[BX] = bnStar;
   UR[BX] = bnStar;
-  const int status = mhd_flux_x<R, HLLD>(c, UL, UR, F);
+  const int status = mhd_flux_x<R, HLLD, S>(c, UL, UR, F, miss);
   F[BX] = psiStar;
   F[PSI] = ch * ch * bnStar;
   return status;
+}
+template <class R, bool HLLD>
+MHD_DEV int glm_flux_x(const Phys<R>& c, R UL[9], R UR[9], R ch, R F[9]) {
+  bool miss = false;
+  return glm_flux_x<R, HLLD, false>(c, UL, UR, ch, F, miss);
 }
 
 // Usability of reconstructed primitives (mhd2d_glm_cpu.cpp:105-111).

@@ -111,6 +111,22 @@ template <> __device__ __forceinline__ float bits_speed<float>(unsigned long lon
   return __uint_as_float((unsigned)b);
 }
 
+// Wave-uniform copies (readfirstlane): values loaded from memory that the
+// compiler cannot prove unclobbered (inside the persistent kernels' loops)
+// would otherwise occupy vector registers.
+__device__ __forceinline__ int uni(int v) { return __builtin_amdgcn_readfirstlane(v); }
+__device__ __forceinline__ unsigned long long uni(unsigned long long v) {
+  const unsigned lo = (unsigned)__builtin_amdgcn_readfirstlane((int)(unsigned)v);
+  const unsigned hi = (unsigned)__builtin_amdgcn_readfirstlane((int)(unsigned)(v >> 32));
+  return ((unsigned long long)hi << 32) | lo;
+}
+__device__ __forceinline__ size_t uni_size(size_t v) { return (size_t)uni((unsigned long long)v); }
+__device__ __forceinline__ double uni(double v) {
+  return __longlong_as_double((long long)uni((unsigned long long)__double_as_longlong(v)));
+}
+__device__ __forceinline__ float uni(float v) {
+  return __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(v)));
+}
 template <class R>
 __device__ __forceinline__ Phys<R> make_phys(double tol) {
   Phys<R> c;
@@ -212,22 +228,6 @@ __device__ __forceinline__ double cell_speed(const double U9[9], bool& valid) {
 }
 }  // namespace refmix
 
-// Wave-uniform copies (readfirstlane): values loaded from memory that the
-// compiler cannot prove unclobbered (inside the persistent kernels' loops)
-// would otherwise occupy vector registers.
-__device__ __forceinline__ int uni(int v) { return __builtin_amdgcn_readfirstlane(v); }
-__device__ __forceinline__ unsigned long long uni(unsigned long long v) {
-  const unsigned lo = (unsigned)__builtin_amdgcn_readfirstlane((int)(unsigned)v);
-  const unsigned hi = (unsigned)__builtin_amdgcn_readfirstlane((int)(unsigned)(v >> 32));
-  return ((unsigned long long)hi << 32) | lo;
-}
-__device__ __forceinline__ size_t uni_size(size_t v) { return (size_t)uni((unsigned long long)v); }
-__device__ __forceinline__ double uni(double v) {
-  return __longlong_as_double((long long)uni((unsigned long long)__double_as_longlong(v)));
-}
-__device__ __forceinline__ float uni(float v) {
-  return __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(v)));
-}
 template <class P>
 __device__ __forceinline__ P* uni_ptr(P* p) {
   return (P*)uni((unsigned long long)(uintptr_t)p);
@@ -742,46 +742,61 @@ __device__ __forceinline__ void stage_tile(const StageArgs<Pol>& a, int tile,
       out[7] = sP[7 * HN + cell];
       out[8] = sP[8 * HN + cell];
     };
-    // Face states in RP, widened to the flux type R (a no-op but for the
-    // reference-mixed policy, whose face states are FP32 and flux FP64).
-    R UL[9], UR[9];
-    if (MUSCL) {
-      RP Pa[9], Pb[9], Pc[9], Pf[9];
-      load(base, Pa);
-      load(base + stride, Pb);
-      load(base + 2 * stride, Pc);
-      reconstruct_face<RP, true>(cp, Pa, Pb, Pc, Pf);
-      face_cons<Pol>(cp, Pf, UL);
-      load(base + 3 * stride, Pa);
-      reconstruct_face<RP, false>(cp, Pb, Pc, Pa, Pf);
-      face_cons<Pol>(cp, Pf, UR);
-    } else {
-      RP Ua[9], Ub[9];
-      load(base + stride, Ua);
-      load(base + 2 * stride, Ub);
+    // One evaluation of the face: states in RP, widened to the flux type R
+    // (a no-op but for the reference-mixed policy, whose face states are
+    // FP32 and flux FP64), the flux, its store to LDS.  With sharing (FP64:
+    // divisions by one denominator share its reciprocal) a lane may come
+    // back with `miss` (an operand range where the sharing does not hold);
+    // the face is then evaluated again from LDS with plain division.  Only
+    // the face index and the status live across, so the second evaluation
+    // adds no register pressure.
+    auto evaluate = [&](auto share, bool& miss) -> int {
+      R UL[9], UR[9];
+      if (MUSCL) {
+        RP Pa[9], Pb[9], Pc[9], Pf[9];
+        load(base, Pa);
+        load(base + stride, Pb);
+        load(base + 2 * stride, Pc);
+        reconstruct_face<RP, true>(cp, Pa, Pb, Pc, Pf);
+        face_cons<Pol>(cp, Pf, UL);
+        load(base + 3 * stride, Pa);
+        reconstruct_face<RP, false>(cp, Pb, Pc, Pa, Pf);
+        face_cons<Pol>(cp, Pf, UR);
+      } else {
+        RP Ua[9], Ub[9];
+        load(base + stride, Ua);
+        load(base + 2 * stride, Ub);
 #pragma unroll
-      for (int k = 0; k < 9; k++) {
-        UL[k] = (R)Ua[k];
-        UR[k] = (R)Ub[k];
+        for (int k = 0; k < 9; k++) {
+          UL[k] = (R)Ua[k];
+          UR[k] = (R)Ub[k];
+        }
       }
+      R F[9];
+      const int st = glm_flux_x<R, HLLD, decltype(share)::value>(c, UL, UR, ch, F, miss);
+      const int p1 = (ydir ? 2 : 1) * NF, p2 = (ydir ? 1 : 2) * NF;
+      const int p5 = (ydir ? 6 : 5) * NF, p6 = (ydir ? 5 : 6) * NF;
+      sF[f] = (RS)F[0];
+      sF[p1 + f] = (RS)F[1];
+      sF[p2 + f] = (RS)F[2];
+      sF[3 * NF + f] = (RS)F[3];
+      sF[4 * NF + f] = (RS)F[4];
+      sF[p5 + f] = (RS)F[5];
+      sF[p6 + f] = (RS)F[6];
+      sF[7 * NF + f] = (RS)F[7];
+      sF[8 * NF + f] = (RS)F[8];
+      return st;
+    };
+    bool miss = false;
+    int st = evaluate(std::integral_constant<bool, sizeof(R) == 8>{}, miss);
+    if (miss) {
+      bool unused = false;
+      st = evaluate(std::integral_constant<bool, false>{}, unused);
     }
-    R F[9];
-    const int st = glm_flux_x<R, HLLD>(c, UL, UR, ch, F);
     if (HLLD && owned) {
       degen += (st == HLLD_DEGENERATE);
       nonphys += (st == HLLD_NONPHYSICAL);
     }
-    const int p1 = (ydir ? 2 : 1) * NF, p2 = (ydir ? 1 : 2) * NF;
-    const int p5 = (ydir ? 6 : 5) * NF, p6 = (ydir ? 5 : 6) * NF;
-    sF[f] = (RS)F[0];
-    sF[p1 + f] = (RS)F[1];
-    sF[p2 + f] = (RS)F[2];
-    sF[3 * NF + f] = (RS)F[3];
-    sF[4 * NF + f] = (RS)F[4];
-    sF[p5 + f] = (RS)F[5];
-    sF[p6 + f] = (RS)F[6];
-    sF[7 * NF + f] = (RS)F[7];
-    sF[8 * NF + f] = (RS)F[8];
   }
   __syncthreads();
 
