@@ -900,9 +900,17 @@ __device__ __forceinline__ void stage_tile(const StageArgs<Pol>& a, int tile,
   }
 }
 
+// Waves per SIMD every stage kernel is register-budgeted for: what LDS
+// allows (FP64: 64 KB per 4-wave block, 2 blocks per CU; FP32 flux: 32 KB,
+// 4 blocks).  Without the target a kernel a register or two over 256 (the
+// FP64 list and queue kernels, whose loops keep a little more live) runs at
+// half the occupancy.
+template <class Pol> struct StageWaves { static constexpr int value = sizeof(typename Pol::R) == 8 ? 2 : 4; };
+
 // One instance, one block per tile (XCD-banded tile order).
 template <class Pol, bool MUSCL, bool HLLD, int STAGE, int TX, int TY>
-__global__ __launch_bounds__((TX * TY + 63) / 64 * 64) void k_stage(StageArgs<Pol> a) {
+__global__ __launch_bounds__((TX * TY + 63) / 64 * 64)
+__attribute__((amdgpu_waves_per_eu(StageWaves<Pol>::value))) void k_stage(StageArgs<Pol> a) {
   __shared__ TileSharedOf<Pol, TX, TY> sh;
   Ctrl<typename Pol::T>* ctrl = a.ctrl;
   if (ctrl->run != RUN_ACTIVE) return;
@@ -947,7 +955,8 @@ struct ListSegs {
 };
 
 template <class Pol, bool MUSCL, bool HLLD, int STAGE, int TX, int TY>
-__global__ __launch_bounds__((TX * TY + 63) / 64 * 64) void k_stage_list(ListSegs<Pol> g) {
+__global__ __launch_bounds__((TX * TY + 63) / 64 * 64)
+__attribute__((amdgpu_waves_per_eu(StageWaves<Pol>::value))) void k_stage_list(ListSegs<Pol> g) {
   __shared__ TileSharedOf<Pol, TX, TY> sh;
   int n[kListSegs], total = 0;
 #pragma unroll
@@ -1174,16 +1183,10 @@ __device__ __forceinline__ int find_instance(const int* prefix, int n, int item)
   return lo;
 }
 
-// Waves per SIMD the persistent kernels are register-budgeted for: the
-// single-grid kernels' occupancy, which LDS allows (FP64: 64 KB per 4-wave
-// block, 2 blocks per CU; FP32 flux: 32 KB, 4 blocks).  Without the target
-// the queue loop costs the FP64 kernel one register too many (257) and
-// half its occupancy.
-template <class Pol> struct QueueWaves { static constexpr int value = sizeof(typename Pol::R) == 8 ? 2 : 4; };
 
 template <class Pol, bool MUSCL, bool HLLD, int STAGE, int TX, int TY>
 __global__ __launch_bounds__((TX * TY + 63) / 64 * 64)
-__attribute__((amdgpu_waves_per_eu(QueueWaves<Pol>::value))) void k_stage_queue(EnsArgs<Pol> e) {
+__attribute__((amdgpu_waves_per_eu(StageWaves<Pol>::value))) void k_stage_queue(EnsArgs<Pol> e) {
   __shared__ TileSharedOf<Pol, TX, TY> sh;
   __shared__ int s_item, s_count;
   // Items are claimed kClaim at a time (consecutive tiles, nearly always of
@@ -1240,7 +1243,8 @@ __attribute__((amdgpu_waves_per_eu(QueueWaves<Pol>::value))) void k_stage_queue(
 // FP64 fallback of an ensemble: re-evaluates the work items a mixed stage
 // queued (count in queue[16 + STAGE - 1]); a fixed grid strides over them.
 template <class Pol, bool MUSCL, bool HLLD, int STAGE, int TX, int TY>
-__global__ __launch_bounds__((TX * TY + 63) / 64 * 64) void k_queue_redo(EnsArgs<Pol> e,
+__global__ __launch_bounds__((TX * TY + 63) / 64 * 64)
+__attribute__((amdgpu_waves_per_eu(StageWaves<Pol>::value))) void k_queue_redo(EnsArgs<Pol> e,
                                                                           const int* list) {
   __shared__ TileSharedOf<Pol, TX, TY> sh;
   const int count = __builtin_amdgcn_readfirstlane((int)e.queue[16 + STAGE - 1]);
