@@ -695,15 +695,25 @@ __device__ __forceinline__ void stage_tile(const StageArgs<Pol>& a, int tile,
   const int i0 = tx * TX, j0 = ty * TY;
   const int tid = (int)threadIdx.x;
 
-  // Phase A: tile + halo (no corners) -> primitives in LDS.
-  for (int h = tid; h < HN; h += NTHR) {
-    const int r = h / HX, cc = h - r * HX;
-    if ((r < 2 || r >= TY + 2) && (cc < 2 || cc >= TX + 2)) continue;
-    const size_t g = (size_t)wrap(j0 + r - 2, ny) * nx + wrap(i0 + cc - 2, nx);
-    CellIO<Pol> u;
-    u.load(a.src, ps, g);
+  // Phase A: tile + halo (no corners) -> primitives in LDS.  A thread's
+  // cells (at most kCells) are all loaded before any is recovered, so their
+  // loads are in flight together: one memory round trip, not one per cell.
+  constexpr int kCells = (HN + NTHR - 1) / NTHR;
+  static_assert(kCells <= 2, "phase A holds at most two cells per thread");
+  CellIO<Pol> u[kCells];
+  bool have[kCells];
+#pragma unroll
+  for (int q = 0; q < kCells; q++) {
+    const int h = tid + q * NTHR, r = h / HX, cc = h - r * HX;
+    have[q] = h < HN && !((r < 2 || r >= TY + 2) && (cc < 2 || cc >= TX + 2));
+    if (have[q]) u[q].load(a.src, ps, (size_t)wrap(j0 + r - 2, ny) * nx + wrap(i0 + cc - 2, nx));
+  }
+#pragma unroll
+  for (int q = 0; q < kCells; q++) {
+    if (!have[q]) continue;
+    const int h = tid + q * NTHR;
     RP P[9];
-    recover<MUSCL>(cp, u, a.lowbeta_ratio, P);
+    recover<MUSCL>(cp, u[q], a.lowbeta_ratio, P);
 #pragma unroll
     for (int k = 0; k < 9; k++) sP[k * HN + h] = P[k];
   }
