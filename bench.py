@@ -25,6 +25,9 @@ reported on the same JSON line:
   selector_run -- Orszag-Tang 2048^2 to t=0.5, FP64 vs FP64-FMA, mixed, and
                   mixed with the learned tile selector: rates, FP64 tile
                   share, audits, relative L2;
+  selector_breakdown -- (--selector-breakdown N) whole runs at N^2 splitting
+                  the selector's cost per step over the mixed step into its
+                  own launches, the audits and the FP64 tiles;
   ensemble     -- the batched workload: 64 Orszag-Tang instances (seeded
                   sizes in {128..1024}^2, B amplitude and sine phases varied)
                   advanced together to t=0.5 by the persistent work-queue
@@ -176,6 +179,44 @@ def accuracy_check(mhdglm, n, forms, problem="ot", t_end=0.5):
     return out
 
 
+def selector_breakdown(mhdglm, n, t_end=0.5):
+    """Whole Orszag-Tang runs at n^2 to t_end in the mixed form, and in the
+    mixed form with the selector as shipped, with audits off, with nothing
+    flagged (threshold 2), and with neither: what a step costs over the
+    mixed step for the selector's own launches, the audits and the FP64
+    tiles."""
+    from mhdglm import selector
+
+    model = selector.load(selector.DEFAULT_MODEL)
+    variants = [("mixed", None), ("selector", {}), ("selector_no_audits", {"audit_frac": 0.0}),
+                ("selector_nothing_flagged", {"threshold": 2.0}),
+                ("selector_launches_only", {"threshold": 2.0, "audit_frac": 0.0})]
+    runs = {}
+    for name, kw in variants:
+        s = mhdglm.Solver(mhdglm.SolverConfig(nx=n, ny=n, precision=mhdglm.PREC_MIXED))
+        s.init_problem("ot")
+        if kw is not None:
+            s.configure_selector(model, **kw)
+        t0 = time.perf_counter()
+        st = s.advance(t_end)
+        wall = time.perf_counter() - t0
+        row = {"steps": st.steps, "code": st.code, "wall_s": round(wall, 3),
+               "us_per_step": round(wall / max(st.steps, 1) * 1e6, 2)}
+        if kw is not None:
+            sel = s.selector_stats()
+            row["fp64_tile_fraction"] = round(sel["fp64_tile_steps"] / max(sel["steps"] * s.selector_tiles(), 1), 5)
+            row["audits"], row["misses"] = sel["audits"], sel["misses"]
+        s.close()
+        runs[name] = row
+        log(f"bench: selector breakdown {n}^2 {name}: {row}")
+    us = {k: v["us_per_step"] for k, v in runs.items()}
+    return {"n": n, "t_end": t_end, "runs": runs, "us_per_step_over_mixed": {
+        "selector_launches": round(us["selector_launches_only"] - us["mixed"], 2),
+        "audits": round(us["selector_nothing_flagged"] - us["selector_launches_only"], 2),
+        "fp64_tiles": round(us["selector_no_audits"] - us["selector_launches_only"], 2),
+        "all": round(us["selector"] - us["mixed"], 2)}}
+
+
 ENSEMBLE_SIZES = (128, 192, 256, 384, 512, 768, 1024)
 
 
@@ -241,6 +282,8 @@ def main():
     ap.add_argument("--kh-t-end", type=float, default=5.0)
     ap.add_argument("--kh-forms", default="fp64_fma,mixed",
                     help="forms of the Kelvin-Helmholtz whole runs, against FP64")
+    ap.add_argument("--selector-breakdown", type=int, default=0,
+                    help="whole runs at this size splitting the selector's cost (0 to skip)")
     ap.add_argument("--selector-n", type=int, default=2048,
                     help="Orszag-Tang whole runs to t=0.5, FP64 vs selector (0 to skip)")
     ap.add_argument("--ensemble", type=int, default=64,
@@ -328,6 +371,7 @@ def main():
     selector_run = None
     if args.selector_n:
         selector_run = accuracy_check(mhdglm, args.selector_n, ["fp64_fma", "mixed", "selector"])
+    breakdown = selector_breakdown(mhdglm, args.selector_breakdown) if args.selector_breakdown else None
     ensemble = None
     if args.ensemble:
         ensemble = run_ensemble(mhdglm, torch, args.ensemble, args.ensemble_form,
@@ -358,6 +402,7 @@ def main():
         "large": large,
         "kh": kh,
         "selector_run": selector_run,
+        "selector_breakdown": breakdown,
         "ensemble": ensemble,
         "data": "synthetic: Orszag-Tang initial condition evaluated on the host",
         "config": {

@@ -14,7 +14,10 @@
 //   k_mlp_grad       forward (MFMA) + backward (FP32) of a minibatch,
 //                    gradients accumulated into a global buffer
 //   k_adam           Adam update of the FP32 master weights
-//   k_audit_compare  audit of FP32 tiles against an FP64 recomputation
+//   k_route          FP64 and mixed tile lists of the current flags
+//   k_audit_pick     the step's audit sample (and the control-block snapshot)
+//   k_tile_errors    (audit mode) audited FP32 tiles against their FP64
+//                    recomputation; misses become sticky FP64 tiles
 #include <hip/hip_runtime.h>
 
 #include "mhd_select.h"
