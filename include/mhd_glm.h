@@ -178,15 +178,18 @@ int mhd_fast_speed_batch(int precision, int dir, int n, const void* W,
  * stage-kernel tile and step an MLP 12 -> 64 -> 64 -> 1 (BF16 MFMA, FP32
  * accumulate, sigmoid) scores 12 tile features (refreshed every 8 steps);
  * tiles scoring above the threshold are evaluated by the FP64 kernels, the
- * rest by the FP32-flux kernels.  A fraction of the FP32 tiles is recomputed in FP64 each step
- * (audit); a scaled difference above the budget makes the tile FP64 for the
- * rest of the run.  params holds mhd_mlp_param_count() FP32 weights in the
+ * rest by the FP32-flux kernels.  A fraction audit_frac of the FP32 tiles
+ * (a systematic sample, new each step) is recomputed in FP64 (audit); a
+ * scaled difference above the budget makes the tile FP64 for the rest of
+ * the run.  params holds mhd_mlp_param_count() FP32 weights in the
  * layout W1[16][64], b1[64], W2[64][64], b2[64], w3[64], b3; mean/stdev
  * standardise the 12 features.  params NULL turns the selector off. */
 int mhd_selector_configure(mhd_ctx* ctx, const float* params, const float* mean,
                            const float* stdev, float threshold, float audit_frac,
                            float audit_budget);
-/* out[0..3]: audited tiles, audit misses, FP64-routed tile-steps, steps. */
+/* out[0..3]: audited tiles, audit misses, FP64-routed tile-steps, steps --
+ * counted over the steps the run actually took (the no-op tail of a batch of
+ * steps after the run finished or halted counts nothing). */
 int mhd_selector_stats(mhd_ctx* ctx, unsigned long long* out4);
 /* Tiles of the selector's decomposition of this context (-1 if not MIXED). */
 int mhd_selector_tiles(mhd_ctx* ctx);
