@@ -94,7 +94,8 @@ enum TileShape {
 };
 constexpr int kDefaultTile = TILE_16x15;
 inline void tile_dims(int tile, int& tx, int& ty) {
-  static const int dims[TILE_COUNT][2] = {{16, 16}, {32, 8}, {16, 8}, {16, 15}, {32, 7}, {16, 7}};
+  static constexpr int dims[][2] = {{16, 16}, {32, 8}, {16, 8}, {16, 15}, {32, 7}, {16, 7}};
+  static_assert(sizeof(dims) / sizeof(dims[0]) == TILE_COUNT, "one entry per tile shape");
   if (tile < 0 || tile >= TILE_COUNT) tile = kDefaultTile;
   tx = dims[tile][0];
   ty = dims[tile][1];

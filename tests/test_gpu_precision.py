@@ -119,3 +119,30 @@ def test_checkpoint_restart_bitwise(mhd, tmp_path, form):
     with pytest.raises(mhd.MhdError):
         wrong.load(path)
     wrong.close()
+
+
+@pytest.mark.parametrize("form,tol", [("mixed", 1e-6), ("mixed_kahan", 1e-6), ("fp64_fma", 1e-12)])
+@pytest.mark.parametrize("nx,ny", [(64, 48), (70, 33)])
+def test_forms_with_every_tile_shape(mhd, form, tol, nx, ny):
+    """Each form runs with every tile shape and stays within its tolerance of
+    FP64 (which is bitwise the same for every shape).  The contraction-on
+    forms are not bitwise across shapes: each shape is a separate
+    instantiation, and the compiler may contract its FMAs differently."""
+    say(f"{form} {nx}x{ny} all tile shapes")
+    ref = None
+    for tile in range(6):
+        u = {}
+        for name, prec in (("fp64", mhd.PREC_FP64), (form, mhd.PRECISIONS[form])):
+            s = mhd.Solver(mhd.SolverConfig(nx=nx, ny=ny, precision=prec, tile=tile))
+            s.init_problem("ot")
+            st = s.advance(0.5, 25)
+            u[name] = s.get_state(0).astype(np.float64)
+            s.close()
+            assert st.code == 0 and st.steps == 25 and st.fallback_tiles == 0, (name, tile)
+        if ref is None:
+            ref = u["fp64"]
+        assert np.array_equal(u["fp64"].view(np.uint64), ref.view(np.uint64)), tile
+        for k in range(8):
+            scale = np.sqrt(np.mean(ref[k] ** 2)) + 1e-300
+            err = np.sqrt(np.mean((u[form][k] - ref[k]) ** 2)) / scale
+            assert err < tol, (tile, k, err)
