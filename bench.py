@@ -254,6 +254,9 @@ def run_ensemble(mhdglm, torch, n, form, t_end):
         "wall_s": round(wall, 3), "cell_updates": updates,
         "value": round(updates / wall, 1), "unit": "cell-updates/s",
         "all_done": all(st.done == 1 for st in sts), "codes": sorted(set(codes)),
+        "instances_done": sum(1 for st in sts if st.done == 1),
+        "instances_halted": [{"instance": i, "size": sizes[i], "code": st.code, "t": round(st.t, 4)}
+                             for i, st in enumerate(sts) if st.code != 0],
         "steps_min_max": [min(st.steps for st in sts), max(st.steps for st in sts)],
     }
 
