@@ -661,7 +661,7 @@ struct TileShared {
   static constexpr int NTHR = NW * 64;  // threads per block: whole waves
   RP sP[9 * HN];
   RS sF[9 * NF];
-  unsigned long long sRed[3 * NW];
+  unsigned long long sRed[4 * NW];  // per wave: counters, speed, valid, fallback vote
 };
 template <class Pol, int TX, int TY>
 using TileSharedOf = TileShared<typename Pol::RP, typename Pol::RS, TX, TY>;
@@ -862,26 +862,8 @@ __device__ __forceinline__ void stage_tile(const StageArgs<Pol>& a, int tile,
       }
     }
   }
-  if constexpr (Pol::kRedo) {
-    // FP64 fallback: a tile with an unusable cell writes nothing (its
-    // inputs -- the stage source and, in stage 2, its own cells of U^n --
-    // stay intact) and is queued for the FP64 kernel, which also publishes
-    // its counters and speeds.
-    if (a.redo_list && __syncthreads_or((unusable || a.redo_all) ? 1 : 0)) {
-      if (tid == 0) {
-        const int slot = atomicAdd(a.redo_counter, 1);
-        a.redo_list[slot] = a.redo_base + tile;
-        atomicAdd(&ctrl->redo_tiles, 1ull);
-      }
-      return;
-    }
-    if (live) {
-#pragma unroll
-      for (int k = 0; k < 9; k++) a.dst[k * a.plane + g] = out[k];
-    }
-  }
-
-  // Block reductions: one atomic per block and counter.
+  // Block reductions (one atomic per block and counter) and, with the FP64
+  // fallback, the tile's vote, over one barrier.
   const int lane = tid & 63, wid = tid >> 6;
   const unsigned wd = wave_sum_u32(degen), wn = wave_sum_u32(nonphys);
   unsigned long long wm = 0;
@@ -890,12 +872,41 @@ __device__ __forceinline__ void stage_tile(const StageArgs<Pol>& a, int tile,
     wm = wave_max_u64(sbits);
     wv = wave_sum_u32(svalid);
   }
+  bool voting = false;
+  unsigned long long wvote = 0;
+  if constexpr (Pol::kRedo) {
+    voting = a.redo_list != nullptr;
+    if (voting) wvote = __ballot((unusable || a.redo_all) ? 1 : 0);
+  }
   if (lane == 0) {
     sRed[wid] = ((unsigned long long)wn << 32) | wd;
     sRed[NW + wid] = wm;
     sRed[2 * NW + wid] = wv;
+    sRed[3 * NW + wid] = wvote;
   }
   __syncthreads();
+  if constexpr (Pol::kRedo) {
+    // FP64 fallback: a tile with an unusable cell writes nothing (its
+    // inputs -- the stage source and, in stage 2, its own cells of U^n --
+    // stay intact) and is queued for the FP64 kernel, which also publishes
+    // its counters and speeds.
+    if (voting) {
+      unsigned long long any = 0;
+      for (int w = 0; w < NW; w++) any |= sRed[3 * NW + w];
+      if (any) {
+        if (tid == 0) {
+          const int slot = atomicAdd(a.redo_counter, 1);
+          a.redo_list[slot] = a.redo_base + tile;
+          atomicAdd(&ctrl->redo_tiles, 1ull);
+        }
+        return;
+      }
+    }
+    if (live) {
+#pragma unroll
+      for (int k = 0; k < 9; k++) a.dst[k * a.plane + g] = out[k];
+    }
+  }
   if (tid == 0) {
     unsigned long long d = 0, n = 0, m = 0, v = 0;
     for (int w = 0; w < NW; w++) {
