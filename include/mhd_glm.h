@@ -73,6 +73,14 @@ typedef struct mhd_cfg {
                         (default), 1 = off (the failure halts, as in the
                         other forms), 2 = every tile (check mode: the step
                         is then the FP64 step) */
+  int fuse;          /* SSPRK2 stages per kernel launch (fusion depth).
+                        1: one launch per stage (U1 through HBM); 2: one
+                        launch per step, stage 1 recomputed on a 2-cell halo
+                        ring of each tile (MUSCL + HLLD, tile 16x15, not with
+                        fallback 2; the learned selector runs one stage per
+                        launch).  Results as with 1: bitwise in the exact
+                        forms, the same scheme in the mixed ones.  0 = the
+                        autotuned choice when tile is -1, else 1 */
 } mhd_cfg;
 
 /* Progress and diagnostics of a context. */
@@ -89,6 +97,10 @@ typedef struct mhd_status {
   int fail_step;            /* step index of the failure */
   unsigned long long fallback_tiles; /* MIXED: stage tiles re-evaluated in
                                         FP64 by the fallback, whole run */
+  unsigned long long fused_redo_steps; /* MIXED, fuse 2: steps in which a tile
+                                          met an unusable cell, recomputed by
+                                          the one-stage kernels (with their
+                                          FP64 fallback) */
 } mhd_status;
 
 typedef struct mhd_ctx mhd_ctx;
@@ -152,6 +164,10 @@ int mhd_checkpoint_load(mhd_ctx* ctx, const char* path);
 /* Tile shape a context of this form and size gets when cfg.tile is -1
  * (the autotuner's table, else the default).  No device needed. */
 int mhd_autotuned_tile(int precision, int nx, int ny);
+/* Stages per launch (1 or 2) a context of this form and size gets when
+ * cfg.tile is -1 and cfg.fuse is 0.  No device needed.  No reference
+ * counterpart (the reference launches one kernel per operation). */
+int mhd_autotuned_fuse(int precision, int nx, int ny);
 
 /* Measurement (no reference counterpart; the reference times its loop with
  * a host wall clock, mhd2d_glm_cpu.cpp:512,571-572): runs nsteps steps with

@@ -40,6 +40,12 @@ struct mhd_ctx {
   hipStream_t stream = nullptr;
   void* u = nullptr;       // device state U   (9 planes)
   void* u1 = nullptr;      // device state U1  (9 planes)
+  // Two-stage steps (cfg.fuse 2): the state alternates between u and ualt
+  // (Ctrl::cur); all_tiles lists every tile for the mixed form's
+  // recomputation of a voted-out step.
+  bool fused = false;
+  void* ualt = nullptr;
+  int* all_tiles = nullptr;
   void* ctrl = nullptr;    // device Ctrl<R>
   void* ctrl_host = nullptr;  // pinned host copy of the control block
   int* redo_lists = nullptr;  // MIXED FP64 fallback: 2 x ntiles tile indices
@@ -147,6 +153,7 @@ void fill_status_of(int hlld, int nx, int ny, const Ctrl<R>* h, mhd_status* st) 
   }
   st->fail_step = h->fail_step;
   st->fallback_tiles = h->redo_tiles;
+  st->fused_redo_steps = h->fz_redo_steps;
 }
 
 template <class R>
@@ -209,6 +216,25 @@ hipError_t download_state(const mhd::Launchers* L, const void* dev, size_t plane
 int read_ctrl(mhd_ctx* c) {
   HIP_OR_FAIL(c, hipMemcpyAsync(c->ctrl_host, c->ctrl, c->L->ctrl_bytes,
                                 hipMemcpyDeviceToHost, c->stream));
+  HIP_OR_FAIL(c, hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+// Two-stage steps leave the state in either buffer (Ctrl::cur).  Moves it
+// back to `u` (one device copy) so that readers of the state and the
+// single-stage paths find it there.
+int settle_state(mhd_ctx* c) {
+  if (!c->fused || !c->have_state) return 0;
+  if (int rc = read_ctrl(c)) return rc;
+  const size_t off =
+      c->L->time_is_float ? offsetof(Ctrl<float>, cur) : offsetof(Ctrl<double>, cur);
+  int* hcur = (int*)((char*)c->ctrl_host + off);
+  if (*hcur == 0) return 0;
+  const size_t bytes = (size_t)c->L->state_planes * c->plane * c->L->state_elem;
+  HIP_OR_FAIL(c, hipMemcpyAsync(c->u, c->ualt, bytes, hipMemcpyDeviceToDevice, c->stream));
+  *hcur = 0;
+  HIP_OR_FAIL(c, hipMemcpyAsync((char*)c->ctrl + off, hcur, sizeof(int), hipMemcpyHostToDevice,
+                                c->stream));
   HIP_OR_FAIL(c, hipStreamSynchronize(c->stream));
   return 0;
 }
@@ -297,6 +323,8 @@ hipError_t enqueue_step_selected(mhd_ctx* c) {
 
 hipError_t enqueue_step(mhd_ctx* c, hipEvent_t* ev = nullptr) {
   if (c->sel.on && !ev) return enqueue_step_selected(c);
+  if (c->fused)
+    return c->L->step_fused(c->grid, c->u, c->ualt, c->u1, c->ctrl, c->all_tiles, c->stream, ev);
   return c->L->step(c->grid, c->u, c->u1, c->ctrl, c->stream, ev);
 }
 
@@ -328,15 +356,17 @@ int ensure_graph(mhd_ctx* c, int batch) {
   return 0;
 }
 
-// Tile shape chosen by the autotuner (mhdglm.autotune) for a form and grid
-// size.  The table is the text file `autotune.txt` next to the library's
-// lib/ directory (or $MHD_AUTOTUNE_FILE): lines "form max_cells tile", the
-// first line with the form and max_cells >= cells wins; '#' starts a
-// comment.  Without a match the built-in default tile is used.
+// Tile shape and stages per launch chosen by the autotuner (mhdglm.autotune)
+// for a form and grid size.  The table is the text file `autotune.txt` next
+// to the library's lib/ directory (or $MHD_AUTOTUNE_FILE): lines "form
+// max_cells tile [fuse]", the first line with the form and max_cells >=
+// cells wins; '#' starts a comment.  Without a match the built-in default
+// tile and one stage per launch are used.
 struct TuneRow {
   int form;
   long long max_cells;
   int tile;
+  int fuse;  // stages per launch (1 or 2)
 };
 
 const std::vector<TuneRow>& tune_table() {
@@ -360,8 +390,9 @@ const std::vector<TuneRow>& tune_table() {
     while (std::fgets(line, sizeof(line), f)) {
       if (line[0] == '#') continue;
       TuneRow r;
-      if (std::sscanf(line, "%d %lld %d", &r.form, &r.max_cells, &r.tile) == 3 && r.tile >= 0 &&
-          r.tile < mhd::TILE_COUNT)
+      r.fuse = 1;
+      const int n = std::sscanf(line, "%d %lld %d %d", &r.form, &r.max_cells, &r.tile, &r.fuse);
+      if (n >= 3 && r.tile >= 0 && r.tile < mhd::TILE_COUNT && (r.fuse == 1 || r.fuse == 2))
         out.push_back(r);
     }
     std::fclose(f);
@@ -370,10 +401,20 @@ const std::vector<TuneRow>& tune_table() {
   return rows;
 }
 
-int tuned_tile(int form, long long cells) {
+const TuneRow* tuned_row(int form, long long cells) {
   for (const TuneRow& r : tune_table())
-    if (r.form == form && cells <= r.max_cells) return r.tile;
-  return mhd::kDefaultTile;
+    if (r.form == form && cells <= r.max_cells) return &r;
+  return nullptr;
+}
+
+int tuned_tile(int form, long long cells) {
+  const TuneRow* r = tuned_row(form, cells);
+  return r ? r->tile : mhd::kDefaultTile;
+}
+
+int tuned_fuse(int form, long long cells) {
+  const TuneRow* r = tuned_row(form, cells);
+  return r ? r->fuse : 1;
 }
 
 bool use_graphs() {
@@ -403,6 +444,10 @@ int mhd_autotuned_tile(int precision, int nx, int ny) {
   return tuned_tile(precision, (long long)nx * ny);
 }
 
+int mhd_autotuned_fuse(int precision, int nx, int ny) {
+  return tuned_fuse(precision, (long long)nx * ny);
+}
+
 const char* mhd_last_error(const mhd_ctx* ctx) {
   return ctx ? ctx->err.c_str() : kNoCtx;
 }
@@ -417,17 +462,35 @@ int mhd_create(const mhd_cfg* cfg, mhd_ctx** out) {
   if (!launchers_for(cfg->precision)) return 1;
   if (cfg->precision == MHD_PREC_REF_MIXED && !cfg->muscl) return 1;
   if ((long long)cfg->nx * cfg->ny > (long long)INT_MAX) return 1;
+  if (cfg->fuse < 0 || cfg->fuse > 2) return 1;
+  const long long ncells = (long long)cfg->nx * cfg->ny;
+  int tile = cfg->tile, fuse = cfg->fuse;
+  if (tile < 0 || tile >= mhd::TILE_COUNT) {
+    tile = tuned_tile(cfg->precision, ncells);
+    if (fuse == 0) fuse = tuned_fuse(cfg->precision, ncells);
+  }
+  if (fuse == 0) fuse = 1;
+  // Only the MUSCL + HLLD scheme has every tile shape; the others run the
+  // default tile (the selector's per-tile flags follow this shape).
+  if (!(cfg->muscl && cfg->hlld)) tile = mhd::kDefaultTile;
+  // Two stages per launch: MUSCL + HLLD on 16x15 tiles, not in the
+  // fallback's check mode.  Asked for explicitly elsewhere, it is an error;
+  // an autotuned choice falls back to one stage per launch.
+  const bool can_fuse =
+      cfg->muscl && cfg->hlld && tile == mhd::TILE_16x15 && cfg->fallback != 2;
+  if (fuse == 2 && !can_fuse) {
+    if (cfg->fuse == 2) return 1;
+    fuse = 1;
+  }
   mhd_ctx* c = new mhd_ctx();
   c->cfg = *cfg;
   c->L = launchers_for(cfg->precision);
   if (!(c->cfg.lowbeta_ratio > 0.0)) c->cfg.lowbeta_ratio = 16.0;
   if (c->cfg.batch_steps <= 0) c->cfg.batch_steps = 32;
   if (c->cfg.hlld_tol <= 0.0) c->cfg.hlld_tol = 1.0e-4;
-  if (c->cfg.tile < 0 || c->cfg.tile >= mhd::TILE_COUNT)
-    c->cfg.tile = tuned_tile(cfg->precision, (long long)cfg->nx * cfg->ny);
-  // Only the MUSCL + HLLD scheme has every tile shape; the others run the
-  // default tile (the selector's per-tile flags follow this shape).
-  if (!(cfg->muscl && cfg->hlld)) c->cfg.tile = mhd::kDefaultTile;
+  c->cfg.tile = tile;
+  c->cfg.fuse = fuse;
+  c->fused = fuse == 2;
   // Planes start on 512-byte boundaries.
   const size_t cells = (size_t)cfg->nx * cfg->ny;
   c->plane = (cells + 63) / 64 * 64;
@@ -467,6 +530,27 @@ int mhd_create(const mhd_cfg* cfg, mhd_ctx** out) {
     c->grid.redo = mhd::launchers_exact(mhd::MODE_FP64)->stage_list;
     c->grid.redo_all = cfg->fallback == 2 ? 1 : 0;
   }
+  // Two-stage steps: the alternate state buffer, and (mixed form with the
+  // fallback) the list of every tile for the recomputation of a step.
+  if (c->fused) {
+    e = hipMalloc(&c->ualt, bytes);
+    if (e == hipSuccess && c->redo_lists) {
+      int tx, ty;
+      mhd::tile_dims(c->cfg.tile, tx, ty);
+      const int ntiles = ((cfg->nx + tx - 1) / tx) * ((cfg->ny + ty - 1) / ty);
+      std::vector<int> all(ntiles);
+      for (int q = 0; q < ntiles; q++) all[q] = q;
+      e = hipMalloc((void**)&c->all_tiles, (size_t)ntiles * sizeof(int));
+      if (e == hipSuccess)
+        e = hipMemcpy(c->all_tiles, all.data(), (size_t)ntiles * sizeof(int),
+                      hipMemcpyHostToDevice);
+    }
+    if (e != hipSuccess) {
+      std::fprintf(stderr, "mhd_create: %s\n", hipGetErrorString(e));
+      mhd_destroy(c);
+      return 9;
+    }
+  }
   *out = c;
   return 0;
 }
@@ -483,6 +567,8 @@ void mhd_destroy(mhd_ctx* c) {
     if (p) (void)hipFree(p);
   if (c->u) (void)hipFree(c->u);
   if (c->u1) (void)hipFree(c->u1);
+  if (c->ualt) (void)hipFree(c->ualt);
+  if (c->all_tiles) (void)hipFree(c->all_tiles);
   if (c->redo_lists) (void)hipFree(c->redo_lists);
   if (c->ctrl) (void)hipFree(c->ctrl);
   if (c->ctrl_host) (void)hipHostFree(c->ctrl_host);
@@ -605,6 +691,7 @@ int mhd_advance(mhd_ctx* c, double t_end, long long max_steps, mhd_status* st) {
     if (int rc = read_ctrl(c)) return rc;
     status_of(c, &s);
   }
+  if (int rc = settle_state(c)) return rc;
   if (st) *st = s;
   return s.code;
 }
@@ -619,6 +706,9 @@ int mhd_get_state(mhd_ctx* c, int which, void* soa9, size_t ld) {
   if (!c) return 1;
   if (!soa9 || ld < (size_t)c->cfg.nx * c->cfg.ny || (which != 0 && which != 1))
     return fail(c, 1, "mhd_get_state: bad arguments");
+  if (which == 1 && c->fused)
+    return fail(c, 1, "mhd_get_state: two-stage steps keep no U1");
+  if (int rc = settle_state(c)) return rc;
   const size_t es = c->L->state_elem;
   const size_t cells = (size_t)c->cfg.nx * c->cfg.ny;
   const char* src = (const char*)(which == 0 ? c->u : c->u1);
@@ -643,6 +733,7 @@ int mhd_get_state(mhd_ctx* c, int which, void* soa9, size_t ld) {
 
 int mhd_device_state(mhd_ctx* c, void** dev_ptr, size_t* ld) {
   if (!c || !dev_ptr || !ld) return 1;
+  if (int rc = settle_state(c)) return rc;
   *dev_ptr = c->u;
   *ld = c->plane;
   return 0;
@@ -661,6 +752,7 @@ struct CkHeader {
 int mhd_checkpoint_save(mhd_ctx* c, const char* path) {
   if (!c || !path) return 1;
   if (!c->have_state) return fail(c, 1, "mhd_checkpoint_save: no state");
+  if (int rc = settle_state(c)) return rc;
   const size_t cells = (size_t)c->cfg.nx * c->cfg.ny;
   const size_t es = c->L->state_elem;
   std::vector<char> buf((size_t)c->L->state_planes * cells * es);
@@ -1258,6 +1350,7 @@ int mhd_tile_features(mhd_ctx* c, float* feats) {
   if (!c->have_state) return fail(c, 1, "mhd_tile_features: no state");
   if (int rc = read_ctrl(c)) return rc;
   if (status_of(c, nullptr) != 0) return fail(c, 1, "mhd_tile_features: the run has halted");
+  if (int rc = settle_state(c)) return rc;
   if (int rc = sel_alloc(c)) return rc;
   auto& S = c->sel;
   HIP_OR_FAIL(c, prepared_snapshot(c));
@@ -1276,6 +1369,7 @@ int mhd_pair_tile_errors(mhd_ctx* c, float* err) {
   if (!c->have_state) return fail(c, 1, "mhd_pair_tile_errors: no state");
   if (int rc = read_ctrl(c)) return rc;
   if (status_of(c, nullptr) != 0) return fail(c, 1, "mhd_pair_tile_errors: the run has halted");
+  if (int rc = settle_state(c)) return rc;
   if (int rc = sel_alloc(c)) return rc;
   auto& S = c->sel;
   const mhd::Launchers* F64 = mhd::launchers_exact(mhd::MODE_FP64);
@@ -1317,6 +1411,11 @@ int mhd_selector_configure(mhd_ctx* c, const float* params, const float* mean, c
   }
   if (c->cfg.precision != MHD_PREC_MIXED) return fail(c, 1, "selector needs the MIXED form");
   if (!c->have_state) return fail(c, 1, "mhd_selector_configure: set a state first");
+  // The selector routes tiles of the single-stage step: two-stage steps
+  // end for this context.
+  if (int rc = settle_state(c)) return rc;
+  c->fused = false;
+  c->cfg.fuse = 1;
   if (int rc = sel_alloc(c)) return rc;
   auto& S = c->sel;
   float istd[mhd::sel::NFEAT], mu[mhd::sel::NFEAT];

@@ -47,6 +47,19 @@ struct Ctrl {
   int redo_count[2];
   unsigned long long redo_tiles;
   int rate_f32;  // CFL rate in float (the reference-mixed form)
+  // Two-stage steps (fusion depth 2): the buffer that holds the current
+  // state (0: the context's state buffer, 1: its alternate); the step's HLLD
+  // counters per stage, added by the finish kernel once the step's outcome
+  // is known; in the mixed form with the FP64 fallback, the flag of a step
+  // in which some tile met an unusable cell, the tile count that the
+  // unfused stage kernels then recompute the step over (else 0), and the
+  // number of steps so recomputed.
+  int cur;
+  int fz_fail;
+  int fz_count;
+  int fz_pad;
+  unsigned long long fz_degenerate[2], fz_nonphysical[2];
+  unsigned long long fz_redo_steps;
   // Per-block partial maxima of the signal speed, spread over kSpeedSlots
   // addresses (block b uses slot b % kSpeedSlots) so that tens of thousands
   // of blocks do not serialise on one atomic; folded into speed_acc by the
@@ -132,6 +145,13 @@ struct ListJob {
   void* ctrl;
   const int* list;
   const int* count;
+  // With `cur` non-null the buffers follow the device word *cur: when it is
+  // non-zero the *_alt buffers are used instead (two-stage steps, whose
+  // state alternates between two buffers).
+  const int* cur = nullptr;
+  const void* src_alt = nullptr;
+  const void* uold_alt = nullptr;
+  void* dst_alt = nullptr;
 };
 
 // Launchers of one arithmetic form.  All launches go to `stream`; none
@@ -150,6 +170,14 @@ struct Launchers {
   // kernels and after the last (ev[0..3]).
   hipError_t (*step)(const GridDesc&, void* u, void* u1, void* ctrl, hipStream_t,
                      hipEvent_t* ev);
+  // One step as one two-stage kernel (MUSCL + HLLD, 16x15 tiles): the state
+  // moves between u and ualt (Ctrl::cur names the current one).  With the
+  // mixed form's FP64 fallback (g.redo_list set), a step in which a tile met
+  // an unusable cell is recomputed by the unfused stage kernels over
+  // all_tiles (a device list 0..ntiles-1), with u1 as their stage-1 output.
+  // Events as for `step` (ev[1]..ev[2] brackets that recomputation).
+  hipError_t (*step_fused)(const GridDesc&, void* u, void* ualt, void* u1, void* ctrl,
+                           const int* all_tiles, hipStream_t, hipEvent_t* ev);
   // One stage (1 or 2) with explicit buffers: src (state L() is evaluated
   // on), uold (stage 2: U^n), dst; and the end-of-step kernel.
   hipError_t (*stage)(const GridDesc&, int stage, const void* src, const void* uold, void* dst,

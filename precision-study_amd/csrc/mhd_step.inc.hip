@@ -426,6 +426,26 @@ struct StageArgs {
   int redo_all;                // queue every tile (check mode)
 };
 
+// Buffers that follow a device word (two-stage steps, whose state
+// alternates between two buffers): with *cur non-zero the stage reads and
+// writes the *_alt buffers instead of StageArgs' own.
+template <class Pol>
+struct CurBuffers {
+  const int* cur;  // null: StageArgs' buffers
+  const typename Pol::S* src_alt;
+  const typename Pol::S* uold_alt;
+  typename Pol::S* dst_alt;
+};
+
+template <class Pol>
+__device__ __forceinline__ void follow_cur(StageArgs<Pol>& a, const CurBuffers<Pol>& b) {
+  if (b.cur && uni(*b.cur) != 0) {
+    a.src = b.src_alt;
+    a.uold = b.uold_alt;
+    a.dst = b.dst_alt;
+  }
+}
+
 // Cell state as loaded from HBM, widened to double where the policy stores
 // more than the flux type.
 template <class Pol> struct CellIO;
@@ -438,6 +458,10 @@ struct CellIO<ExactPol<R>> {
     for (int k = 0; k < 9; k++) v[k] = p[k * ps + g];
   }
   __device__ __forceinline__ R psi() const { return v[PSI]; }
+  __device__ __forceinline__ void store(R* p, size_t ps, size_t g) const {
+#pragma unroll
+    for (int k = 0; k < 9; k++) p[k * ps + g] = v[k];
+  }
 };
 
 template <>
@@ -448,6 +472,10 @@ struct CellIO<RefMixPol> {
     for (int k = 0; k < 9; k++) v[k] = p[k * ps + g];
   }
   __device__ __forceinline__ double psi() const { return v[PSI]; }
+  __device__ __forceinline__ void store(double* p, size_t ps, size_t g) const {
+#pragma unroll
+    for (int k = 0; k < 9; k++) p[k * ps + g] = v[k];
+  }
 };
 
 template <>
@@ -460,6 +488,10 @@ struct CellIO<MixedPol> {
   __device__ __forceinline__ void wide(double out[9]) const {
 #pragma unroll
     for (int k = 0; k < 9; k++) out[k] = v[k];
+  }
+  __device__ __forceinline__ void store(double* p, size_t ps, size_t g) const {
+#pragma unroll
+    for (int k = 0; k < 9; k++) p[k * ps + g] = v[k];
   }
 };
 
@@ -476,6 +508,13 @@ struct CellIO<KahanPol> {
   __device__ __forceinline__ void wide(double out[9]) const {
 #pragma unroll
     for (int k = 0; k < 9; k++) out[k] = (double)hi[k] + (double)lo[k];
+  }
+  __device__ __forceinline__ void store(float* p, size_t ps, size_t g) const {
+#pragma unroll
+    for (int k = 0; k < 9; k++) {
+      p[k * ps + g] = hi[k];
+      p[(9 + k) * ps + g] = lo[k];
+    }
   }
 };
 
@@ -559,14 +598,17 @@ __device__ __forceinline__ A flux_div(R fr, R fl, R gt, R gb, bool pow2, A dx, A
   return -((A)fr - (A)fl) / dx - ((A)gt - (A)gb) / dy;
 }
 
-// Phase C update of one cell (exact policy): rhs, stage update, store.
-// Returns the new state in `out` (as T) for the positivity/speed checks.
+// Phase C update of one cell: the stage updates of mhd2d_glm_cpu.cpp:528-535
+// (stage 1: U1 = U + dt rhs) and 549-557 (stage 2: U/2 + (U1 + dt rhs)/2),
+// from the stage source `src`, U^n (`old`, stage 2) and the rhs flux
+// differences.  `nu` is the new state as stored, `out` the same in T for the
+// positivity and speed checks.
+//
+// Exact policies: everything in R.
 template <int STAGE, class R>
-__device__ __forceinline__ void update_cell(const StageArgs<ExactPol<R>>& a,
-                                            const CellIO<ExactPol<R>>& src, size_t g,
-                                            const R rhs_flux[9], R dt, R damping, R out[9]) {
-  CellIO<ExactPol<R>> old;
-  if (STAGE == 2) old.load(a.uold, a.plane, g);
+__device__ __forceinline__ void advance_cell(const CellIO<ExactPol<R>>& src,
+                                             const CellIO<ExactPol<R>>& old, const R rhs_flux[9],
+                                             R dt, R damping, CellIO<ExactPol<R>>& nu, R out[9]) {
 #pragma unroll
   for (int k = 0; k < 9; k++) {
     R rhs = rhs_flux[k];
@@ -575,21 +617,18 @@ __device__ __forceinline__ void update_cell(const StageArgs<ExactPol<R>>& a,
       out[k] = src.v[k] + dt * rhs;
     else
       out[k] = (R)0.5 * old.v[k] + (R)0.5 * (src.v[k] + dt * rhs);
+    nu.v[k] = out[k];
   }
-#pragma unroll
-  for (int k = 0; k < 9; k++) a.dst[k * a.plane + g] = out[k];
 }
 
 // Reference-mixed policy: rhs in FP32 with the psi damping in FP32
 // (rhsKernel, mhd2d_glm_gpu_mixed.cu:355-375), the stage update in FP64
 // (stage1Kernel/stage2Kernel, :377-410).
 template <int STAGE>
-__device__ __forceinline__ void update_cell(const StageArgs<RefMixPol>& a,
-                                            const CellIO<RefMixPol>& src, size_t g,
-                                            const float rhs_flux[9], double dt, double damping,
-                                            double out[9]) {
-  CellIO<RefMixPol> old;
-  if (STAGE == 2) old.load(a.uold, a.plane, g);
+__device__ __forceinline__ void advance_cell(const CellIO<RefMixPol>& src,
+                                             const CellIO<RefMixPol>& old, const float rhs_flux[9],
+                                             double dt, double damping, CellIO<RefMixPol>& nu,
+                                             double out[9]) {
 #pragma unroll
   for (int k = 0; k < 9; k++) {
     float rhs = rhs_flux[k];
@@ -598,20 +637,16 @@ __device__ __forceinline__ void update_cell(const StageArgs<RefMixPol>& a,
       out[k] = src.v[k] + dt * (double)rhs;
     else
       out[k] = 0.5 * old.v[k] + 0.5 * (src.v[k] + dt * (double)rhs);
+    nu.v[k] = out[k];
   }
-#pragma unroll
-  for (int k = 0; k < 9; k++) a.dst[k * a.plane + g] = out[k];
 }
 
-// Mixed policy: computes the new state only; stage_tile stores it once the
-// tile has passed the positivity check (FP64 fallback).
+// Mixed policy: FP64 rhs accumulation and update.
 template <int STAGE>
-__device__ __forceinline__ void update_cell(const StageArgs<MixedPol>& a,
-                                            const CellIO<MixedPol>& src, size_t g,
-                                            const double rhs_flux[9], double dt,
-                                            double damping, double out[9]) {
-  CellIO<MixedPol> old;
-  if (STAGE == 2) old.load(a.uold, a.plane, g);
+__device__ __forceinline__ void advance_cell(const CellIO<MixedPol>& src,
+                                             const CellIO<MixedPol>& old, const double rhs_flux[9],
+                                             double dt, double damping, CellIO<MixedPol>& nu,
+                                             double out[9]) {
 #pragma unroll
   for (int k = 0; k < 9; k++) {
     double rhs = rhs_flux[k];
@@ -620,16 +655,16 @@ __device__ __forceinline__ void update_cell(const StageArgs<MixedPol>& a,
       out[k] = src.v[k] + dt * rhs;
     else
       out[k] = 0.5 * old.v[k] + 0.5 * (src.v[k] + dt * rhs);
+    nu.v[k] = out[k];
   }
 }
 
+// Compensated policy: FP32 rhs, float-float accumulation of the update.
 template <int STAGE>
-__device__ __forceinline__ void update_cell(const StageArgs<KahanPol>& a,
-                                            const CellIO<KahanPol>& src, size_t g,
-                                            const float rhs_flux[9], double dt,
-                                            double damping, double out[9]) {
-  CellIO<KahanPol> old;
-  if (STAGE == 2) old.load(a.uold, a.plane, g);
+__device__ __forceinline__ void advance_cell(const CellIO<KahanPol>& src,
+                                             const CellIO<KahanPol>& old, const float rhs_flux[9],
+                                             double dt, double damping, CellIO<KahanPol>& nu,
+                                             double out[9]) {
   const float dtf = (float)dt, dampf = (float)damping;
 #pragma unroll
   for (int k = 0; k < 9; k++) {
@@ -639,16 +674,109 @@ __device__ __forceinline__ void update_cell(const StageArgs<KahanPol>& a,
     float h, l;
     ff_add_f(src.hi[k], src.lo[k], d, h, l);
     if (STAGE == 2) ff_add(0.5f * old.hi[k], 0.5f * old.lo[k], 0.5f * h, 0.5f * l, h, l);
-    a.dst[k * a.plane + g] = h;
-    a.dst[(9 + k) * a.plane + g] = l;
+    nu.hi[k] = h;
+    nu.lo[k] = l;
     out[k] = (double)h + (double)l;
   }
+}
+
+// rhs, stage update and (but for the mixed policy, whose stage_tile stores
+// the new state once the tile has passed the positivity check, for the FP64
+// fallback) store of one cell.
+template <int STAGE, class Pol, class A, class T>
+__device__ __forceinline__ void update_cell(const StageArgs<Pol>& a, const CellIO<Pol>& src,
+                                            size_t g, const A rhs_flux[9], T dt, T damping,
+                                            T out[9]) {
+  CellIO<Pol> old, nu;
+  if (STAGE == 2) old.load(a.uold, a.plane, g);
+  advance_cell<STAGE>(src, old, rhs_flux, dt, damping, nu, out);
+  if constexpr (!Pol::kRedo) nu.store(a.dst, a.plane, g);
 }
 
 // rhs type of each policy.
 template <class Pol> struct RhsType { using type = typename Pol::T; };
 template <> struct RhsType<KahanPol> { using type = float; };
 template <> struct RhsType<RefMixPol> { using type = float; };
+
+// One face of a tile pass: the 4-cell stencil at base + {0,1,2,3} * stride in
+// the LDS primitive planes (PN elements apart), reconstructed (minmod MUSCL)
+// or taken first order, its GLM-HLLD flux stored at `slot` of the flux
+// planes (FN apart).  A y face is evaluated in the x frame by reading the
+// (1,2) and (5,6) slots swapped.  Returns the HLLD status.
+template <class Pol, bool MUSCL, bool HLLD, int PN, int FN>
+__device__ __forceinline__ int face_flux(const Phys<typename Pol::R>& c,
+                                         const Phys<typename Pol::RP>& cp, typename Pol::R ch,
+                                         const typename Pol::RP* sP, int base, int stride,
+                                         bool ydir, typename Pol::RS* sF, int slot) {
+  using R = typename Pol::R;
+  using RP = typename Pol::RP;
+  using RS = typename Pol::RS;
+  const int q1 = (ydir ? 2 : 1) * PN, q2 = (ydir ? 1 : 2) * PN;
+  const int q5 = (ydir ? 6 : 5) * PN, q6 = (ydir ? 5 : 6) * PN;
+  auto load = [&](int cell, RP out[9]) {
+    out[0] = sP[cell];
+    out[1] = sP[q1 + cell];
+    out[2] = sP[q2 + cell];
+    out[3] = sP[3 * PN + cell];
+    out[4] = sP[4 * PN + cell];
+    out[5] = sP[q5 + cell];
+    out[6] = sP[q6 + cell];
+    out[7] = sP[7 * PN + cell];
+    out[8] = sP[8 * PN + cell];
+  };
+  // One evaluation of the face: states in RP, widened to the flux type R
+  // (a no-op but for the reference-mixed policy, whose face states are
+  // FP32 and flux FP64), the flux, its store to LDS.  With sharing (FP64:
+  // divisions by one denominator share its reciprocal) a lane may come
+  // back with `miss` (an operand range where the sharing does not hold);
+  // the face is then evaluated again from LDS with plain division.  Only
+  // the face index and the status live across, so the second evaluation
+  // adds no register pressure.
+  auto evaluate = [&](auto share, bool& miss) -> int {
+    R UL[9], UR[9];
+    if (MUSCL) {
+      RP Pa[9], Pb[9], Pc[9], Pf[9];
+      load(base, Pa);
+      load(base + stride, Pb);
+      load(base + 2 * stride, Pc);
+      reconstruct_face<RP, true>(cp, Pa, Pb, Pc, Pf);
+      face_cons<Pol>(cp, Pf, UL);
+      load(base + 3 * stride, Pa);
+      reconstruct_face<RP, false>(cp, Pb, Pc, Pa, Pf);
+      face_cons<Pol>(cp, Pf, UR);
+    } else {
+      RP Ua[9], Ub[9];
+      load(base + stride, Ua);
+      load(base + 2 * stride, Ub);
+#pragma unroll
+      for (int k = 0; k < 9; k++) {
+        UL[k] = (R)Ua[k];
+        UR[k] = (R)Ub[k];
+      }
+    }
+    R F[9];
+    const int st = glm_flux_x<R, HLLD, decltype(share)::value>(c, UL, UR, ch, F, miss);
+    const int p1 = (ydir ? 2 : 1) * FN, p2 = (ydir ? 1 : 2) * FN;
+    const int p5 = (ydir ? 6 : 5) * FN, p6 = (ydir ? 5 : 6) * FN;
+    sF[slot] = (RS)F[0];
+    sF[p1 + slot] = (RS)F[1];
+    sF[p2 + slot] = (RS)F[2];
+    sF[3 * FN + slot] = (RS)F[3];
+    sF[4 * FN + slot] = (RS)F[4];
+    sF[p5 + slot] = (RS)F[5];
+    sF[p6 + slot] = (RS)F[6];
+    sF[7 * FN + slot] = (RS)F[7];
+    sF[8 * FN + slot] = (RS)F[8];
+    return st;
+  };
+  bool miss = false;
+  int st = evaluate(std::integral_constant<bool, sizeof(R) == 8>{}, miss);
+  if (miss) {
+    bool unused = false;
+    st = evaluate(std::integral_constant<bool, false>{}, unused);
+  }
+  return st;
+}
 
 // LDS footprint of one tile: primitives (tile + halo) in RP, face fluxes in
 // RS.
@@ -721,8 +849,7 @@ __device__ __forceinline__ void stage_tile(const StageArgs<Pol>& a, int tile,
 
   // Phase B: faces.  Faces [0, NXF) are x faces (row r, face column fc
   // between local cells fc-1 and fc); [NXF, NF) are y faces (column cc,
-  // face row fr between local rows fr-1 and fr).  A y face is evaluated in
-  // the x frame by reading the (1,2) and (5,6) slots swapped.
+  // face row fr between local rows fr-1 and fr).
   unsigned degen = 0, nonphys = 0;
   for (int f = tid; f < NF; f += NTHR) {
     const bool ydir = f >= NXF;
@@ -739,70 +866,7 @@ __device__ __forceinline__ void stage_tile(const StageArgs<Pol>& a, int tile,
       stride = HX;
       owned = (i0 + cc < nx) && (fr >= 1 ? (j0 + fr - 1 < ny) : (j0 == 0));
     }
-    const int q1 = (ydir ? 2 : 1) * HN, q2 = (ydir ? 1 : 2) * HN;
-    const int q5 = (ydir ? 6 : 5) * HN, q6 = (ydir ? 5 : 6) * HN;
-    auto load = [&](int cell, RP out[9]) {
-      out[0] = sP[cell];
-      out[1] = sP[q1 + cell];
-      out[2] = sP[q2 + cell];
-      out[3] = sP[3 * HN + cell];
-      out[4] = sP[4 * HN + cell];
-      out[5] = sP[q5 + cell];
-      out[6] = sP[q6 + cell];
-      out[7] = sP[7 * HN + cell];
-      out[8] = sP[8 * HN + cell];
-    };
-    // One evaluation of the face: states in RP, widened to the flux type R
-    // (a no-op but for the reference-mixed policy, whose face states are
-    // FP32 and flux FP64), the flux, its store to LDS.  With sharing (FP64:
-    // divisions by one denominator share its reciprocal) a lane may come
-    // back with `miss` (an operand range where the sharing does not hold);
-    // the face is then evaluated again from LDS with plain division.  Only
-    // the face index and the status live across, so the second evaluation
-    // adds no register pressure.
-    auto evaluate = [&](auto share, bool& miss) -> int {
-      R UL[9], UR[9];
-      if (MUSCL) {
-        RP Pa[9], Pb[9], Pc[9], Pf[9];
-        load(base, Pa);
-        load(base + stride, Pb);
-        load(base + 2 * stride, Pc);
-        reconstruct_face<RP, true>(cp, Pa, Pb, Pc, Pf);
-        face_cons<Pol>(cp, Pf, UL);
-        load(base + 3 * stride, Pa);
-        reconstruct_face<RP, false>(cp, Pb, Pc, Pa, Pf);
-        face_cons<Pol>(cp, Pf, UR);
-      } else {
-        RP Ua[9], Ub[9];
-        load(base + stride, Ua);
-        load(base + 2 * stride, Ub);
-#pragma unroll
-        for (int k = 0; k < 9; k++) {
-          UL[k] = (R)Ua[k];
-          UR[k] = (R)Ub[k];
-        }
-      }
-      R F[9];
-      const int st = glm_flux_x<R, HLLD, decltype(share)::value>(c, UL, UR, ch, F, miss);
-      const int p1 = (ydir ? 2 : 1) * NF, p2 = (ydir ? 1 : 2) * NF;
-      const int p5 = (ydir ? 6 : 5) * NF, p6 = (ydir ? 5 : 6) * NF;
-      sF[f] = (RS)F[0];
-      sF[p1 + f] = (RS)F[1];
-      sF[p2 + f] = (RS)F[2];
-      sF[3 * NF + f] = (RS)F[3];
-      sF[4 * NF + f] = (RS)F[4];
-      sF[p5 + f] = (RS)F[5];
-      sF[p6 + f] = (RS)F[6];
-      sF[7 * NF + f] = (RS)F[7];
-      sF[8 * NF + f] = (RS)F[8];
-      return st;
-    };
-    bool miss = false;
-    int st = evaluate(std::integral_constant<bool, sizeof(R) == 8>{}, miss);
-    if (miss) {
-      bool unused = false;
-      st = evaluate(std::integral_constant<bool, false>{}, unused);
-    }
+    const int st = face_flux<Pol, MUSCL, HLLD, HN, NF>(c, cp, ch, sP, base, stride, ydir, sF, f);
     if (HLLD && owned) {
       degen += (st == HLLD_DEGENERATE);
       nonphys += (st == HLLD_NONPHYSICAL);
@@ -959,6 +1023,7 @@ struct ListSeg {
   StageArgs<Pol> a;
   const int* list;   // null: empty segment
   const int* count;
+  CurBuffers<Pol> bufs;
 };
 
 template <class Pol, int STAGE>
@@ -988,7 +1053,9 @@ __attribute__((amdgpu_waves_per_eu(StageWaves<Pol>::value))) void k_stage_list(L
   for (int e = (int)blockIdx.x; e < total; e += (int)gridDim.x) {
     int k = 0, x = e;
     while (x >= n[k]) x -= n[k++];
-    stage_tile<Pol, MUSCL, HLLD, STAGE, TX, TY>(g.seg[k].a, uni(g.seg[k].list[x]), sh);
+    StageArgs<Pol> a = g.seg[k].a;
+    follow_cur(a, g.seg[k].bufs);
+    stage_tile<Pol, MUSCL, HLLD, STAGE, TX, TY>(a, uni(g.seg[k].list[x]), sh);
     __syncthreads();
   }
 }
@@ -1406,9 +1473,11 @@ static hipError_t launch_stage_t(const GridDesc& g, int stage, const void* src, 
 // listed tiles at once; an empty list costs one short launch.
 constexpr int kRedoBlocks = 512;
 
+// With `queue` (mixed form) a listed tile with an unusable cell is queued
+// there for the FP64 kernel instead of halting the run.
 template <class Pol, bool MUSCL, bool HLLD, int TX, int TY>
 static hipError_t launch_list_t(const GridDesc& g, int stage, const ListJob* jobs, int njobs,
-                                hipStream_t s) {
+                                hipStream_t s, int* queue = nullptr) {
   using S = typename Pol::S;
   auto seg = [&](const ListJob& j) {
     ListSeg<Pol> q;
@@ -1417,9 +1486,17 @@ static hipError_t launch_list_t(const GridDesc& g, int stage, const ListJob* job
     q.a.tiles = nullptr;
     q.a.tile_count = nullptr;
     q.a.redo_all = 0;
+    if (Pol::kRedo && queue && j.ctrl) {
+      q.a.redo_list = queue;
+      q.a.redo_counter = redo_counter_of<typename Pol::T>(j.ctrl, stage);
+    }
     q.a.src = (const S*)j.src;
     q.a.uold = (const S*)j.uold;
     q.a.dst = (S*)j.dst;
+    q.bufs.cur = j.cur;
+    q.bufs.src_alt = (const S*)j.src_alt;
+    q.bufs.uold_alt = (const S*)j.uold_alt;
+    q.bufs.dst_alt = (S*)j.dst_alt;
     q.list = j.list;
     q.count = j.count;
     return q;
@@ -1673,6 +1750,8 @@ static hipError_t ens_set_target(void* ctrls, int n, double t_end, unsigned* que
   return hipGetLastError();
 }
 
+#include "mhd_fused.inc.hip"
+
 template <class Pol>
 static Launchers make_launchers(int mode) {
   Launchers L;
@@ -1685,6 +1764,7 @@ static Launchers make_launchers(int mode) {
   L.commit = &launch_commit<typename Pol::T>;
   L.set_target = &launch_target<typename Pol::T>;
   L.step = &launch_step<Pol>;
+  L.step_fused = &launch_step_fused<Pol>;
   L.stage = &launch_stage<Pol>;
   L.finish = &launch_finish<typename Pol::T>;
   if constexpr (std::is_same<Pol, ExactPol<double>>::value) {

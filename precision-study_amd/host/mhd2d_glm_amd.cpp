@@ -15,7 +15,8 @@
 // MHD_PRECISION (fp64, fp32, mixed, mixed_kahan, fp64_fma, ref_mixed).  MHD_HLLD_TOLERANCE sets the HLLD tolerance
 // (default 1e-4, the value of every reference timing and checksum run);
 // MHD_TILE the stage tile shape, MHD_LOWBETA_RATIO the mixed form's FP64
-// recovery switch and MHD_FALLBACK its FP64 tile fallback (0 on, 1 off).
+// recovery switch, MHD_FALLBACK its FP64 tile fallback (0 on, 1 off) and
+// MHD_FUSE the SSPRK2 stages per kernel launch (1 or 2).
 //
 // This program only talks to the solver through include/mhd_glm.h.
 #include "mhd_glm.h"
@@ -157,6 +158,7 @@ int run(const Args& a, int precision, const char* prec_name, double tol) {
   if (const char* s = std::getenv("MHD_LOWBETA_RATIO")) cfg.lowbeta_ratio = std::atof(s);
   if (const char* s = std::getenv("MHD_TILE")) cfg.tile = std::atoi(s);
   if (const char* s = std::getenv("MHD_FALLBACK")) cfg.fallback = std::atoi(s);
+  if (const char* s = std::getenv("MHD_FUSE")) cfg.fuse = std::atoi(s);
   mhd_ctx* ctx = nullptr;
   int rc = mhd_create(&cfg, &ctx);
   if (rc != 0) {

@@ -49,7 +49,7 @@ EXPORTED = (
     "mhd_ensemble_create", "mhd_ensemble_destroy", "mhd_ensemble_last_error",
     "mhd_ensemble_set_state", "mhd_ensemble_init_problem", "mhd_ensemble_advance",
     "mhd_ensemble_get_status", "mhd_ensemble_get_state", "mhd_ensemble_enqueue_steps",
-    "mhd_ensemble_time_kernels", "mhd_autotuned_tile",
+    "mhd_ensemble_time_kernels", "mhd_autotuned_tile", "mhd_autotuned_fuse",
     "mhd_selector_configure", "mhd_selector_stats", "mhd_selector_tiles", "mhd_tile_features",
     "mhd_pair_tile_errors", "mhd_mlp_param_count", "mhd_mlp_forward", "mhd_mlp_gradients",
     "mhd_mlp_train",
@@ -72,6 +72,7 @@ class Cfg(ctypes.Structure):
         ("precision", ctypes.c_int), ("hlld_tol", ctypes.c_double),
         ("tile", ctypes.c_int), ("batch_steps", ctypes.c_int),
         ("lowbeta_ratio", ctypes.c_double), ("fallback", ctypes.c_int),
+        ("fuse", ctypes.c_int),
     ]
 
 
@@ -82,7 +83,7 @@ class Status(ctypes.Structure):
         ("x_evaluations", ctypes.c_ulonglong), ("y_evaluations", ctypes.c_ulonglong),
         ("degenerate", ctypes.c_ulonglong), ("nonphysical", ctypes.c_ulonglong),
         ("bad_i", ctypes.c_int), ("bad_j", ctypes.c_int), ("fail_step", ctypes.c_int),
-        ("fallback_tiles", ctypes.c_ulonglong),
+        ("fallback_tiles", ctypes.c_ulonglong), ("fused_redo_steps", ctypes.c_ulonglong),
     ]
 
     def as_dict(self) -> dict:
@@ -160,6 +161,8 @@ def load(path: str = LIB_PATH) -> ctypes.CDLL:
     lib.mhd_mlp_train.restype = I
     lib.mhd_autotuned_tile.argtypes = [I, I, I]
     lib.mhd_autotuned_tile.restype = I
+    lib.mhd_autotuned_fuse.argtypes = [I, I, I]
+    lib.mhd_autotuned_fuse.restype = I
     lib.mhd_problem_state_ex.argtypes = [ctypes.c_char_p, P, I, I, I, I, P, S]
     lib.mhd_problem_state_ex.restype = I
     lib.mhd_ensemble_create.argtypes = [ctypes.POINTER(Cfg), I, P, P, ctypes.POINTER(P)]
@@ -228,6 +231,7 @@ class SolverConfig:
     batch_steps: int = 0
     lowbeta_ratio: float = 0.0
     fallback: int = 0  # MIXED: 0 FP64 fallback on, 1 off, 2 every tile (check mode)
+    fuse: int = 0  # stages per launch: 1, 2 (one launch per step); 0 autotuned (tile -1) else 1
 
 
 class Solver:
@@ -389,7 +393,7 @@ def mlp_train(params, x, y, w, epochs=20, batch=1024, lr=1e-3, seed=0):
 def _cfg(cfg: "SolverConfig", nx: int = 0, ny: int = 0) -> Cfg:
     return Cfg(nx or cfg.nx, ny or cfg.ny, cfg.cfl, cfg.alpha, int(cfg.muscl), int(cfg.hlld),
                cfg.precision, cfg.hlld_tol, cfg.tile, cfg.batch_steps, cfg.lowbeta_ratio,
-               cfg.fallback)
+               cfg.fallback, cfg.fuse)
 
 
 class Ensemble:

@@ -1,12 +1,14 @@
-"""Autotuner of the stage kernels' tile shape.
+"""Autotuner of the step kernels' tile shape and fusion depth.
 
     python -m mhdglm.autotune [--forms fp64,mixed] [--sizes 256,1024,4096] [--out FILE]
 
-For every arithmetic form and grid size it times each tile shape on the
-current device (HIP events around both stage kernels, after a warm-up) and
-writes the table the library reads when a caller leaves the tile at -1:
-lines "form max_cells tile", sizes ascending, so the best tile for the
-smallest tuned size at or above a grid's cell count is used.
+For every arithmetic form and grid size it times, on the current device,
+each tile shape with one launch per SSPRK2 stage and the 16x15 tile with
+both stages in one launch (HIP events around the step's kernels, after a
+warm-up), and writes the table the library reads when a caller leaves the
+tile at -1 (and the fusion depth at 0): lines "form max_cells tile fuse",
+sizes ascending, so the best choice for the smallest tuned size at or above
+a grid's cell count is used.
 """
 from __future__ import annotations
 
@@ -18,46 +20,60 @@ import sys
 from . import PKG_ROOT, PRECISIONS, Solver, SolverConfig
 
 TILES = {0: "16x16", 1: "32x8", 2: "16x8", 3: "16x15", 4: "32x7", 5: "16x7"}
+# (tile, stages per launch) candidates: every tile one stage per launch, and
+# the two-stage kernel (16x15 only).
+CHOICES = [(t, 1) for t in TILES] + [(3, 2)]
 DEFAULT_OUT = os.path.join(PKG_ROOT, "autotune.txt")
 
 
-def time_tile(form: str, n: int, tile: int, steps: int = 10) -> float:
-    """Average ms of one step's two stage kernels (OT n x n)."""
-    s = Solver(SolverConfig(nx=n, ny=n, precision=PRECISIONS[form], tile=tile))
+def label(choice) -> str:
+    tile, fuse = choice
+    return TILES[tile] + ("" if fuse == 1 else " fused")
+
+
+def time_choice(form: str, n: int, tile: int, fuse: int = 1, steps: int = 10) -> float:
+    """Average ms of one step's kernels (OT n x n)."""
+    s = Solver(SolverConfig(nx=n, ny=n, precision=PRECISIONS[form], tile=tile, fuse=fuse))
     s.init_problem("ot")
     s.advance(0.5, 3)
     ms = s.time_kernels(0.5, steps)
     s.close()
-    return ms[0] + ms[1]
+    return ms[0] + ms[1] + ms[2]
 
 
 def tune(forms, sizes, log=print):
     rows, report = [], []
     for form in forms:
         for n in sorted(sizes):
-            times = {t: time_tile(form, n, t) for t in TILES}
+            times = {ch: time_choice(form, n, *ch) for ch in CHOICES}
             best = min(times, key=times.get)
-            rows.append((PRECISIONS[form], n * n, best))
+            rows.append((PRECISIONS[form], n * n, best[0], best[1]))
             report.append((form, n, best, times))
             log(f"autotune {form} {n}^2: " +
-                ", ".join(f"{TILES[t]} {times[t]:.4f} ms" for t in TILES) + f" -> {TILES[best]}")
+                ", ".join(f"{label(c)} {times[c]:.4f} ms" for c in CHOICES) +
+                f" -> {label(best)}")
     return rows, report
 
 
 def write_table(rows, path: str, note: str = ""):
+    """rows: (form, cells, tile) or (form, cells, tile, fuse)."""
     with open(path, "w") as f:
-        f.write("# stage-kernel tile per arithmetic form and grid size, written by mhdglm.autotune\n")
-        f.write(f"# {note}\n# form max_cells tile   (form: 0 fp64, 1 fp32, 2 mixed, 3 mixed_kahan,"
-                " 4 fp64_fma; tile: 0 16x16, 1 32x8, 2 16x8, 3 16x15, 4 32x7, 5 16x7)\n")
+        f.write("# step-kernel tile and stages per launch per arithmetic form and grid size,"
+                " written by mhdglm.autotune\n")
+        f.write(f"# {note}\n# form max_cells tile fuse   (form: 0 fp64, 1 fp32, 2 mixed,"
+                " 3 mixed_kahan, 4 fp64_fma, 5 ref_mixed; tile: 0 16x16, 1 32x8, 2 16x8,"
+                " 3 16x15, 4 32x7, 5 16x7; fuse: stages per launch)\n")
         by_form = {}
-        for form, cells, tile in rows:
-            by_form.setdefault(form, []).append((cells, tile))
+        for row in rows:
+            form, cells, tile = row[:3]
+            fuse = row[3] if len(row) > 3 else 1
+            by_form.setdefault(form, []).append((cells, tile, fuse))
         for form in sorted(by_form):
             entries = sorted(by_form[form])
-            for k, (cells, tile) in enumerate(entries):
+            for k, (cells, tile, fuse) in enumerate(entries):
                 # The largest tuned size also covers anything bigger.
                 limit = cells if k + 1 < len(entries) else 2**62
-                f.write(f"{form} {limit} {tile}\n")
+                f.write(f"{form} {limit} {tile} {fuse}\n")
 
 
 def main(argv=None) -> int:

@@ -108,12 +108,18 @@ def test_autotune_table_lookup(tmp_path):
 
     table = tmp_path / "autotune.txt"
     autotune.write_table([(0, 256 * 256, 1), (0, 1024 * 1024, 0), (0, 4096 * 4096, 4),
-                          (2, 512 * 512, 5)], str(table), "test")
+                          (2, 512 * 512, 5), (2, 2048 * 2048, 3, 2)], str(table), "test")
+    # A table of the earlier three-column format reads with one stage per launch.
+    with open(table, "a") as f:
+        f.write("1 4096 2\n")
+    cases = ((0, 128), (0, 256), (0, 512), (0, 1024), (0, 8192), (2, 512), (2, 4096),
+             (1, 64), (1, 128))
     code = ("import sys; sys.path.insert(0, sys.argv[1]); import mhdglm; l = mhdglm.load(); "
-            "print([l.mhd_autotuned_tile(p, n, n) for p, n in "
-            "((0, 128), (0, 256), (0, 512), (0, 1024), (0, 8192), (2, 512), (2, 4096), (1, 64))])")
+            f"cases = {cases!r}; "
+            "print([l.mhd_autotuned_tile(p, n, n) for p, n in cases], "
+            "[l.mhd_autotuned_fuse(p, n, n) for p, n in cases])")
     env = dict(os.environ, MHD_AUTOTUNE_FILE=str(table))
     r = subprocess.run([sys.executable, "-c", code, os.path.join(ROOT, "precision-study_amd")],
                        env=env, capture_output=True, text=True)
     assert r.returncode == 0, r.stderr
-    assert r.stdout.strip() == "[1, 1, 0, 0, 4, 5, 5, 3]"
+    assert r.stdout.strip() == "[1, 1, 0, 0, 4, 5, 3, 2, 3] [1, 1, 1, 1, 1, 1, 2, 1, 1]"
