@@ -61,8 +61,45 @@ struct FusedGeom {
   static constexpr int NXF = (TX + 1) * TY, NF = NXF + TX * (TY + 1);
   __device__ static constexpr int xslot(int xf, int y) { return (y - 2) * XW + (xf - 2); }
   __device__ static constexpr int yslot(int x, int yf) { return NXS + (yf - 2) * YW + (x - 2); }
-  // A stage-1 arm coordinate: k in [0, 4) -> 2, 3, n+4, n+5.
+  // A stage-1 arm coordinate: k in [0, 4) -> 2, 3, n+4, n+5; the U^n
+  // outer arm beyond it: 0, 1, n+6, n+7.
   __device__ static constexpr int arm(int k, int n) { return k < 2 ? 2 + k : n + 2 + k; }
+  __device__ static constexpr int outer(int k, int n) { return k < 2 ? k : n + 4 + k; }
+  // U^n cells outside the stage-1 cells: the outer arms and the 2x2
+  // corners of the tile + 2 box.
+  static constexpr int NOUT = 4 * TY + 4 * TX + 16;
+  // Box cell of stage-1 cell e (tile cells first, row-major; then the arms
+  // on the tile's rows, then those on its columns).
+  __device__ static void c1_cell(int e, int& x, int& y) {
+    if (e < NT) {
+      const int r = e / TX;
+      x = 4 + (e - r * TX);
+      y = 4 + r;
+    } else if (e < NT + 4 * TY) {
+      const int m = e - NT;
+      x = arm(m & 3, TX);
+      y = 4 + (m >> 2);
+    } else {
+      const int m = e - NT - 4 * TY, k = m / TX;
+      x = 4 + (m - k * TX);
+      y = arm(k, TY);
+    }
+  }
+  // Box cell of outer cell o.
+  __device__ static void outer_cell(int o, int& x, int& y) {
+    if (o < 4 * TY) {
+      x = outer(o & 3, TX);
+      y = 4 + (o >> 2);
+    } else if (o < 4 * TY + 4 * TX) {
+      const int m = o - 4 * TY, k = m / TX;
+      x = 4 + (m - k * TX);
+      y = outer(k, TY);
+    } else {
+      const int m = o - 4 * TY - 4 * TX;
+      x = arm(m & 3, TX);
+      y = arm(m >> 2, TY);
+    }
+  }
 };
 
 template <class RP, class RS, int TX, int TY>
@@ -82,13 +119,14 @@ template <class Pol, int TX, int TY>
 using FusedSharedOf = FusedShared<typename Pol::RP, typename Pol::RS, TX, TY>;
 
 // Waves per SIMD the LDS footprint allows (160 KB per CU, 4 SIMDs), at
-// most the single-stage kernels' register budget (an FP64 flux needs about
-// 250 registers: 2 waves).
+// most what the registers allow: an FP64 flux (about 250 registers) plus
+// the U^n and U1 states this kernel holds across its face phases need more
+// than the 256 of two waves, so FP64-flux forms run one.
 template <class Pol, int TX, int TY>
 struct FusedWaves {
   static constexpr int blocks = (int)(163840 / sizeof(FusedSharedOf<Pol, TX, TY>));
   static constexpr int w = blocks * FusedGeom<TX, TY>::NW / 4;
-  static constexpr int cap = StageWaves<Pol>::value;
+  static constexpr int cap = sizeof(typename Pol::R) == 8 ? 1 : StageWaves<Pol>::value;
   static constexpr int value = w < 1 ? 1 : (w > cap ? cap : w);
 };
 
@@ -129,28 +167,37 @@ __device__ __forceinline__ void step_tile_fused(const StageArgs<Pol>& a, int til
     return (size_t)wrap(j0 + y - 4, ny) * nx + wrap(i0 + x - 4, nx);
   };
 
-  // A1: U^n -> primitives.  A thread's cells are all loaded before any is
+  // A1: U^n -> primitives.  Thread tid loads stage-1 cells tid and
+  // tid + NTHR, whose U^n it keeps in registers for C1 (and, for its tile
+  // cell, C2), and outer cell tid; all loads are issued before any cell is
   // recovered (one memory round trip).
-  constexpr int KA = (BN + NTHR - 1) / NTHR;
-  static_assert(KA <= 3, "phase A1 holds at most three cells per thread");
+  static_assert(G::NC1 <= 2 * NTHR && G::NOUT <= NTHR, "phase A1 takes three cells per thread");
+  CellIO<Pol> un[2];
   {
-    CellIO<Pol> u[KA];
-    bool have[KA];
+    bool have[3];
+    int hh[3];
+    CellIO<Pol> uo;
 #pragma unroll
-    for (int q = 0; q < KA; q++) {
-      const int h = tid + q * NTHR, y = h / BX, x = h - y * BX;
-      have[q] = h < BN && ((y >= 4 && y < TY + 4) || (x >= 4 && x < TX + 4) ||
-                           (x >= 2 && x < TX + 6 && y >= 2 && y < TY + 6));
-      if (have[q]) u[q].load(a.src, ps, gidx(x, y));
+    for (int q = 0; q < 3; q++) {
+      int x = 0, y = 0;
+      if (q < 2) {
+        const int e = tid + q * NTHR;
+        have[q] = e < G::NC1;
+        if (have[q]) G::c1_cell(e, x, y);
+      } else {
+        have[q] = tid < G::NOUT;
+        if (have[q]) G::outer_cell(tid, x, y);
+      }
+      hh[q] = y * BX + x;
+      if (have[q]) (q < 2 ? un[q] : uo).load(a.src, ps, gidx(x, y));
     }
 #pragma unroll
-    for (int q = 0; q < KA; q++) {
+    for (int q = 0; q < 3; q++) {
       if (!have[q]) continue;
-      const int h = tid + q * NTHR;
       RP P[9];
-      recover<MUSCL>(cp, u[q], a.lowbeta_ratio, P);
+      recover<MUSCL>(cp, q < 2 ? un[q] : uo, a.lowbeta_ratio, P);
 #pragma unroll
-      for (int k = 0; k < 9; k++) sP1[k * BN + h] = P[k];
+      for (int k = 0; k < 9; k++) sP1[k * BN + hh[q]] = P[k];
     }
   }
   __syncthreads();
@@ -205,9 +252,8 @@ __device__ __forceinline__ void step_tile_fused(const StageArgs<Pol>& a, int til
   }
   __syncthreads();
 
-  // C1: U1 of the stage-1 cells, tile cells first (thread tid < NT holds
-  // tile cell tid, the one it updates again in C2).
-  static_assert(G::NC1 <= 2 * NTHR, "phase C1 takes at most two cells per thread");
+  // C1: U1 of the stage-1 cells the thread loaded in A1 (thread tid < NT
+  // holds tile cell tid, the one it updates again in C2).
   CellIO<Pol> u1own;
   bool fail = false;
 #pragma unroll
@@ -215,19 +261,7 @@ __device__ __forceinline__ void step_tile_fused(const StageArgs<Pol>& a, int til
     const int e = tid + q * NTHR;
     if (e >= G::NC1) continue;
     int x, y;
-    if (e < NT) {
-      const int r = e / TX;
-      x = 4 + (e - r * TX);
-      y = 4 + r;
-    } else if (e < NT + 4 * TY) {
-      const int m = e - NT;
-      x = G::arm(m & 3, TX);
-      y = 4 + (m >> 2);
-    } else {
-      const int m = e - NT - 4 * TY, k = m / TX;
-      x = 4 + (m - k * TX);
-      y = G::arm(k, TY);
-    }
+    G::c1_cell(e, x, y);
     const int xl = G::xslot(x, y), yb = G::yslot(x, y);
     A rhs[9];
 #pragma unroll
@@ -235,8 +269,8 @@ __device__ __forceinline__ void step_tile_fused(const StageArgs<Pol>& a, int til
       rhs[k] = flux_div<A, RS>(sF1[k * NS1 + xl + 1], sF1[k * NS1 + xl],
                                sF1[k * NS1 + yb + G::YW], sF1[k * NS1 + yb], pow2, adx, ady,
                                aidx, aidy);
-    CellIO<Pol> src, nu;
-    src.load(a.src, ps, gidx(x, y));
+    const CellIO<Pol>& src = un[q];
+    CellIO<Pol> nu;
     T out[9];
     advance_cell<1>(src, src, rhs, dt, damping, nu, out);
     const bool own = e < NT;
@@ -307,10 +341,9 @@ __device__ __forceinline__ void step_tile_fused(const StageArgs<Pol>& a, int til
     for (int k = 0; k < 9; k++)
       rhs[k] = flux_div<A, RS>(sF2[k * NF + xl + 1], sF2[k * NF + xl], sF2[k * NF + yb + TX],
                                sF2[k * NF + yb], pow2, adx, ady, aidx, aidy);
-    CellIO<Pol> old, nu;
-    old.load(a.uold, ps, g);
+    CellIO<Pol> nu;
     T out[9];
-    advance_cell<2>(u1own, old, rhs, dt, damping, nu, out);
+    advance_cell<2>(u1own, un[0], rhs, dt, damping, nu, out);
     nu.store(a.dst, ps, g);
     if constexpr (sizeof(R) == sizeof(T)) {
       const Phys<T> ct = make_phys<T>(a.tol);
