@@ -93,11 +93,13 @@ def cpu_baseline(n: int, t_end: float):
     }
 
 
-def make_solver(mhdglm, form, n, tile=-1, problem="ot"):
+def make_solver(mhdglm, form, n, tile=-1, problem="ot", fuse=0):
     """n^2 solver of `problem` in an arithmetic form; "selector" is the mixed
-    form with the committed selector model deciding FP64 tiles."""
+    form with the committed selector model deciding FP64 tiles (which runs
+    one stage per launch whatever `fuse` says)."""
     prec = mhdglm.PREC_MIXED if form == "selector" else mhdglm.PRECISIONS[form]
-    solver = mhdglm.Solver(mhdglm.SolverConfig(nx=n, ny=n, precision=prec, tile=tile))
+    solver = mhdglm.Solver(mhdglm.SolverConfig(nx=n, ny=n, precision=prec, tile=tile,
+                                               fuse=fuse))
     solver.init_problem(problem)
     if form == "selector":
         from mhdglm import selector
@@ -105,10 +107,10 @@ def make_solver(mhdglm, form, n, tile=-1, problem="ot"):
     return solver
 
 
-def time_form(mhdglm, torch, form, n, steps, warmup, tile=-1):
+def time_form(mhdglm, torch, form, n, steps, warmup, tile=-1, fuse=0):
     """Times `steps` SSPRK2 steps of OT n^2 in one arithmetic form."""
     log(f"bench: OT {n}x{n} {form}, warmup {warmup}, steps {steps}")
-    solver = make_solver(mhdglm, form, n, tile)
+    solver = make_solver(mhdglm, form, n, tile, fuse=fuse)
     t_end = 0.5
     solver.enqueue_steps(t_end, warmup)
     torch.cuda.synchronize()
@@ -271,6 +273,8 @@ def main():
                     choices=("fp64", "fp32", "mixed", "mixed_kahan", "fp64_fma", "ref_mixed"),
                     default="fp64")
     ap.add_argument("--tile", type=int, default=-1)
+    ap.add_argument("--fuse", type=int, default=0,
+                    help="SSPRK2 stages per launch: 1, 2 (16x15 tiles), 0 = the autotuned choice")
     ap.add_argument("--cpu-n", type=int, default=512)
     ap.add_argument("--cpu-t-end", type=float, default=0.006)
     ap.add_argument("--large-n", type=int, default=8192,
@@ -320,15 +324,26 @@ def main():
         print("bench.py: no GPU visible", file=sys.stderr)
         sys.exit(2)
     n = args.n
-    res = time_form(mhdglm, torch, args.precision, n, args.steps, args.warmup, args.tile)
+    res = time_form(mhdglm, torch, args.precision, n, args.steps, args.warmup, args.tile,
+                    args.fuse)
     value, wall, ms = res["value"], res["wall"], res["kernel_ms"]
     esize = 4 if args.precision == "fp32" else 8
-    stage = 1 if ms[0] >= ms[1] else 2
-    alg_bytes = STAGE_BYTES_PER_CELL[stage] * esize * n * n
+    fused = args.fuse == 2
+    if fused:
+        # One two-stage kernel per step: U^n in, U^{n+1} out.
+        stage, per_cell, kname = 1, 18, "k_step_fused"
+        tfile = os.path.join(ROOT, "profiles", f"traffic_step_fused_ot{n}_{args.precision}.json")
+    else:
+        stage = 1 if ms[0] >= ms[1] else 2
+        per_cell, kname = STAGE_BYTES_PER_CELL[stage], f"k_stage<{stage}>"
+        tfile = os.path.join(ROOT, "profiles", f"traffic_stage_ot{n}_{args.precision}.json")
+    stages_per_launch = args.fuse or (
+        mhdglm.load().mhd_autotuned_fuse(mhdglm.PRECISIONS[args.precision], n, n)
+        if args.tile < 0 else 1)
+    alg_bytes = per_cell * esize * n * n
     achieved = alg_bytes / (ms[stage - 1] * 1e-3) / 1e9
     traffic = None
     valu = None
-    tfile = os.path.join(ROOT, "profiles", f"traffic_stage_ot{n}_{args.precision}.json")
     if os.path.exists(tfile):
         with open(tfile) as f:
             prof = json.load(f)
@@ -344,13 +359,13 @@ def main():
     roofline = {
         "bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
         "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic, "valu": valu,
-        "kernel": f"k_stage<{stage}> ({STAGE_BYTES_PER_CELL[stage] * esize} B/cell compulsory)",
+        "kernel": f"{kname} ({per_cell * esize} B/cell compulsory)",
         "kernel_ms": {"stage1": round(ms[0], 4), "stage2": round(ms[1], 4),
                       "finish": round(ms[2], 4)},
     }
     forms = {args.precision: round(value, 1)}
     for form in [f for f in args.forms.split(",") if f and f != args.precision]:
-        r = time_form(mhdglm, torch, form, n, args.steps, args.warmup, args.tile)
+        r = time_form(mhdglm, torch, form, n, args.steps, args.warmup, args.tile, args.fuse)
         forms[form] = round(r["value"], 1)
         if form == "selector":
             forms["selector_fp64_tile_fraction"] = round(r["fp64_tile_fraction"], 5)
@@ -413,6 +428,7 @@ def main():
             "nx": n, "ny": n, "problem": "ot", "precision": args.precision,
             "cfl": 0.25, "alpha": 0.18, "hlld_tol": 1e-4,
             "step": "one SSPRK2 step of the whole grid",
+            "stages_per_launch": stages_per_launch,
         },
         "roofline": roofline,
         "cpu_baseline": cpu,
