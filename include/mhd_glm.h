@@ -77,10 +77,11 @@ typedef struct mhd_cfg {
                         1: one launch per stage (U1 through HBM); 2: one
                         launch per step, stage 1 recomputed on a 2-cell halo
                         ring of each tile (MUSCL + HLLD, tile 16x15, not with
-                        fallback 2; the learned selector runs one stage per
-                        launch).  Results as with 1: bitwise in the exact
-                        forms, the same scheme in the mixed ones.  0 = the
-                        autotuned choice when tile is -1, else 1 */
+                        fallback 2; the learned selector and ensembles run
+                        one stage per launch).  Results as with 1: bitwise
+                        in the exact forms, the same scheme in the mixed
+                        ones.  0 = the autotuned choice when tile is -1,
+                        else 1 */
 } mhd_cfg;
 
 /* Progress and diagnostics of a context. */
