@@ -796,8 +796,11 @@ using TileSharedOf = TileShared<typename Pol::RP, typename Pol::RS, TX, TY>;
 
 // One stage on one tile (tile index `tile` of the instance described by
 // `a`), by one workgroup.  The caller has checked that the instance is
-// running; every thread of the block must call this.
-template <class Pol, bool MUSCL, bool HLLD, int STAGE, int TX, int TY>
+// running; every thread of the block must call this.  With HOLD, a thread
+// loads its own tile cell in phase A and keeps that state in registers for
+// phase C instead of reading it again (for kernels with the registers to
+// spare).
+template <class Pol, bool MUSCL, bool HLLD, int STAGE, int TX, int TY, bool HOLD = false>
 __device__ __forceinline__ void stage_tile(const StageArgs<Pol>& a, int tile,
                                            TileSharedOf<Pol, TX, TY>& sh) {
   using R = typename Pol::R;
@@ -826,20 +829,47 @@ __device__ __forceinline__ void stage_tile(const StageArgs<Pol>& a, int tile,
   // Phase A: tile + halo (no corners) -> primitives in LDS.  A thread's
   // cells (at most kCells) are all loaded before any is recovered, so their
   // loads are in flight together: one memory round trip, not one per cell.
+  // Cells in box raster order, or with HOLD the tile's cells first (thread
+  // tid < NT takes tile cell tid, the one it updates in phase C), then the
+  // halo: 2-cell arms on the tile's rows, then on its columns.
   constexpr int kCells = (HN + NTHR - 1) / NTHR;
   static_assert(kCells <= 2, "phase A holds at most two cells per thread");
+  static_assert(!HOLD || NT + 4 * TX + 4 * TY <= 2 * NTHR, "phase A takes two cells per thread");
   CellIO<Pol> u[kCells];
   bool have[kCells];
+  int hbox[kCells];
 #pragma unroll
   for (int q = 0; q < kCells; q++) {
-    const int h = tid + q * NTHR, r = h / HX, cc = h - r * HX;
-    have[q] = h < HN && !((r < 2 || r >= TY + 2) && (cc < 2 || cc >= TX + 2));
+    int r, cc;
+    if constexpr (HOLD) {
+      const int e = tid + q * NTHR;
+      if (e < NT) {
+        r = e / TX;
+        cc = 2 + (e - r * TX);
+        r += 2;
+      } else if (e < NT + 4 * TY) {
+        const int m = e - NT, k = m & 3;
+        cc = k < 2 ? k : TX + k;
+        r = 2 + (m >> 2);
+      } else {
+        const int m = e - NT - 4 * TY, k = m / TX;
+        cc = 2 + (m - k * TX);
+        r = k < 2 ? k : TY + k;
+      }
+      have[q] = e < NT + 4 * TX + 4 * TY;
+    } else {
+      const int h = tid + q * NTHR;
+      r = h / HX;
+      cc = h - r * HX;
+      have[q] = h < HN && !((r < 2 || r >= TY + 2) && (cc < 2 || cc >= TX + 2));
+    }
+    hbox[q] = r * HX + cc;
     if (have[q]) u[q].load(a.src, ps, (size_t)wrap(j0 + r - 2, ny) * nx + wrap(i0 + cc - 2, nx));
   }
 #pragma unroll
   for (int q = 0; q < kCells; q++) {
     if (!have[q]) continue;
-    const int h = tid + q * NTHR;
+    const int h = hbox[q];
     RP P[9];
     recover<MUSCL>(cp, u[q], a.lowbeta_ratio, P);
 #pragma unroll
@@ -895,7 +925,10 @@ __device__ __forceinline__ void stage_tile(const StageArgs<Pol>& a, int tile,
       rhs[k] = flux_div<A, RS>(sF[k * NF + xr], sF[k * NF + xl], sF[k * NF + yt],
                               sF[k * NF + yb], pow2, adx, ady, aidx, aidy);
     CellIO<Pol> src;
-    src.load(a.src, ps, g);
+    if constexpr (HOLD)
+      src = u[0];
+    else
+      src.load(a.src, ps, g);
     const T damping = ch_t / smax((T)a.alpha, (T)1.0e-12);
     update_cell<STAGE>(a, src, g, rhs, dt, damping, out);
     if constexpr (sizeof(R) == sizeof(T)) {
@@ -1009,7 +1042,7 @@ __attribute__((amdgpu_waves_per_eu(StageWaves<Pol>::value))) void k_stage(StageA
     tile = xcd_tile((int)blockIdx.x, a.ntiles);
   }
   if (a.flags && a.flags[tile] != 0) return;
-  stage_tile<Pol, MUSCL, HLLD, STAGE, TX, TY>(a, tile, sh);
+  stage_tile<Pol, MUSCL, HLLD, STAGE, TX, TY, sizeof(typename Pol::R) == 4>(a, tile, sh);
 }
 
 // One stage over listed tiles (the mixed form's FP64 fallback queue, the
