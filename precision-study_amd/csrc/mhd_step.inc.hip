@@ -794,15 +794,35 @@ struct TileShared {
 template <class Pol, int TX, int TY>
 using TileSharedOf = TileShared<typename Pol::RP, typename Pol::RS, TX, TY>;
 
+// Where phase C finds a thread's own cell state: read again from memory
+// (HOLD_NONE), kept in registers since phase A (HOLD_REGS, for kernels with
+// the registers to spare), or stored to LDS in phase A (HOLD_LDS, for
+// kernels with the LDS to spare: `sU`, 9 planes of NT states).
+enum Hold { HOLD_NONE = 0, HOLD_REGS = 1, HOLD_LDS = 2 };
+
+// The single-grid stage kernel's choice: registers for FP32-flux forms,
+// LDS for FP64-flux forms where the tile's states fit beside the tile's
+// primitives and fluxes within 80 KB (two blocks per CU).
+template <class Pol, int TX, int TY>
+struct StageHold {
+  static constexpr int value =
+      sizeof(typename Pol::R) == 4
+          ? HOLD_REGS
+          : (sizeof(TileSharedOf<Pol, TX, TY>) + 9 * TX * TY * sizeof(typename Pol::S) <= 81920
+                 ? HOLD_LDS
+                 : HOLD_NONE);
+};
+
 // One stage on one tile (tile index `tile` of the instance described by
 // `a`), by one workgroup.  The caller has checked that the instance is
-// running; every thread of the block must call this.  With HOLD, a thread
-// loads its own tile cell in phase A and keeps that state in registers for
-// phase C instead of reading it again (for kernels with the registers to
-// spare).
-template <class Pol, bool MUSCL, bool HLLD, int STAGE, int TX, int TY, bool HOLD = false>
+// running; every thread of the block must call this.  With a HOLD other
+// than HOLD_NONE a thread loads its own tile cell first in phase A, and
+// phase C takes that state from registers or from `sU` instead of reading
+// it again.
+template <class Pol, bool MUSCL, bool HLLD, int STAGE, int TX, int TY, int HOLD = HOLD_NONE>
 __device__ __forceinline__ void stage_tile(const StageArgs<Pol>& a, int tile,
-                                           TileSharedOf<Pol, TX, TY>& sh) {
+                                           TileSharedOf<Pol, TX, TY>& sh,
+                                           typename Pol::S* sU = nullptr) {
   using R = typename Pol::R;
   using RP = typename Pol::RP;
   using RS = typename Pol::RS;
@@ -834,14 +854,15 @@ __device__ __forceinline__ void stage_tile(const StageArgs<Pol>& a, int tile,
   // halo: 2-cell arms on the tile's rows, then on its columns.
   constexpr int kCells = (HN + NTHR - 1) / NTHR;
   static_assert(kCells <= 2, "phase A holds at most two cells per thread");
-  static_assert(!HOLD || NT + 4 * TX + 4 * TY <= 2 * NTHR, "phase A takes two cells per thread");
+  static_assert(HOLD == HOLD_NONE || NT + 4 * TX + 4 * TY <= 2 * NTHR,
+                "phase A takes two cells per thread");
   CellIO<Pol> u[kCells];
   bool have[kCells];
   int hbox[kCells];
 #pragma unroll
   for (int q = 0; q < kCells; q++) {
     int r, cc;
-    if constexpr (HOLD) {
+    if constexpr (HOLD != HOLD_NONE) {
       const int e = tid + q * NTHR;
       if (e < NT) {
         r = e / TX;
@@ -865,6 +886,12 @@ __device__ __forceinline__ void stage_tile(const StageArgs<Pol>& a, int tile,
     }
     hbox[q] = r * HX + cc;
     if (have[q]) u[q].load(a.src, ps, (size_t)wrap(j0 + r - 2, ny) * nx + wrap(i0 + cc - 2, nx));
+  }
+  if constexpr (HOLD == HOLD_LDS) {
+    if (tid < NT) {
+#pragma unroll
+      for (int k = 0; k < 9; k++) sU[k * NT + tid] = u[0].v[k];
+    }
   }
 #pragma unroll
   for (int q = 0; q < kCells; q++) {
@@ -925,10 +952,14 @@ __device__ __forceinline__ void stage_tile(const StageArgs<Pol>& a, int tile,
       rhs[k] = flux_div<A, RS>(sF[k * NF + xr], sF[k * NF + xl], sF[k * NF + yt],
                               sF[k * NF + yb], pow2, adx, ady, aidx, aidy);
     CellIO<Pol> src;
-    if constexpr (HOLD)
+    if constexpr (HOLD == HOLD_REGS) {
       src = u[0];
-    else
+    } else if constexpr (HOLD == HOLD_LDS) {
+#pragma unroll
+      for (int k = 0; k < 9; k++) src.v[k] = sU[k * NT + tid];
+    } else {
       src.load(a.src, ps, g);
+    }
     const T damping = ch_t / smax((T)a.alpha, (T)1.0e-12);
     update_cell<STAGE>(a, src, g, rhs, dt, damping, out);
     if constexpr (sizeof(R) == sizeof(T)) {
@@ -1029,7 +1060,9 @@ template <class Pol> struct StageWaves { static constexpr int value = sizeof(typ
 template <class Pol, bool MUSCL, bool HLLD, int STAGE, int TX, int TY>
 __global__ __launch_bounds__((TX * TY + 63) / 64 * 64)
 __attribute__((amdgpu_waves_per_eu(StageWaves<Pol>::value))) void k_stage(StageArgs<Pol> a) {
+  constexpr int hold = StageHold<Pol, TX, TY>::value;
   __shared__ TileSharedOf<Pol, TX, TY> sh;
+  __shared__ typename Pol::S sU[hold == HOLD_LDS ? 9 * TX * TY : 1];
   Ctrl<typename Pol::T>* ctrl = a.ctrl;
   if (ctrl->run != RUN_ACTIVE) return;
   if (STAGE == 2 && ctrl->bad[0] != INT_MAX) return;
@@ -1042,7 +1075,7 @@ __attribute__((amdgpu_waves_per_eu(StageWaves<Pol>::value))) void k_stage(StageA
     tile = xcd_tile((int)blockIdx.x, a.ntiles);
   }
   if (a.flags && a.flags[tile] != 0) return;
-  stage_tile<Pol, MUSCL, HLLD, STAGE, TX, TY, sizeof(typename Pol::R) == 4>(a, tile, sh);
+  stage_tile<Pol, MUSCL, HLLD, STAGE, TX, TY, hold>(a, tile, sh, sU);
 }
 
 // One stage over listed tiles (the mixed form's FP64 fallback queue, the
