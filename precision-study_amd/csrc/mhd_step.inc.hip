@@ -1341,7 +1341,9 @@ __device__ __forceinline__ int find_instance(const int* prefix, int n, int item)
 template <class Pol, bool MUSCL, bool HLLD, int STAGE, int TX, int TY>
 __global__ __launch_bounds__((TX * TY + 63) / 64 * 64)
 __attribute__((amdgpu_waves_per_eu(StageWaves<Pol>::value))) void k_stage_queue(EnsArgs<Pol> e) {
+  constexpr int hold = StageHold<Pol, TX, TY>::value;
   __shared__ TileSharedOf<Pol, TX, TY> sh;
+  __shared__ typename Pol::S sU[hold == HOLD_LDS ? 9 * TX * TY : 1];
   __shared__ int s_item, s_count;
   // Items are claimed kClaim at a time (consecutive tiles, nearly always of
   // one instance): fewer queue atomics per tile at the cost of at most
@@ -1387,8 +1389,8 @@ __attribute__((amdgpu_waves_per_eu(StageWaves<Pol>::value))) void k_stage_queue(
       const InstDesc<Pol>& d = e.desc[cur];
       if (uni(d.ctrl->run) != RUN_ACTIVE) continue;
       if (STAGE == 2 && uni(d.ctrl->bad[0]) != INT_MAX) continue;
-      stage_tile<Pol, MUSCL, HLLD, STAGE, TX, TY>(inst_args<Pol, STAGE>(e, d, cur_lo),
-                                                  item - cur_lo, sh);
+      stage_tile<Pol, MUSCL, HLLD, STAGE, TX, TY, hold>(inst_args<Pol, STAGE>(e, d, cur_lo),
+                                                        item - cur_lo, sh, sU);
       __syncthreads();  // LDS is reused by the next item
     }
   }

@@ -1,13 +1,9 @@
 """GPU tests of ensembles (persistent work-queue stage kernels).
 
 Contract: an instance advanced inside an ensemble is bitwise the same
-instance advanced alone by a single context (same exact form, same tile
-shape), for uneven sizes, instances that finish at different step counts,
-and an instance that halts (which must not disturb the others).  The
-contraction-on mixed form is held to round-off instead: the persistent
-kernel and the single-grid kernel are separate instantiations (the latter
-keeps each thread's cell state in registers between its phases), and the
-compiler may contract their FMAs differently.
+instance advanced alone by a single context (same form, same tile shape),
+for uneven sizes, instances that finish at different step counts, and an
+instance that halts (which must not disturb the others).
 """
 import sys
 
@@ -40,7 +36,7 @@ def bits(u):
     return np.ascontiguousarray(u).view(np.uint8)
 
 
-@pytest.mark.parametrize("form", ["fp64", "fp32", "mixed"])
+@pytest.mark.parametrize("form", ["fp64", "fp32", "mixed", "mixed_kahan"])
 def test_ensemble_matches_single(mhd, form):
     cfg = mhd.SolverConfig(nx=2, ny=2, precision=mhd.PRECISIONS[form])
     sizes = [(64, 64), (48, 80), (96, 32), (40, 40), (33, 57)]
@@ -56,14 +52,6 @@ def test_ensemble_matches_single(mhd, form):
         u_s, st_s = single(mhd, cfg, nx, ny, "otv", p, 0.05)
         assert sts[i].code == 0 and sts[i].done == 1
         assert sts[i].steps == st_s.steps and sts[i].t == st_s.t
-        if form == "mixed":
-            for k in range(8):
-                ref = u_s[k].astype(np.float64)
-                scale = np.sqrt(np.mean(ref ** 2))
-                if scale > 0:
-                    err = np.sqrt(np.mean((u_e[k] - ref) ** 2)) / scale
-                    assert err < 1e-6, (i, k, err)
-            continue
         assert (sts[i].degenerate, sts[i].nonphysical) == (st_s.degenerate, st_s.nonphysical)
         assert np.array_equal(bits(u_e), bits(u_s)), f"instance {i} {nx}x{ny}"
     # Different sizes take different step counts to t_end.

@@ -129,11 +129,7 @@ def test_fallback_limit_trajectory_divergence(mhd):
     FP64 finishes; the mixed (and FP32) trajectory reaches a state at t ~ 0.42
     from which even the FP64 step is not physical, so the fallback fires and
     the FP64 check halts the run (code 3), as the FP64 form itself would from
-    that state.  An ensemble holding the instance halts the same way: after
-    the fallback fires, with the FP64 check's code, near the same time (its
-    persistent kernel is another instantiation of the mixed arithmetic, so
-    on this chaotic trajectory the step differs).  The other instance
-    finishes."""
+    that state.  An ensemble holding the instance halts identically."""
     n = 1024
     say("hard OT variant 1024^2")
     u64, s64 = run(mhd, n, mhd.PREC_FP64, problem="otv", params=OTV_HARD)
@@ -151,9 +147,7 @@ def test_fallback_limit_trajectory_divergence(mhd):
     sts = ens.advance(0.5)
     ue = ens.get_state(1)
     ens.close()
-    say(f"  ensemble: code {sts[1].code} step {sts[1].fail_step} t {sts[1].t:.4f} (alone "
-        f"{sm.t:.4f}), fallback tiles {sts[1].fallback_tiles}")
     assert sts[0].code == 0 and sts[0].done == 1
-    assert sts[1].code == 3 and sts[1].fallback_tiles >= 1
-    assert abs(sts[1].t - sm.t) < 0.05
-    assert np.all(np.isfinite(ue))
+    assert sts[1].code == 3 and sts[1].fail_step == sm.fail_step
+    assert sts[1].fallback_tiles == sm.fallback_tiles
+    assert np.array_equal(ue.view(np.uint64), um.view(np.uint64))
