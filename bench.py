@@ -1,12 +1,14 @@
 """Throughput benchmark of the MI355X 2D MHD-GLM step.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--n 4096] [--precision fp64]
+                    [--tile T] [--fuse 1|2]
 
 Workload: Orszag-Tang on an n x n periodic grid (default 4096^2, the size the
 headline metric is quoted on), MUSCL + HLLD + mixed GLM, SSPRK2, CFL 0.25,
 alpha 0.18, HLLD tolerance 1e-4.  One "step" is one full SSPRK2 step of the
-whole grid (two fused stage kernels and one control kernel); the state stays
-resident in HBM.  value = n*n*K / (wall time of K steps) in cell-updates/s,
+whole grid (two fused stage kernels and one control kernel, the autotuned
+choice; --fuse 2 runs both stages in one kernel); the state stays resident in
+HBM.  value = n*n*K / (wall time of K steps) in cell-updates/s,
 the reference's "Mcell-steps/s" unit (mhd2d_glm_cpu.cpp:589-591) times 1e6.
 
 `value` is the plain FP64 path (bitwise the reference CPU build).  Also
@@ -34,7 +36,8 @@ reported on the same JSON line:
                   kernels, each with its own dt; aggregate cell-updates/s;
   roofline     -- the dominant (stage) kernel's algorithmic HBM bytes per
                   launch over its average duration measured here with HIP
-                  events on the solver stream, against 8 TB/s;
+                  events on the solver stream, against 8 TB/s (with --fuse 2
+                  the two-stage kernel, 144 B per cell in FP64 state);
   large        -- FP64 and mixed steps at 8192^2 (4.8 GB per FP64 state);
   cpu_baseline -- the reference CPU build (oracle/_ref, serial, one core) on
                   a bounded sample of Orszag-Tang at 512^2 (the first ~50
