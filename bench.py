@@ -93,7 +93,22 @@ def cpu_baseline(n: int, t_end: float):
         "kind": kind,
         "sample": f"Orszag-Tang {n}x{n} FP64 MUSCL+HLLD to t={t_end} ({steps} SSPRK2 steps, "
                   f"{float(m.group(1)):.1f} s loop, {wall:.1f} s process), serial g++ -O2 build",
+        "threading": "1 thread per run",
+        "host_cpu": host_cpu_model(),
+        "host_cpus_visible": os.cpu_count(),
     }
+
+
+def host_cpu_model() -> str:
+    """The host CPU's model name (Linux /proc/cpuinfo), or "unknown"."""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
 
 
 def make_solver(mhdglm, form, n, tile=-1, problem="ot", fuse=0):
