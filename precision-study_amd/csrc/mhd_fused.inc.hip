@@ -7,7 +7,8 @@
 // One workgroup advances its TX x TY tile by a whole step
 // (mhd2d_glm_cpu.cpp:526-566):
 //   A1  U^n on the tile and a 4-cell halo (without the corners beyond 2
-//       cells, which no stencil reaches) -> primitives in LDS;
+//       cells, which no stencil reaches) -> primitives in LDS; each thread
+//       keeps the U^n of its stage-1 cells in registers for C1 and C2;
 //   B1  the stage-1 faces of the tile and of a 2-cell halo ring around it:
 //       the cells whose U1 the tile's stage-2 faces read;
 //   C1  U1 of those cells (rhs, update, positivity check) -> primitives in
