@@ -140,11 +140,15 @@ int mhd_advance(mhd_ctx* ctx, double t_end, long long max_steps,
 int mhd_get_status(mhd_ctx* ctx, mhd_status* st);
 
 /* Downloads the current state U (which = 0) or the stage-1 state U1
- * (which = 1, the state the reference prints on an exit-3 failure). */
+ * (which = 1, the state the reference prints on an exit-3 failure; not kept
+ * by two-stage steps, cfg.fuse 2, which return 1 for it). */
 int mhd_get_state(mhd_ctx* ctx, int which, void* soa9, size_t ld);
 
 /* Device pointer of state plane 0 (planes are ctx-internal `ld` apart,
- * reported in *ld) for callers that keep data on the device. */
+ * reported in *ld) for callers that keep data on the device.  Waits for the
+ * enqueued steps.  With two-stage steps (cfg.fuse 2) the state moves between
+ * two buffers as steps run: the pointer holds the current state until the
+ * next step is enqueued (mhd_advance and this call move it back here). */
 int mhd_device_state(mhd_ctx* ctx, void** dev_ptr, size_t* ld);
 /* HIP stream of the context (as void*: hipStream_t). */
 void* mhd_stream(mhd_ctx* ctx);
