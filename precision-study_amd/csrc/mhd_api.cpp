@@ -20,6 +20,7 @@
 #include <dlfcn.h>
 
 #include <climits>
+#include <cstdint>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -839,6 +840,130 @@ int mhd_time_kernels(mhd_ctx* c, double t_end, int nsteps, double* avg_ms) {
 }
 
 }  // extern "C"
+
+// ---------------------------------------------------------------------------
+// No-allocation run: a caller's device state advanced with U1, the control
+// block and the fallback lists in the fixed scratch array of k_noalloc.hip.
+// ---------------------------------------------------------------------------
+namespace {
+
+size_t align256(size_t x) { return (x + 255) / 256 * 256; }
+
+int noalloc_reject(const char* why) {
+  std::fprintf(stderr, "mhd_run_noalloc: %s\n", why);
+  return 1;
+}
+
+int noalloc_hip(hipError_t e, const char* what) {
+  std::fprintf(stderr, "mhd_run_noalloc: HIP error in %s: %s\n", what, hipGetErrorString(e));
+  return 9;
+}
+
+}  // namespace
+
+extern "C" int mhd_run_noalloc(const mhd_cfg* cfg_in, void* state, size_t ld, double t_end,
+                               long long max_steps, void* stream, mhd_status* st) {
+  if (!cfg_in || !state) return noalloc_reject("null configuration or state");
+  mhd_cfg cfg = *cfg_in;
+  if (cfg.nx < 2 || cfg.ny < 2 || (long long)cfg.nx * cfg.ny > (long long)INT_MAX)
+    return noalloc_reject("grid must be at least 2 x 2");
+  if (!(cfg.cfl > 0.0 && cfg.cfl < 1.0) || !(cfg.alpha > 0.0))
+    return noalloc_reject("cfl must lie in (0,1) and alpha be positive");
+  if (cfg.precision == MHD_PREC_MIXED_KAHAN || !launchers_for(cfg.precision))
+    return noalloc_reject("unsupported precision (MIXED_KAHAN keeps two planes per variable)");
+  if (cfg.precision == MHD_PREC_REF_MIXED && !cfg.muscl)
+    return noalloc_reject("REF_MIXED runs MUSCL only");
+  if (cfg.fuse == 2) return noalloc_reject("one stage per launch only (fuse 0 or 1)");
+  const mhd::Launchers* L = launchers_for(cfg.precision);
+  const size_t cells = (size_t)cfg.nx * cfg.ny;
+  if (ld < cells) return noalloc_reject("ld is smaller than nx * ny");
+  if ((uintptr_t)state % L->state_elem != 0) return noalloc_reject("misaligned state pointer");
+  if (!(cfg.lowbeta_ratio > 0.0)) cfg.lowbeta_ratio = 16.0;
+  if (cfg.batch_steps <= 0) cfg.batch_steps = 32;
+  if (cfg.hlld_tol <= 0.0) cfg.hlld_tol = 1.0e-4;
+  int tile = cfg.tile;
+  if (tile < 0 || tile >= mhd::TILE_COUNT) tile = tuned_tile(cfg.precision, (long long)cells);
+  if (!(cfg.muscl && cfg.hlld)) tile = mhd::kDefaultTile;
+  cfg.tile = tile;
+  int tx, ty;
+  mhd::tile_dims(tile, tx, ty);
+  const size_t ntiles = (size_t)((cfg.nx + tx - 1) / tx) * ((cfg.ny + ty - 1) / ty);
+  const bool lists = cfg.precision == MHD_PREC_MIXED && cfg.fallback != 1;
+  // Scratch layout: U1 (planes ld apart, like the caller's state), the
+  // control block, then the two fallback lists.
+  const size_t u1_bytes = align256((size_t)L->state_planes * ld * L->state_elem);
+  const size_t ctrl_bytes = align256(L->ctrl_bytes);
+  const size_t list_bytes = lists ? align256(2 * ntiles * sizeof(int)) : 0;
+  const size_t need = u1_bytes + ctrl_bytes + list_bytes;
+  void* base = nullptr;
+  size_t cap = 0;
+  if (hipError_t e = mhd::noalloc_scratch(&base, &cap)) return noalloc_hip(e, "scratch lookup");
+  if (need > cap) {
+    char msg[160];
+    std::snprintf(msg, sizeof(msg), "the grid needs %zu bytes of scratch; the maximum is %zu",
+                  need, cap);
+    return noalloc_reject(msg);
+  }
+  char* sc = (char*)base;
+  void* u1 = sc;
+  void* ctrl = sc + u1_bytes;
+  int* redo = lists ? (int*)(sc + u1_bytes + ctrl_bytes) : nullptr;
+  hipStream_t s = (hipStream_t)stream;
+
+  mhd::GridDesc g;
+  g.nx = cfg.nx;
+  g.ny = cfg.ny;
+  g.plane = ld;
+  g.muscl = cfg.muscl ? 1 : 0;
+  g.hlld = cfg.hlld ? 1 : 0;
+  g.alpha = cfg.alpha;
+  g.tol = cfg.hlld_tol;
+  g.tile = tile;
+  g.lowbeta_ratio = cfg.lowbeta_ratio;
+  if (lists) {
+    g.redo_list[0] = redo;
+    g.redo_list[1] = redo + ntiles;
+    g.redo = mhd::launchers_exact(mhd::MODE_FP64)->stage_list;
+    g.redo_all = cfg.fallback == 2 ? 1 : 0;
+  }
+  // The control block starts from t = 0 with every counter and speed slot
+  // cleared (the array keeps the previous call's contents).
+  alignas(16) unsigned char host_ctrl[sizeof(Ctrl<double>) > sizeof(Ctrl<float>)
+                                          ? sizeof(Ctrl<double>)
+                                          : sizeof(Ctrl<float>)];
+  auto read_back = [&]() -> hipError_t {
+    hipError_t e = hipMemcpyAsync(host_ctrl, ctrl, L->ctrl_bytes, hipMemcpyDeviceToHost, s);
+    return e == hipSuccess ? hipStreamSynchronize(s) : e;
+  };
+  auto status = [&](mhd_status* out) {
+    if (L->time_is_float)
+      fill_status_of(cfg.hlld, cfg.nx, cfg.ny, (const Ctrl<float>*)host_ctrl, out);
+    else
+      fill_status_of(cfg.hlld, cfg.nx, cfg.ny, (const Ctrl<double>*)host_ctrl, out);
+  };
+  if (L->time_is_float) init_ctrl((Ctrl<float>*)host_ctrl, cfg);
+  else init_ctrl((Ctrl<double>*)host_ctrl, cfg);
+  hipError_t e = hipMemcpyAsync(ctrl, host_ctrl, L->ctrl_bytes, hipMemcpyHostToDevice, s);
+  if (e == hipSuccess) e = hipStreamSynchronize(s);  // host_ctrl is reused below
+  if (e == hipSuccess) e = L->speed(g, state, ctrl, s);
+  if (e == hipSuccess) e = L->commit(ctrl, s);
+  if (e == hipSuccess) e = L->set_target(ctrl, t_end, s);
+  if (e == hipSuccess) e = read_back();
+  if (e != hipSuccess) return noalloc_hip(e, "start of the run");
+  mhd_status cur;
+  status(&cur);
+  long long left = max_steps < 0 ? LLONG_MAX : max_steps;
+  while (left > 0 && cur.code == 0 && !cur.done) {
+    const int n = (int)(left < cfg.batch_steps ? left : cfg.batch_steps);
+    for (int q = 0; q < n && e == hipSuccess; q++) e = L->step(g, state, u1, ctrl, s, nullptr);
+    if (e == hipSuccess) e = read_back();
+    if (e != hipSuccess) return noalloc_hip(e, "step");
+    left -= n;
+    status(&cur);
+  }
+  if (st) *st = cur;
+  return cur.code;
+}
 
 // ---------------------------------------------------------------------------
 // Operator-level batches.

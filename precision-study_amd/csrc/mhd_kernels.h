@@ -208,6 +208,15 @@ const Launchers* launchers_exact(int mode);  // MODE_FP64, MODE_FP32, MODE_REF_M
 const Launchers* launchers_fma();            // MODE_FP64_FMA
 const Launchers* launchers_mixed(int mode);  // MODE_MIXED, MODE_MIXED_KAHAN
 
+// Scratch of the no-allocation run (k_noalloc.hip): a fixed device array per
+// device, reserved by the code object's loader.  24 MiB holds the stage-1
+// state of an FP64 grid of up to 3.4e5 cells (512 x 512 with room to spare),
+// its control block and the mixed form's fallback lists.
+constexpr size_t kNoAllocScratchBytes = (size_t)24 << 20;
+// Device address of the current device's scratch array (looked up once per
+// device) and its size in bytes.
+hipError_t noalloc_scratch(void** ptr, size_t* bytes);
+
 // Operator-level batch kernels (exact arithmetic; device arrays, AoS 9 per
 // state).  R is double or float.
 template <class R>

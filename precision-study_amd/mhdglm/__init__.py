@@ -52,7 +52,7 @@ EXPORTED = (
     "mhd_ensemble_time_kernels", "mhd_autotuned_tile", "mhd_autotuned_fuse",
     "mhd_selector_configure", "mhd_selector_stats", "mhd_selector_tiles", "mhd_tile_features",
     "mhd_pair_tile_errors", "mhd_mlp_param_count", "mhd_mlp_forward", "mhd_mlp_gradients",
-    "mhd_mlp_train",
+    "mhd_mlp_train", "mhd_run_noalloc",
 )
 
 
@@ -134,6 +134,9 @@ def load(path: str = LIB_PATH) -> ctypes.CDLL:
     lib.mhd_checkpoint_load.restype = I
     lib.mhd_time_kernels.argtypes = [P, D, I, ctypes.POINTER(ctypes.c_double)]
     lib.mhd_time_kernels.restype = I
+    lib.mhd_run_noalloc.argtypes = [ctypes.POINTER(Cfg), P, S, D, ctypes.c_longlong, P,
+                                    ctypes.POINTER(Status)]
+    lib.mhd_run_noalloc.restype = I
     lib.mhd_glm_flux_batch.argtypes = [I, I, I, P, P, D, I, D, P, P]
     lib.mhd_glm_flux_batch.restype = I
     lib.mhd_cons_to_prim_batch.argtypes = [I, I, P, P]
@@ -288,6 +291,12 @@ class Solver:
     def enqueue_steps(self, t_end: float, nsteps: int):
         self._check(self.lib.mhd_enqueue_steps(self.h, t_end, nsteps))
 
+    def device_state(self):
+        """(device pointer of state plane 0, elements between planes)."""
+        ptr, ld = ctypes.c_void_p(), ctypes.c_size_t()
+        self._check(self.lib.mhd_device_state(self.h, ctypes.byref(ptr), ctypes.byref(ld)))
+        return ptr.value, ld.value
+
     def save(self, path: str):
         """Binary checkpoint (control block + raw state planes)."""
         self._check(self.lib.mhd_checkpoint_save(self.h, os.fsencode(path)))
@@ -341,6 +350,18 @@ class Solver:
 
     def stream(self) -> int:
         return self.lib.mhd_stream(self.h)
+
+
+def run_noalloc(cfg: SolverConfig, state_ptr: int, ld: int, t_end: float,
+                max_steps: int = -1, stream: int = 0) -> Status:
+    """mhd_run_noalloc: advances a device-resident state (pointer to 9 planes
+    `ld` elements apart) in place from t=0, with no device allocation."""
+    st = Status()
+    rc = load().mhd_run_noalloc(ctypes.byref(_cfg(cfg)), ctypes.c_void_p(state_ptr), ld, t_end,
+                                max_steps, ctypes.c_void_p(stream), ctypes.byref(st))
+    if rc not in (0, 2, 3, 4):
+        raise MhdError(rc, "mhd_run_noalloc failed (see stderr)")
+    return st
 
 
 def mlp_param_count() -> int:
