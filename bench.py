@@ -237,23 +237,9 @@ def selector_breakdown(mhdglm, n, t_end=0.5):
         "all": round(us["selector"] - us["mixed"], 2)}}
 
 
-ENSEMBLE_SIZES = (128, 192, 256, 384, 512, 768, 1024)
-
-
-def ensemble_workload(n, seed=2026):
-    """The batched-ensemble workload: n Orszag-Tang instances with sizes drawn
-    from ENSEMBLE_SIZES, B amplitude uniform in [0.8, 1.2] and the phases of
-    the four sine terms uniform in [0, 2 pi), from a seeded generator."""
-    import numpy as np
-
-    rng = np.random.default_rng(seed)
-    sizes = [int(x) for x in rng.choice(ENSEMBLE_SIZES, size=n)]
-    params = [(float(rng.uniform(0.8, 1.2)),) + tuple(float(x) for x in rng.uniform(0, 2 * np.pi, 4))
-              for _ in range(n)]
-    return sizes, params
-
-
 def run_ensemble(mhdglm, torch, n, form, t_end):
+    from mhdglm.workloads import ENSEMBLE_SEED, ensemble_workload
+
     sizes, params = ensemble_workload(n)
     log(f"bench: ensemble of {n} OT instances ({form}), sizes {sorted(set(sizes))}, to t={t_end}")
     ens = mhdglm.Ensemble(mhdglm.SolverConfig(nx=2, ny=2, precision=mhdglm.PRECISIONS[form]),
@@ -269,7 +255,7 @@ def run_ensemble(mhdglm, torch, n, form, t_end):
     codes = [s.code for s in sts]
     updates = sum(s * s * st.steps for s, st in zip(sizes, sts))
     return {
-        "instances": n, "form": form, "t_end": t_end, "seed": 2026,
+        "instances": n, "form": form, "t_end": t_end, "seed": ENSEMBLE_SEED,
         "sizes": {str(k): sizes.count(k) for k in sorted(set(sizes))},
         "wall_s": round(wall, 3), "cell_updates": updates,
         "value": round(updates / wall, 1), "unit": "cell-updates/s",

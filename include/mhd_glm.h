@@ -82,6 +82,19 @@ typedef struct mhd_cfg {
                         in the exact forms, the same scheme in the mixed
                         ones.  0 = the autotuned choice when tile is -1,
                         else 1 */
+  int rollback_steps; /* MIXED with the fallback on (fallback 0): a run that
+                        halts anyway (an FP64-confirmed unusable cell after
+                        the trajectory drifted from the FP64 one) is rolled
+                        back to a device snapshot at least this many steps
+                        before the failure (snapshots every this many steps;
+                        the state given at set_state is always kept) and
+                        re-run in FP64 until this many steps past the
+                        failure, then resumes mixed.  A failure inside such
+                        a window restarts the run from the given state in
+                        FP64 throughout, which halts only where the FP64
+                        form halts.  Costs three more state buffers.
+                        0 = default (256), < 0 = off (halts as the other
+                        forms do, the reference's contract) */
 } mhd_cfg;
 
 /* Progress and diagnostics of a context. */
@@ -102,6 +115,11 @@ typedef struct mhd_status {
                                           met an unusable cell, recomputed by
                                           the one-stage kernels (with their
                                           FP64 fallback) */
+  unsigned long long exact_steps; /* MIXED rollback: steps of the run taken
+                                     with every tile in FP64 */
+  int rollbacks;                  /* MIXED rollback: windows re-run in FP64 */
+  int restarts;                   /* MIXED rollback: restarts from the given
+                                     state in FP64 throughout (0 or 1) */
 } mhd_status;
 
 typedef struct mhd_ctx mhd_ctx;

@@ -45,16 +45,21 @@ using namespace oracle;
       reconstruct(c, Pm + 9 * (size_t)s, P0 + 9 * (size_t)s,                   \
                   Pp + 9 * (size_t)s, Pl + 9 * (size_t)s, Pr + 9 * (size_t)s); \
   }                                                                            \
-  extern "C" void orc_init_##SUF(const char* problem, int nx, int ny, R* u) {  \
+  extern "C" void orc_init_params_##SUF(const char* problem,                   \
+                                         const double* params, int nx, int ny, \
+                                         R* u) {                               \
     const Consts<R> c(1.0e-4);                                                 \
     const R dx = ((R)1.0 - (R)0.0) / (R)nx, dy = ((R)1.0 - (R)0.0) / (R)ny;    \
     const std::string prob(problem);                                           \
     for (int j = 0; j < ny; j++)                                               \
       for (int i = 0; i < nx; i++) {                                           \
         R P[NV];                                                               \
-        initial_primitive<R>(prob, i, j, dx, dy, P);                           \
+        initial_primitive<R>(prob, i, j, dx, dy, P, params);                   \
         prim_to_state(c, P, u + ((size_t)j * nx + i) * 9);                     \
       }                                                                        \
+  }                                                                            \
+  extern "C" void orc_init_##SUF(const char* problem, int nx, int ny, R* u) {  \
+    orc_init_params_##SUF(problem, nullptr, nx, ny, u);                        \
   }                                                                            \
   extern "C" double orc_max_signal_speed_##SUF(int nx, int ny, const R* u) {   \
     const Consts<R> c(1.0e-4);                                                 \

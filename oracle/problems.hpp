@@ -7,6 +7,9 @@
 //    vy=0.01*sin(4 pi x)*(exp(-(y-0.25)^2/0.005)+exp(-(y-0.75)^2/0.005)),
 //    Bx=0.1); it has no reference implementation, so its arithmetic is this
 //   file's and the product host mirrors it.
+// otv: Orszag-Tang with the ensemble workload's variations, params = {B
+//   amplitude, phases added to the sine arguments of vx, vy, Bx, By}; with
+//   {1, 0, 0, 0, 0} it is `ot` bit for bit (no reference counterpart either).
 #ifndef ORACLE_PROBLEMS_HPP
 #define ORACLE_PROBLEMS_HPP
 
@@ -18,7 +21,7 @@ namespace oracle {
 // Fills P[9] (primitives + psi) at cell (i,j) of an nx*ny grid on [0,1]^2.
 template <class R>
 inline void initial_primitive(const std::string& problem, int i, int j, R dx,
-                              R dy, R P[9]) {
+                              R dy, R P[9], const double* params = nullptr) {
   const R x_min = (R)0.0, y_min = (R)0.0;
   const R x = x_min + ((R)i + (R)0.5) * dx;
   const R y = y_min + ((R)j + (R)0.5) * dy;
@@ -51,6 +54,18 @@ inline void initial_primitive(const std::string& problem, int i, int j, R dx,
     P[4] = (R)2.5;
     P[5] = (R)0.1;
     P[6] = (R)0.0;
+    P[7] = (R)0.0;
+    P[8] = (R)0.0;
+  } else if (problem == "otv") {
+    const R b = (R)params[0];
+    const R inv_sqrt_4pi = (R)1.0 / std::sqrt((R)4.0 * pi);
+    P[0] = (R)25.0 / ((R)36.0 * pi);
+    P[1] = -std::sin(two_pi * y + (R)params[1]);
+    P[2] = std::sin(two_pi * x + (R)params[2]);
+    P[3] = (R)0.0;
+    P[4] = (R)5.0 / ((R)12.0 * pi);
+    P[5] = -b * std::sin(two_pi * y + (R)params[3]) * inv_sqrt_4pi;
+    P[6] = b * std::sin((R)2.0 * two_pi * x + (R)params[4]) * inv_sqrt_4pi;
     P[7] = (R)0.0;
     P[8] = (R)0.0;
   } else {

@@ -50,6 +50,9 @@ class Oracle:
             g = getattr(self.lib, f"orc_init_{suf}")
             g.argtypes = [ctypes.c_char_p, I, I, P]
             g.restype = None
+            g = getattr(self.lib, f"orc_init_params_{suf}")
+            g.argtypes = [ctypes.c_char_p, P, I, I, P]
+            g.restype = None
             g = getattr(self.lib, f"orc_max_signal_speed_{suf}")
             g.argtypes = [I, I, P]
             g.restype = D
@@ -96,10 +99,15 @@ class Oracle:
         self._fn("reconstruct_batch", dtype)(a.shape[0], _p(a), _p(b), _p(c), _p(Pl), _p(Pr))
         return Pl, Pr
 
-    def init(self, problem, nx, ny, dtype=np.float64):
-        """Initial state as AoS (ny, nx, 9)."""
+    def init(self, problem, nx, ny, dtype=np.float64, params=None):
+        """Initial state as AoS (ny, nx, 9); `params` for "otv" (5 values)."""
         u = np.empty((ny, nx, 9), dtype=dtype)
-        self._fn("init", dtype)(problem.encode(), nx, ny, _p(u))
+        if params is None:
+            self._fn("init", dtype)(problem.encode(), nx, ny, _p(u))
+        else:
+            p = np.ascontiguousarray(params, dtype=np.float64)
+            assert problem == "otv" and p.size == 5
+            self._fn("init_params", dtype)(problem.encode(), _p(p), nx, ny, _p(u))
         return u
 
     def max_signal_speed(self, u_aos):

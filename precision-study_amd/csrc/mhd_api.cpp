@@ -34,6 +34,28 @@
 using mhd::Ctrl;
 using mhd::GridDesc;
 
+namespace {
+
+// Rollback of the MIXED form (mhd_cfg.rollback_steps): device copies of one
+// instance's state with host copies of its control block.  Slot 0 holds the
+// state given at set_state (or a loaded checkpoint) for the whole run; slots
+// 1 and 2 take turns holding snapshots of the run, every `every` steps.
+struct Rollback {
+  int every = 0;                // snapshot interval and window margin (0: off)
+  void* slot[3] = {nullptr, nullptr, nullptr};
+  std::vector<char> ctrl[3];
+  int step[3] = {-1, -1, -1};   // step of the slot's state (-1: empty)
+  int newest = 0;               // slot written last
+  int rollbacks = 0;            // windows re-run in FP64
+  int restarts = 0;             // restarts from slot 0 in FP64 throughout
+  bool all_exact = false;       // the run is FP64 from slot 0 on
+};
+
+// More windows than this in one run restart it in FP64 throughout.
+constexpr int kMaxRollbacks = 16;
+
+}  // namespace
+
 struct mhd_ctx {
   mhd_cfg cfg;
   GridDesc grid;
@@ -76,6 +98,7 @@ struct mhd_ctx {
     void* u1x = nullptr;            // FP64 scratch stage output
     void* ctrl_scratch = nullptr;   // control-block snapshot for side evaluations
   } sel;
+  Rollback rb;
 };
 
 namespace {
@@ -155,6 +178,7 @@ void fill_status_of(int hlld, int nx, int ny, const Ctrl<R>* h, mhd_status* st) 
   st->fail_step = h->fail_step;
   st->fallback_tiles = h->redo_tiles;
   st->fused_redo_steps = h->fz_redo_steps;
+  st->exact_steps = h->exact_steps;
 }
 
 template <class R>
@@ -246,8 +270,118 @@ int status_of(mhd_ctx* c, mhd_status* st) {
     fill_status(c, (const Ctrl<float>*)c->ctrl_host, &tmp);
   else
     fill_status(c, (const Ctrl<double>*)c->ctrl_host, &tmp);
+  tmp.rollbacks = c->rb.rollbacks;
+  tmp.restarts = c->rb.restarts;
   if (st) *st = tmp;
   return tmp.code;
+}
+
+// --- Rollback of the MIXED form ---------------------------------------------
+bool rollback_enabled(const mhd_cfg& cfg) {
+  return cfg.precision == MHD_PREC_MIXED && cfg.fallback == 0 && cfg.rollback_steps >= 0;
+}
+
+int rollback_interval(const mhd_cfg& cfg) {
+  return cfg.rollback_steps > 0 ? cfg.rollback_steps : 256;
+}
+
+hipError_t rb_alloc(Rollback& rb, size_t bytes) {
+  for (void*& p : rb.slot)
+    if (!p)
+      if (hipError_t e = hipMalloc(&p, bytes)) return e;
+  return hipSuccess;
+}
+
+void rb_free(Rollback& rb) {
+  for (void*& p : rb.slot)
+    if (p) {
+      (void)hipFree(p);
+      p = nullptr;
+    }
+}
+
+int ctrl_step(const void* ctrl_host) { return ((const Ctrl<double>*)ctrl_host)->step; }
+
+// Starts the record of a run: slot 0 takes the given state (a device copy
+// on the stream) and its control block as just read back.
+hipError_t rb_start(Rollback& rb, const void* u, size_t bytes, const void* ctrl_host, size_t cb,
+                    hipStream_t s) {
+  rb.ctrl[0].assign((const char*)ctrl_host, (const char*)ctrl_host + cb);
+  rb.step[0] = ctrl_step(ctrl_host);
+  rb.step[1] = rb.step[2] = -1;
+  rb.newest = 0;
+  rb.rollbacks = rb.restarts = 0;
+  rb.all_exact = false;
+  return hipMemcpyAsync(rb.slot[0], u, bytes, hipMemcpyDeviceToDevice, s);
+}
+
+bool rb_snapshot_due(const Rollback& rb, const void* ctrl_host) {
+  return ctrl_step(ctrl_host) - rb.step[rb.newest] >= rb.every;
+}
+
+// Snapshot of a running state into the rotating slot not written last.
+hipError_t rb_snapshot(Rollback& rb, const void* u, size_t bytes, const void* ctrl_host,
+                       size_t cb, hipStream_t s) {
+  const int k = rb.newest == 1 ? 2 : 1;
+  rb.ctrl[k].assign((const char*)ctrl_host, (const char*)ctrl_host + cb);
+  rb.step[k] = ctrl_step(ctrl_host);
+  rb.newest = k;
+  return hipMemcpyAsync(rb.slot[k], u, bytes, hipMemcpyDeviceToDevice, s);
+}
+
+// How a run that halted with code 3/4 continues: the slot to restore and the
+// step up to which every tile then runs in FP64.  A halt inside such a
+// window (an FP64-confirmed failure of the drifted state), or the
+// kMaxRollbacks-th one, restarts from slot 0 in FP64 throughout; a halt of
+// that run stands (false), as it would in the FP64 form.  Otherwise the
+// newest snapshot at least `every` steps before the failure is restored
+// (slot 0 if none) and re-run in FP64 until `every` steps past it.
+bool rb_plan(Rollback& rb, const void* ctrl_host, int& slot, int& exact_until) {
+  if (rb.all_exact) return false;
+  const Ctrl<double>* h = (const Ctrl<double>*)ctrl_host;
+  const int fail = h->fail_step;
+  slot = 0;
+  if (fail < h->exact_until || rb.rollbacks >= kMaxRollbacks) {
+    exact_until = INT_MAX;
+    rb.all_exact = true;
+    rb.restarts += 1;
+    rb.step[1] = rb.step[2] = -1;
+    rb.newest = 0;
+    return true;
+  }
+  for (int k = 1; k <= 2; k++)
+    if (rb.step[k] >= 0 && rb.step[k] <= fail - rb.every && rb.step[k] > rb.step[slot]) slot = k;
+  exact_until = fail > INT_MAX - rb.every ? INT_MAX : fail + rb.every;
+  rb.rollbacks += 1;
+  // Snapshots later than the restored one belong to the abandoned trajectory.
+  for (int k = 1; k <= 2; k++)
+    if (k != slot && rb.step[k] > rb.step[slot]) rb.step[k] = -1;
+  rb.newest = slot;
+  return true;
+}
+
+// Restores `slot` into a state buffer and its device control block (through
+// the pinned host copy `ctrl_host`), opens the FP64 window and re-arms the
+// step for t_end.  The window's first c_h comes from the FP64 form's signal
+// speed of the restored state (the snapshot's was the mixed form's), so a
+// restart from the given state is the FP64 run bit for bit.
+hipError_t rb_restore(const Rollback& rb, int slot, int exact_until, const GridDesc& g, void* u,
+                      size_t bytes, void* ctrl, void* ctrl_host, size_t cb, double t_end,
+                      hipStream_t s) {
+  const mhd::Launchers* F64 = mhd::launchers_exact(mhd::MODE_FP64);
+  hipError_t e = hipMemcpyAsync(u, rb.slot[slot], bytes, hipMemcpyDeviceToDevice, s);
+  std::memcpy(ctrl_host, rb.ctrl[slot].data(), cb);
+  ((Ctrl<double>*)ctrl_host)->exact_until = exact_until;
+  if (e == hipSuccess) e = hipMemcpyAsync(ctrl, ctrl_host, cb, hipMemcpyHostToDevice, s);
+  if (e == hipSuccess) e = F64->speed(g, u, ctrl, s);
+  if (e == hipSuccess) e = F64->commit(ctrl, s);
+  if (e == hipSuccess) e = F64->set_target(ctrl, t_end, s);
+  if (e == hipSuccess) e = hipStreamSynchronize(s);
+  return e;
+}
+
+size_t state_bytes(const mhd_ctx* c) {
+  return (size_t)c->L->state_planes * c->plane * c->L->state_elem;
 }
 
 // Steps between refreshes of the selector's tile features and flags (a
@@ -531,6 +665,15 @@ int mhd_create(const mhd_cfg* cfg, mhd_ctx** out) {
     c->grid.redo = mhd::launchers_exact(mhd::MODE_FP64)->stage_list;
     c->grid.redo_all = cfg->fallback == 2 ? 1 : 0;
   }
+  // Rollback snapshots of the mixed form.
+  if (rollback_enabled(c->cfg)) {
+    c->rb.every = rollback_interval(c->cfg);
+    if ((e = rb_alloc(c->rb, bytes)) != hipSuccess) {
+      std::fprintf(stderr, "mhd_create: %s\n", hipGetErrorString(e));
+      mhd_destroy(c);
+      return 9;
+    }
+  }
   // Two-stage steps: the alternate state buffer, and (mixed form with the
   // fallback) the list of every tile for the recomputation of a step.
   if (c->fused) {
@@ -573,6 +716,7 @@ void mhd_destroy(mhd_ctx* c) {
   if (c->redo_lists) (void)hipFree(c->redo_lists);
   if (c->ctrl) (void)hipFree(c->ctrl);
   if (c->ctrl_host) (void)hipHostFree(c->ctrl_host);
+  rb_free(c->rb);
   if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
 }
@@ -632,6 +776,9 @@ int mhd_set_state(mhd_ctx* c, const void* soa9, size_t ld) {
   if (e == hipSuccess) e = c->L->commit(c->ctrl, c->stream);
   if (e != hipSuccess) return hip_fail(c, e, "initial signal speed");
   if (int rc = read_ctrl(c)) return rc;
+  if (c->rb.every > 0)
+    HIP_OR_FAIL(c, rb_start(c->rb, c->u, state_bytes(c), c->ctrl_host, c->L->ctrl_bytes,
+                            c->stream));
   c->have_state = true;
   return 0;
 }
@@ -640,8 +787,13 @@ int mhd_init_problem(mhd_ctx* c, const char* problem) {
   if (!c) return 1;
   if (!mhd::problem_known(problem)) return fail(c, 1, "mhd_init_problem: unknown problem");
   const size_t cells = (size_t)c->cfg.nx * c->cfg.ny;
+  if (mhd::problem_nparams(problem) != 0)
+    return fail(c, 1, std::string("mhd_init_problem: problem '") + problem +
+                          "' needs parameters (use mhd_problem_state_ex + mhd_set_state)");
   std::vector<char> host(9 * cells * io_elem(c->cfg.precision));
-  mhd_problem_state(problem, c->cfg.nx, c->cfg.ny, c->cfg.precision, host.data(), cells);
+  if (int rc = mhd_problem_state(problem, c->cfg.nx, c->cfg.ny, c->cfg.precision, host.data(),
+                                 cells))
+    return fail(c, rc, "mhd_init_problem: initial condition failed");
   return mhd_set_state(c, host.data(), cells);
 }
 
@@ -672,12 +824,15 @@ int mhd_advance(mhd_ctx* c, double t_end, long long max_steps, mhd_status* st) {
   hipError_t e = enqueue_target(c, t_end);
   if (e != hipSuccess) return hip_fail(c, e, "set target");
   if (int rc = read_ctrl(c)) return rc;
-  long long left = max_steps < 0 ? LLONG_MAX : max_steps;
   const int batch = c->cfg.batch_steps;
   const bool graphs = use_graphs();
   mhd_status s;
   status_of(c, &s);
-  while (left > 0 && s.code == 0 && !s.done) {
+  // Stop at max_steps more completed steps (steps a rollback re-runs count
+  // once).
+  const long long target = max_steps < 0 ? LLONG_MAX : (long long)s.steps + max_steps;
+  while ((long long)s.steps < target && s.code == 0 && !s.done) {
+    const long long left = target - s.steps;
     const int n = (int)(left < batch ? left : batch);
     if (graphs && n == batch) {
       if (int rc = ensure_graph(c, batch)) return rc;
@@ -688,9 +843,20 @@ int mhd_advance(mhd_ctx* c, double t_end, long long max_steps, mhd_status* st) {
         if (e != hipSuccess) return hip_fail(c, e, "step launch");
       }
     }
-    left -= n;
     if (int rc = read_ctrl(c)) return rc;
     status_of(c, &s);
+    if (c->rb.every <= 0) continue;
+    int slot, until;
+    if ((s.code == 3 || s.code == 4) && rb_plan(c->rb, c->ctrl_host, slot, until)) {
+      HIP_OR_FAIL(c, rb_restore(c->rb, slot, until, c->grid, c->u, state_bytes(c), c->ctrl,
+                                c->ctrl_host, c->L->ctrl_bytes, t_end, c->stream));
+      if (int rc = read_ctrl(c)) return rc;
+      status_of(c, &s);
+    } else if (s.code == 0 && !s.done && rb_snapshot_due(c->rb, c->ctrl_host)) {
+      if (int rc = settle_state(c)) return rc;
+      HIP_OR_FAIL(c, rb_snapshot(c->rb, c->u, state_bytes(c), c->ctrl_host, c->L->ctrl_bytes,
+                                 c->stream));
+    }
   }
   if (int rc = settle_state(c)) return rc;
   if (st) *st = s;
@@ -804,6 +970,11 @@ int mhd_checkpoint_load(mhd_ctx* c, const char* path) {
     HIP_OR_FAIL(c, hipMemcpy((char*)c->u + k * c->plane * es, buf.data() + k * cells * es,
                              cells * es, hipMemcpyHostToDevice));
   HIP_OR_FAIL(c, hipMemcpy(c->ctrl, c->ctrl_host, c->L->ctrl_bytes, hipMemcpyHostToDevice));
+  if (c->rb.every > 0) {
+    HIP_OR_FAIL(c, rb_start(c->rb, c->u, state_bytes(c), c->ctrl_host, c->L->ctrl_bytes,
+                            c->stream));
+    HIP_OR_FAIL(c, hipStreamSynchronize(c->stream));
+  }
   c->have_state = true;
   return 0;
 }
@@ -1072,6 +1243,7 @@ struct mhd_ensemble {
   int* prefix = nullptr;       // device prefix of tile counts
   unsigned* queue = nullptr;   // device work-queue counters
   int* redo_items = nullptr;   // MIXED FP64 fallback: 2 x total work items
+  std::vector<Rollback> rb;    // MIXED rollback, per instance
   mhd::EnsLaunch el;
   hipStream_t stream = nullptr;
   hipGraphExec_t graph = nullptr;
@@ -1124,6 +1296,12 @@ void ens_status(const mhd_ensemble* e, int i, mhd_status* st) {
     fill_status_of(e->cfg.hlld, e->nx[i], e->ny[i], (const Ctrl<float>*)h, st);
   else
     fill_status_of(e->cfg.hlld, e->nx[i], e->ny[i], (const Ctrl<double>*)h, st);
+  st->rollbacks = e->rb[i].rollbacks;
+  st->restarts = e->rb[i].restarts;
+}
+
+size_t ens_state_bytes(const mhd_ensemble* e, int i) {
+  return (size_t)e->L->state_planes * e->plane[i] * e->L->state_elem;
 }
 
 hipError_t ens_enqueue_step(mhd_ensemble* e, hipEvent_t* ev = nullptr) {
@@ -1169,6 +1347,7 @@ void mhd_ensemble_destroy(mhd_ensemble* e) {
     if (p) (void)hipFree(p);
   for (void* p : e->u1)
     if (p) (void)hipFree(p);
+  for (Rollback& r : e->rb) rb_free(r);
   if (e->ctrls) (void)hipFree(e->ctrls);
   if (e->ctrls_host) (void)hipHostFree(e->ctrls_host);
   if (e->desc) (void)hipFree(e->desc);
@@ -1208,6 +1387,7 @@ int mhd_ensemble_create(const mhd_cfg* cfg, int n, const int* nx, const int* ny,
   e->u1.assign(n, nullptr);
   e->have.assign(n, 0);
   e->plane.resize(n);
+  e->rb.resize(n);
   int rc = 0;
   auto hipok = [&](hipError_t x, const char* what) {
     if (x != hipSuccess && rc == 0) rc = efail(e, 9, std::string(what) + ": " + hipGetErrorString(x));
@@ -1219,6 +1399,10 @@ int mhd_ensemble_create(const mhd_cfg* cfg, int n, const int* nx, const int* ny,
     const size_t bytes = (size_t)e->L->state_planes * e->plane[i] * e->L->state_elem;
     hipok(hipMalloc(&e->u[i], bytes), "hipMalloc");
     if (rc == 0) hipok(hipMalloc(&e->u1[i], bytes), "hipMalloc");
+    if (rc == 0 && rollback_enabled(e->cfg)) {
+      e->rb[i].every = rollback_interval(e->cfg);
+      hipok(rb_alloc(e->rb[i], bytes), "hipMalloc");
+    }
   }
   const size_t cb = (size_t)n * e->L->ctrl_bytes;
   if (rc == 0) hipok(hipMalloc(&e->ctrls, cb), "hipMalloc");
@@ -1308,6 +1492,11 @@ int mhd_ensemble_set_state(mhd_ensemble* e, int i, const void* soa9, size_t ld) 
   const mhd::GridDesc g = inst_grid(e, i);
   EHIP(e, e->L->speed(g, e->u[i], d, e->stream));
   EHIP(e, e->L->commit(d, e->stream));
+  if (e->rb[i].every > 0) {
+    EHIP(e, hipMemcpyAsync(h, d, e->L->ctrl_bytes, hipMemcpyDeviceToHost, e->stream));
+    EHIP(e, hipStreamSynchronize(e->stream));
+    EHIP(e, rb_start(e->rb[i], e->u[i], ens_state_bytes(e, i), h, e->L->ctrl_bytes, e->stream));
+  }
   EHIP(e, hipStreamSynchronize(e->stream));
   e->have[i] = 1;
   return 0;
@@ -1362,6 +1551,25 @@ int mhd_ensemble_advance(mhd_ensemble* e, double t_end, long long max_steps, mhd
     }
     left -= nb;
     if (int rc = ens_read_ctrls(e)) return rc;
+    // Rollback of halted mixed instances; snapshots of the running ones.
+    bool restored = false;
+    for (int i = 0; i < e->n; i++) {
+      Rollback& r = e->rb[i];
+      if (r.every <= 0 || !e->have[i]) continue;
+      mhd_status s;
+      ens_status(e, i, &s);
+      void* h = ctrl_at(e->ctrls_host, e, i);
+      int slot, until;
+      if ((s.code == 3 || s.code == 4) && rb_plan(r, h, slot, until)) {
+        EHIP(e, rb_restore(r, slot, until, inst_grid(e, i), e->u[i], ens_state_bytes(e, i),
+                           ctrl_at(e->ctrls, e, i), h, e->L->ctrl_bytes, t_end, e->stream));
+        restored = true;
+      } else if (s.code == 0 && !s.done && rb_snapshot_due(r, h)) {
+        EHIP(e, rb_snapshot(r, e->u[i], ens_state_bytes(e, i), h, e->L->ctrl_bytes, e->stream));
+      }
+    }
+    if (restored)
+      if (int rc = ens_read_ctrls(e)) return rc;
   }
   int worst = 0;
   for (int i = 0; i < e->n; i++) {

@@ -420,6 +420,14 @@ __attribute__((amdgpu_waves_per_eu(FusedWaves<Pol, TX, TY>::value))) void k_step
     FusedArgs<Pol> fa) {
   __shared__ FusedSharedOf<Pol, TX, TY> sh;
   if (fa.a.ctrl->run != RUN_ACTIVE) return;
+  if constexpr (Pol::kRedo) {
+    // Rollback window: the step is voted out and recomputed by the
+    // single-stage kernels, whose tiles then all go to the FP64 kernel.
+    if (fa.a.redo_list && uni(fa.a.ctrl->step) < uni(fa.a.ctrl->exact_until)) {
+      if (threadIdx.x == 0 && blockIdx.x == 0) atomicOr(&fa.a.ctrl->fz_fail, 1);
+      return;
+    }
+  }
   StageArgs<Pol> a = fa.a;
   follow_cur(a, fa.bufs);
   step_tile_fused<Pol, MUSCL, HLLD, TX, TY>(a, xcd_tile((int)blockIdx.x, a.ntiles), sh);

@@ -60,6 +60,13 @@ struct Ctrl {
   int fz_pad;
   unsigned long long fz_degenerate[2], fz_nonphysical[2];
   unsigned long long fz_redo_steps;
+  // Rollback window of the mixed form: while step < exact_until every tile
+  // of the instance goes to the FP64 kernel through the fallback queue (set
+  // by the host after it restored an earlier snapshot of a failed run);
+  // exact_steps counts the completed steps so taken.
+  int exact_until;
+  int exact_pad;
+  unsigned long long exact_steps;
   // Per-block partial maxima of the signal speed, spread over kSpeedSlots
   // addresses (block b uses slot b % kSpeedSlots) so that tens of thousands
   // of blocks do not serialise on one atomic; folded into speed_acc by the

@@ -836,6 +836,13 @@ __device__ __forceinline__ void stage_tile(const StageArgs<Pol>& a, int tile,
   unsigned long long* sRed = sh.sRed;
 
   Ctrl<T>* ctrl = a.ctrl;
+  if constexpr (Pol::kRedo) {
+    // Rollback window: the whole tile goes to the FP64 kernel unevaluated.
+    if (a.redo_list && uni(ctrl->step) < uni(ctrl->exact_until)) {
+      if (threadIdx.x == 0) a.redo_list[atomicAdd(a.redo_counter, 1)] = a.redo_base + tile;
+      return;
+    }
+  }
   const T ch_t = uni(ctrl->ch), dt = uni(ctrl->dt), dx = uni(ctrl->dx), dy = uni(ctrl->dy);
   const R ch = (R)ch_t;
   const Phys<R> c = make_phys<R>(a.tol);
@@ -1250,6 +1257,7 @@ __device__ __forceinline__ void finish_one(Ctrl<T>* ctrl) {
     return;
   }
   ctrl->t = ctrl->t + ctrl->dt;
+  if (ctrl->step < ctrl->exact_until) ctrl->exact_steps += 1;
   ctrl->step += 1;
   fold_speed_slots(ctrl);
   ctrl->speed_cur = ctrl->speed_acc;

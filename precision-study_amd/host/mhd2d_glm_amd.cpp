@@ -15,8 +15,9 @@
 // MHD_PRECISION (fp64, fp32, mixed, mixed_kahan, fp64_fma, ref_mixed).  MHD_HLLD_TOLERANCE sets the HLLD tolerance
 // (default 1e-4, the value of every reference timing and checksum run);
 // MHD_TILE the stage tile shape, MHD_LOWBETA_RATIO the mixed form's FP64
-// recovery switch, MHD_FALLBACK its FP64 tile fallback (0 on, 1 off) and
-// MHD_FUSE the SSPRK2 stages per kernel launch (1 or 2).
+// recovery switch, MHD_FALLBACK its FP64 tile fallback (0 on, 1 off),
+// MHD_ROLLBACK its FP64 rollback window in steps (< 0 off) and MHD_FUSE the
+// SSPRK2 stages per kernel launch (1 or 2).
 //
 // This program only talks to the solver through include/mhd_glm.h.
 #include "mhd_glm.h"
@@ -111,7 +112,9 @@ int run(const Args& a, int precision, const char* prec_name, double tol) {
     std::cerr << "Error: alpha must be positive." << std::endl;
     return 1;
   }
-  if (!mhd::problem_known(a.problem.c_str())) {
+  // Problems the command line can name: those without parameters ("otv"
+  // needs five, which argv does not carry).
+  if (!mhd::problem_known(a.problem.c_str()) || mhd::problem_nparams(a.problem.c_str()) != 0) {
     std::cerr << "Error: problem must be 'ot', 'kh' or 'khb'." << std::endl;
     return 1;
   }
@@ -159,6 +162,7 @@ int run(const Args& a, int precision, const char* prec_name, double tol) {
   if (const char* s = std::getenv("MHD_TILE")) cfg.tile = std::atoi(s);
   if (const char* s = std::getenv("MHD_FALLBACK")) cfg.fallback = std::atoi(s);
   if (const char* s = std::getenv("MHD_FUSE")) cfg.fuse = std::atoi(s);
+  if (const char* s = std::getenv("MHD_ROLLBACK")) cfg.rollback_steps = std::atoi(s);
   mhd_ctx* ctx = nullptr;
   int rc = mhd_create(&cfg, &ctx);
   if (rc != 0) {
@@ -245,6 +249,9 @@ int run(const Args& a, int precision, const char* prec_name, double tol) {
   }
   if (st.fallback_tiles > 0)
     std::cout << "FP64 fallback stage tiles = " << st.fallback_tiles << std::endl;
+  if (st.rollbacks > 0 || st.restarts > 0)
+    std::cout << "FP64 rollbacks = " << st.rollbacks << ", restarts = " << st.restarts
+              << ", FP64 steps = " << st.exact_steps << std::endl;
 
   const std::string base = a.problem == "ot" ? "orszag_tang" : a.problem;
   const char* out_env = std::getenv("OUTDIR");

@@ -72,7 +72,7 @@ class Cfg(ctypes.Structure):
         ("precision", ctypes.c_int), ("hlld_tol", ctypes.c_double),
         ("tile", ctypes.c_int), ("batch_steps", ctypes.c_int),
         ("lowbeta_ratio", ctypes.c_double), ("fallback", ctypes.c_int),
-        ("fuse", ctypes.c_int),
+        ("fuse", ctypes.c_int), ("rollback_steps", ctypes.c_int),
     ]
 
 
@@ -84,6 +84,8 @@ class Status(ctypes.Structure):
         ("degenerate", ctypes.c_ulonglong), ("nonphysical", ctypes.c_ulonglong),
         ("bad_i", ctypes.c_int), ("bad_j", ctypes.c_int), ("fail_step", ctypes.c_int),
         ("fallback_tiles", ctypes.c_ulonglong), ("fused_redo_steps", ctypes.c_ulonglong),
+        ("exact_steps", ctypes.c_ulonglong), ("rollbacks", ctypes.c_int),
+        ("restarts", ctypes.c_int),
     ]
 
     def as_dict(self) -> dict:
@@ -235,6 +237,7 @@ class SolverConfig:
     lowbeta_ratio: float = 0.0
     fallback: int = 0  # MIXED: 0 FP64 fallback on, 1 off, 2 every tile (check mode)
     fuse: int = 0  # stages per launch: 1, 2 (one launch per step); 0 autotuned (tile -1) else 1
+    rollback_steps: int = 0  # MIXED: FP64 rollback window/snapshot interval; 0 = 256, < 0 off
 
 
 class Solver:
@@ -414,7 +417,7 @@ def mlp_train(params, x, y, w, epochs=20, batch=1024, lr=1e-3, seed=0):
 def _cfg(cfg: "SolverConfig", nx: int = 0, ny: int = 0) -> Cfg:
     return Cfg(nx or cfg.nx, ny or cfg.ny, cfg.cfl, cfg.alpha, int(cfg.muscl), int(cfg.hlld),
                cfg.precision, cfg.hlld_tol, cfg.tile, cfg.batch_steps, cfg.lowbeta_ratio,
-               cfg.fallback, cfg.fuse)
+               cfg.fallback, cfg.fuse, cfg.rollback_steps)
 
 
 class Ensemble:

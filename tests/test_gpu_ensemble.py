@@ -92,3 +92,66 @@ def test_ensemble_steps_limit_and_graph(mhd):
                         steps=20)
         assert np.array_equal(bits(ens.get_state(i)), bits(u_s))
     ens.close()
+
+
+# ---------------------------------------------------------------------------
+# The batched-ensemble workload itself: 64 seeded Orszag-Tang variants of
+# sizes 128^2-1024^2.
+# ---------------------------------------------------------------------------
+def workload_ensemble(mhd, form, n=64):
+    from mhdglm.workloads import ensemble_workload
+
+    sizes, params = ensemble_workload(n)
+    ens = mhd.Ensemble(mhd.SolverConfig(nx=2, ny=2, precision=mhd.PRECISIONS[form]),
+                       [(s, s) for s in sizes])
+    for i, p in enumerate(params):
+        ens.init_problem(i, "otv", p)
+    return ens, sizes, params
+
+
+def test_workload_fp64_instances_match_oracle(mhd, oracle):
+    """FP64 ensemble of the 64-instance workload, 12 steps: every 128^2
+    instance and the first 192^2 one equal the CPU oracle run on the oracle's
+    own initial state, bitwise."""
+    import pyoracle
+
+    steps = 12
+    say(f"64-instance workload, FP64, {steps} steps")
+    ens, sizes, params = workload_ensemble(mhd, "fp64")
+    sts = ens.advance(0.5, steps)
+    picks = [i for i, s in enumerate(sizes) if s == 128] + [sizes.index(192)]
+    got = {i: ens.get_state(i) for i in picks}
+    ens.close()
+    assert all(st.code == 0 and st.steps == steps for st in sts)
+    for i in picks:
+        n = sizes[i]
+        u0 = oracle.init("otv", n, n, params=params[i])
+        uo, info = oracle.advance(u0, steps)
+        say(f"  instance {i} ({n}^2) vs oracle")
+        assert info["steps"] == steps
+        assert np.array_equal(pyoracle.soa_to_aos(got[i]).view(np.uint64), uo.view(np.uint64)), i
+        assert (sts[i].degenerate, sts[i].nonphysical) == (int(info["counters"][2]),
+                                                          int(info["counters"][3]))
+
+
+def test_workload_mixed_finishes_every_instance(mhd):
+    """The mixed ensemble of the 64-instance workload to t = 0.5: all 64
+    instances finish (instance 16 through the FP64 rollback), and that
+    instance ends at the mixed form's distance from its FP64 run."""
+    from mhdglm import compare
+
+    say("64-instance workload, mixed, to t=0.5")
+    ens, sizes, params = workload_ensemble(mhd, "mixed")
+    sts = ens.advance(0.5)
+    rolled = [i for i, st in enumerate(sts) if st.rollbacks]
+    say(f"  done {sum(st.done for st in sts)}/64, rolled back {rolled}, "
+        f"restarts {sum(st.restarts for st in sts)}")
+    assert all(st.code == 0 and st.done == 1 for st in sts)
+    assert 16 in rolled and all(st.restarts == 0 for st in sts)
+    u16 = ens.get_state(16)
+    ens.close()
+    n = sizes[16]
+    u64, s64 = single(mhd, mhd.SolverConfig(nx=n, ny=n), n, n, "otv", params[16], 0.5)
+    errs = compare.rel_l2_state(u64, u16, ("rho", "p", "|B|"))
+    say(f"  instance 16 relL2 vs FP64 {errs}")
+    assert s64.code == 0 and max(errs.values()) < 1e-3

@@ -8,9 +8,13 @@ nothing and is re-evaluated by the FP64 kernel; only an FP64 failure halts.
   * a low-pressure, high-Mach contact on which the FP32 flux breaks
     positivity within a few steps: with the fallback the mixed form runs on,
     close to FP64;
-  * the limit: an Orszag-Tang variant whose mixed trajectory drifts (by
-    t ~ 0.42) into a state from which even the FP64 step fails halts with the
-    FP64 check's code, for a single grid and inside an ensemble alike.
+  * an Orszag-Tang variant whose mixed trajectory drifts (by t ~ 0.42) into
+    a state from which even the FP64 tile fallback fails: the rollback
+    re-runs an earlier window in FP64 and the run finishes, for a single grid
+    and inside an ensemble alike (without it the run halts, as the other
+    forms do);
+  * a state that the FP64 form cannot advance either: the rollback ends in a
+    restart that is the FP64 run, bitwise, halt included.
 """
 import sys
 
@@ -124,22 +128,42 @@ def test_fallback_rescues_fp32_flux_failure(mhd):
     assert errs["rho"] < 1e-6 and errs["p"] < 1e-2
 
 
-def test_fallback_limit_trajectory_divergence(mhd):
+def run_rb(mhd, n, precision, rollback_steps=0, fallback=0, t_end=0.5, steps=-1, u0=None):
+    s = mhd.Solver(mhd.SolverConfig(nx=n, ny=n, precision=precision, fallback=fallback,
+                                    rollback_steps=rollback_steps))
+    s.set_state(mhd.problem_state_ex("otv", OTV_HARD, n, n) if u0 is None else u0)
+    st = s.advance(t_end, steps)
+    u = s.get_state(0)
+    s.close()
+    return u, st
+
+
+def test_rollback_rescues_trajectory_divergence(mhd):
     """An Orszag-Tang variant (ensemble instance 16 of seed 2026) at 1024^2:
-    FP64 finishes; the mixed (and FP32) trajectory reaches a state at t ~ 0.42
-    from which even the FP64 step is not physical, so the fallback fires and
-    the FP64 check halts the run (code 3), as the FP64 form itself would from
-    that state.  An ensemble holding the instance halts identically."""
+    FP64 finishes; the mixed trajectory drifts and by t ~ 0.42 reaches a state
+    from which even the FP64 tile fallback is not physical.  Without the
+    rollback it halts with the FP64 check's code 3 (the reference's
+    contract); with it the run returns to a snapshot 256-512 steps earlier,
+    re-runs that window in FP64 and finishes mixed, at the mixed form's
+    distance from FP64.  Inside an ensemble the instance does the same,
+    bitwise."""
+    from mhdglm import compare
+
     n = 1024
     say("hard OT variant 1024^2")
-    u64, s64 = run(mhd, n, mhd.PREC_FP64, problem="otv", params=OTV_HARD)
+    u64, s64 = run_rb(mhd, n, mhd.PREC_FP64)
     assert s64.code == 0 and s64.done == 1
-    _, off = run(mhd, n, mhd.PREC_MIXED, fallback=1, problem="otv", params=OTV_HARD)
-    um, sm = run(mhd, n, mhd.PREC_MIXED, fallback=0, problem="otv", params=OTV_HARD)
-    say(f"  fallback off: code {off.code} step {off.fail_step}; on: code {sm.code} step "
-        f"{sm.fail_step}, fallback tiles {sm.fallback_tiles}")
-    assert off.code == 3 and sm.code == 3
-    assert sm.fallback_tiles >= 1 and sm.fail_step == off.fail_step
+    _, off = run_rb(mhd, n, mhd.PREC_MIXED, rollback_steps=-1)
+    say(f"  rollback off: code {off.code} step {off.fail_step} fallback tiles {off.fallback_tiles}")
+    assert off.code == 3 and off.fallback_tiles >= 1
+    um, sm = run_rb(mhd, n, mhd.PREC_MIXED)
+    errs = compare.rel_l2_state(u64, um, ("rho", "p", "|B|"))
+    say(f"  rollback on: code {sm.code} done {sm.done} steps {sm.steps} rollbacks {sm.rollbacks} "
+        f"restarts {sm.restarts} FP64 steps {sm.exact_steps} relL2 {errs}")
+    assert sm.code == 0 and sm.done == 1
+    assert sm.rollbacks >= 1 and sm.restarts == 0
+    assert 256 < sm.exact_steps <= sm.rollbacks * 3 * 256
+    assert max(errs.values()) < 1e-3
     cfg = mhd.SolverConfig(nx=2, ny=2, precision=mhd.PREC_MIXED)
     ens = mhd.Ensemble(cfg, [(128, 128), (n, n)])
     ens.init_problem(0, "ot")
@@ -147,7 +171,52 @@ def test_fallback_limit_trajectory_divergence(mhd):
     sts = ens.advance(0.5)
     ue = ens.get_state(1)
     ens.close()
-    assert sts[0].code == 0 and sts[0].done == 1
-    assert sts[1].code == 3 and sts[1].fail_step == sm.fail_step
-    assert sts[1].fallback_tiles == sm.fallback_tiles
+    assert sts[0].code == 0 and sts[0].done == 1 and sts[0].rollbacks == 0
+    assert sts[1].code == 0 and sts[1].done == 1
+    assert (sts[1].rollbacks, sts[1].exact_steps, sts[1].steps) == (sm.rollbacks, sm.exact_steps,
+                                                                    sm.steps)
     assert np.array_equal(ue.view(np.uint64), um.view(np.uint64))
+
+
+def marginal_state(mhd, oracle, n=32):
+    """Low pressure and a fast square: FP64 itself fails after some steps."""
+    u = oracle.init("ot", n, n)
+    W = oracle.cons_to_prim(u.reshape(-1, 9)).reshape(u.shape)
+    W[..., 4] *= 0.02
+    W[10:20, 10:20, 1] *= 8.0
+    u = oracle.prim_to_cons(W.reshape(-1, 9)).reshape(W.shape)
+    return np.ascontiguousarray(np.moveaxis(u, -1, 0))
+
+
+def test_rollback_restart_ends_as_fp64(mhd, oracle):
+    """A state the FP64 form itself cannot advance past some step: the mixed
+    run's rollback window fails in FP64 too, so the run restarts from the
+    given state in FP64 throughout, and halts exactly as the FP64 form does
+    (step, code, bad cell and state, bitwise)."""
+    u0 = marginal_state(mhd, oracle)
+    runs = {}
+    for name, prec in (("fp64", mhd.PREC_FP64), ("mixed", mhd.PREC_MIXED)):
+        s = mhd.Solver(mhd.SolverConfig(nx=32, ny=32, precision=prec, cfl=0.9, rollback_steps=16))
+        s.set_state(u0)
+        st = s.advance(0.5, 400)
+        runs[name] = (s.get_state(0), st)
+        s.close()
+        say(f"  {name}: code {st.code} steps {st.steps} fail {st.fail_step} bad ({st.bad_i},{st.bad_j}) "
+            f"rollbacks {st.rollbacks} restarts {st.restarts}")
+    (u64, s64), (um, sm) = runs["fp64"], runs["mixed"]
+    assert s64.code in (3, 4)
+    assert sm.restarts == 1
+    assert (sm.code, sm.fail_step, sm.bad_i, sm.bad_j, sm.steps) == (
+        s64.code, s64.fail_step, s64.bad_i, s64.bad_j, s64.steps)
+    assert sm.exact_steps == s64.steps
+    assert np.array_equal(um.view(np.uint64), u64.view(np.uint64))
+
+
+def test_init_problem_rejects_parameterised_problem(mhd):
+    """"otv" needs five parameters: mhd_init_problem refuses it instead of
+    uploading an empty state."""
+    s = mhd.Solver(mhd.SolverConfig(nx=32, ny=32))
+    with pytest.raises(mhd.MhdError) as ei:
+        s.init_problem("otv")
+    assert ei.value.code == 1 and "parameters" in str(ei.value)
+    s.close()

@@ -62,6 +62,7 @@ def run_cli(mhd, args, tmp_path, precision="fp64"):
     (["8", "8", "0.5", "0.25", "muscl", "roe"], "solver must be 'hll' or 'hlld'"),
     (["8", "8", "0.5", "0.25", "muscl", "hlld", "0"], "alpha must be positive"),
     (["8", "8", "0.5", "0.25", "muscl", "hlld", "0.18", "sod"], "problem must be"),
+    (["8", "8", "0.5", "0.25", "muscl", "hlld", "0.18", "otv"], "problem must be"),
     (["1", "8"], "nx, ny >= 2"),
 ])
 def test_cli_usage_errors(mhd, tmp_path, args, msg):
@@ -98,6 +99,21 @@ def test_otv_problem_closed_form(mhd):
     assert np.allclose(u[6], amp * np.sin(4 * np.pi * X + p4) * c, atol=1e-14)
     with pytest.raises(mhd.MhdError):
         mhd.problem_state_ex("otv", (1.0, 0.0), n, n)
+
+
+@pytest.mark.parametrize("dtype,prec", [(np.float64, 0), (np.float32, 1)])
+def test_otv_initial_state_matches_oracle(mhd, oracle, dtype, prec):
+    """The oracle's own otv (closed form pinned by the numpy test above and
+    by its reduction to ot) equals the product host's bit for bit."""
+    params = (1.067437115364036, 0.8064116573994691, 1.759240013263218, 0.20557563316160354,
+              0.5272182398647961)
+    nx, ny = 40, 36
+    u = mhd.problem_state_ex("otv", params, nx, ny, prec)
+    ref = np.moveaxis(oracle.init("otv", nx, ny, dtype=dtype, params=params), -1, 0)
+    assert np.array_equal(u.view(UINT[dtype]), np.ascontiguousarray(ref).view(UINT[dtype]))
+    ot = oracle.init("ot", nx, ny, dtype=dtype)
+    one = oracle.init("otv", nx, ny, dtype=dtype, params=(1.0, 0.0, 0.0, 0.0, 0.0))
+    assert np.array_equal(ot.view(UINT[dtype]), one.view(UINT[dtype]))
 
 
 def test_autotune_table_lookup(tmp_path):
