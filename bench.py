@@ -13,35 +13,40 @@ the reference's "Mcell-steps/s" unit (mhd2d_glm_cpu.cpp:589-591) times 1e6.
 
 `value` is the plain FP64 path (bitwise the reference CPU build).  Also
 reported on the same JSON line:
-  forms        -- throughput of the other arithmetic forms on the same
+  roofline     -- the step against the HBM roofline as the metric defines
+                  it: 144 B per FP64 cell-update (read U^n, write U^{n+1})
+                  times the cell-updates one step performs, over the step's
+                  kernel time measured here with HIP events on the solver
+                  stream (stage 1 + stage 2 + finish), against 8 TB/s; the
+                  same figure per stage kernel (its own compulsory bytes);
+                  the rocprofv3-measured HBM traffic per step (committed
+                  counter runs) and its ratio to the 144 B figure;
+  gate         -- the headline's second half on its own configuration:
+                  whole Orszag-Tang 4096^2 runs to t=0.5 in FP64, in the
+                  mixed form with the learned tile selector, and in FP64 with
+                  FMA contraction (the noise floor: two valid FP64 builds):
+                  wall time, cell-updates/s, speedup over FP64, relative L2
+                  against FP64 of rho, p, |B| and psi, and gate_met (rho, p
+                  and |B| all <= 1e-5);
+  forms        -- throughput of the other arithmetic forms on the timed
                   workload (FP64 with FMA, mixed, mixed with the learned
                   tile selector, mixed + compensated, FP32, and the
                   reference's stage-level mixed form), the mixed-vs-FP64
                   speedup and the selector's FP64 tile share;
-  accuracy     -- relative L2 against FP64 of rho, p, |B| and psi after whole
-                  Orszag-Tang runs to t=0.5 at --accuracy-n (default 1024^2),
-                  for each form, beside the FP64 noise floor (FMA vs no FMA);
-  kh           -- Kelvin-Helmholtz (the 1024^2 benchmark variant) to t=5,
-                  FP64 vs FP64-FMA (the noise floor) and mixed: steps, rates,
-                  relative L2;
-  selector_run -- Orszag-Tang 2048^2 to t=0.5, FP64 vs FP64-FMA, mixed, and
-                  mixed with the learned tile selector: rates, FP64 tile
-                  share, audits, relative L2;
-  selector_breakdown -- (--selector-breakdown N) whole runs at N^2 splitting
-                  the selector's cost per step over the mixed step into its
-                  own launches, the audits and the FP64 tiles;
   ensemble     -- the batched workload: 64 Orszag-Tang instances (seeded
                   sizes in {128..1024}^2, B amplitude and sine phases varied)
                   advanced together to t=0.5 by the persistent work-queue
-                  kernels, each with its own dt; aggregate cell-updates/s;
-  roofline     -- the dominant (stage) kernel's algorithmic HBM bytes per
-                  launch over its average duration measured here with HIP
-                  events on the solver stream, against 8 TB/s (with --fuse 2
-                  the two-stage kernel, 144 B per cell in FP64 state);
+                  kernels, each with its own dt; aggregate cell-updates/s,
+                  instances finished and FP64 rollbacks;
   large        -- FP64 and mixed steps at 8192^2 (4.8 GB per FP64 state);
   cpu_baseline -- the reference CPU build (oracle/_ref, serial, one core) on
                   a bounded sample of Orszag-Tang at 512^2 (the first ~50
                   steps).
+Optional sections (off by default; their results are kept in profiles/):
+  --accuracy-n N        whole runs of every form at N^2 against FP64;
+  --kh-n N              Kelvin-Helmholtz (the 1024^2 variant) to t=5;
+  --selector-n N        Orszag-Tang N^2 whole runs, FP64 vs selector;
+  --selector-breakdown N  the selector's cost per step, by part.
 """
 import argparse
 import json
@@ -57,9 +62,13 @@ sys.path.insert(0, os.path.join(ROOT, "precision-study_amd"))
 
 HBM_PEAK_GBS = 8000.0
 METRIC = "cell-updates/s, 2D MHD-GLM HLLD 4096²; mixed vs FP64 speedup at L2≤1e-5"
-# Compulsory HBM bytes per cell of each stage kernel (9 variables):
+# Algorithmic HBM bytes per cell-update (9 variables): read U^n, write
+# U^{n+1}; every intermediate stays on chip in an ideally fused step.
+STEP_BYTES_PER_CELL = 2 * 9
+# Compulsory HBM bytes per cell of each stage kernel of the two-launch step:
 #   stage 1 reads U, writes U1;  stage 2 reads U1 and U, writes U.
 STAGE_BYTES_PER_CELL = {1: 2 * 9, 2: 3 * 9}
+GATE_L2 = 1e-5
 
 
 def log(msg):
@@ -199,6 +208,42 @@ def accuracy_check(mhdglm, n, forms, problem="ot", t_end=0.5):
     return out
 
 
+def gate_check(mhdglm, n, with_floor=True, t_end=0.5):
+    """The metric's accuracy-gated speedup on its own configuration: whole
+    Orszag-Tang n^2 runs to t_end in FP64, in the mixed form with the learned
+    tile selector and (with_floor) in FP64 with FMA contraction, whose
+    distance from FP64 is the noise floor of this chaotic flow."""
+    from mhdglm import compare
+
+    fields = ("rho", "p", "|B|", "psi")
+    u64, s64, w64, _ = whole_run(mhdglm, "fp64", n)
+    out = {"problem": "ot", "n": n, "t_end": t_end, "gate_relL2": GATE_L2,
+           "fp64": {"steps": s64.steps, "code": s64.code, "wall_s": round(w64, 2),
+                    "cell_updates_per_s": round(n * n * s64.steps / w64, 1)}}
+    forms = ["selector"] + (["fp64_fma"] if with_floor else [])
+    for form in forms:
+        u, st, wall, sel = whole_run(mhdglm, form, n)
+        errs = compare.rel_l2_state(u64, u, fields)
+        row = {"steps": st.steps, "code": st.code, "done": st.done, "wall_s": round(wall, 2),
+               "cell_updates_per_s": round(n * n * st.steps / wall, 1),
+               "speedup_vs_fp64": round(w64 / wall * st.steps / s64.steps, 3),
+               "relL2_vs_fp64": {k: float(f"{v:.3e}") for k, v in errs.items()},
+               "rollbacks": st.rollbacks}
+        if sel:
+            row["selector"] = sel
+        out[form] = row
+    sel = out["selector"]
+    out["speedup"] = sel["speedup_vs_fp64"]
+    out["gate_met"] = bool(sel["code"] == 0 and
+                           all(sel["relL2_vs_fp64"][k] <= GATE_L2 for k in ("rho", "p", "|B|")))
+    if with_floor:
+        floor = out["fp64_fma"]["relL2_vs_fp64"]
+        out["selector_over_floor"] = {k: round(sel["relL2_vs_fp64"][k] / floor[k], 3)
+                                      for k in ("rho", "p", "|B|") if floor[k] > 0}
+        out["floor_below_gate"] = all(floor[k] <= GATE_L2 for k in ("rho", "p", "|B|"))
+    return out
+
+
 def selector_breakdown(mhdglm, n, t_end=0.5):
     """Whole Orszag-Tang runs at n^2 to t_end in the mixed form, and in the
     mixed form with the selector as shipped, with audits off, with nothing
@@ -261,6 +306,9 @@ def run_ensemble(mhdglm, torch, n, form, t_end):
         "value": round(updates / wall, 1), "unit": "cell-updates/s",
         "all_done": all(st.done == 1 for st in sts), "codes": sorted(set(codes)),
         "instances_done": sum(1 for st in sts if st.done == 1),
+        "rollbacks": [{"instance": i, "size": sizes[i], "windows": st.rollbacks,
+                       "fp64_steps": st.exact_steps, "restarted": st.restarts}
+                      for i, st in enumerate(sts) if st.rollbacks or st.restarts],
         "instances_halted": [{"instance": i, "size": sizes[i], "code": st.code, "t": round(st.t, 4)}
                              for i, st in enumerate(sts) if st.code != 0],
         "steps_min_max": [min(st.steps for st in sts), max(st.steps for st in sts)],
@@ -286,16 +334,20 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--forms", default="fp64_fma,mixed,selector,mixed_kahan,fp32,ref_mixed",
                     help="other forms timed beside --precision ('' for none)")
-    ap.add_argument("--accuracy-n", type=int, default=1024,
-                    help="grid of the whole-run accuracy check (0 to skip)")
-    ap.add_argument("--kh-n", type=int, default=1024,
+    ap.add_argument("--gate-n", type=int, default=4096,
+                    help="grid of the headline's whole runs to t=0.5, FP64 vs selector (0 to skip)")
+    ap.add_argument("--no-gate-floor", action="store_true",
+                    help="skip the FP64-FMA noise-floor run of the gate")
+    ap.add_argument("--accuracy-n", type=int, default=0,
+                    help="grid of the whole-run accuracy check of every form (0 to skip)")
+    ap.add_argument("--kh-n", type=int, default=0,
                     help="Kelvin-Helmholtz (khb) whole runs, FP64 vs mixed (0 to skip)")
     ap.add_argument("--kh-t-end", type=float, default=5.0)
     ap.add_argument("--kh-forms", default="fp64_fma,mixed",
                     help="forms of the Kelvin-Helmholtz whole runs, against FP64")
     ap.add_argument("--selector-breakdown", type=int, default=0,
                     help="whole runs at this size splitting the selector's cost (0 to skip)")
-    ap.add_argument("--selector-n", type=int, default=2048,
+    ap.add_argument("--selector-n", type=int, default=0,
                     help="Orszag-Tang whole runs to t=0.5, FP64 vs selector (0 to skip)")
     ap.add_argument("--ensemble", type=int, default=64,
                     help="instances of the batched-ensemble workload (0 to skip)")
@@ -333,39 +385,56 @@ def main():
     value, wall, ms = res["value"], res["wall"], res["kernel_ms"]
     esize = 4 if args.precision == "fp32" else 8
     fused = args.fuse == 2
-    if fused:
-        # One two-stage kernel per step: U^n in, U^{n+1} out.
-        stage, per_cell, kname = 1, 18, "k_step_fused"
-        tfile = os.path.join(ROOT, "profiles", f"traffic_step_fused_ot{n}_{args.precision}.json")
-    else:
-        stage = 1 if ms[0] >= ms[1] else 2
-        per_cell, kname = STAGE_BYTES_PER_CELL[stage], f"k_stage<{stage}>"
-        tfile = os.path.join(ROOT, "profiles", f"traffic_stage_ot{n}_{args.precision}.json")
     stages_per_launch = args.fuse or (
         mhdglm.load().mhd_autotuned_fuse(mhdglm.PRECISIONS[args.precision], n, n)
         if args.tile < 0 else 1)
-    alg_bytes = per_cell * esize * n * n
-    achieved = alg_bytes / (ms[stage - 1] * 1e-3) / 1e9
+    # The step against the HBM roofline: 144 B per FP64 cell-update (72 B in
+    # FP32 state) over the step's kernel time (HIP events on the solver
+    # stream around the stage kernels and the finish kernel).
+    step_ms = ms[0] + ms[1] + ms[2]
+    alg_bytes = STEP_BYTES_PER_CELL * esize * n * n
+    achieved = alg_bytes / (step_ms * 1e-3) / 1e9
+    tname = "step_fused" if fused else "stage"
+    tfile = os.path.join(ROOT, "profiles", f"traffic_{tname}_ot{n}_{args.precision}.json")
     traffic = None
     valu = None
+    kernels = {}
+    prof = None
     if os.path.exists(tfile):
         with open(tfile) as f:
             prof = json.load(f)
-        traffic = prof.get(f"stage{stage}_bytes_per_launch")
-        v = prof.get(f"stage{stage}_valu")
+    if fused:
+        kernels["k_step_fused"] = {"ms": round(ms[0], 4), "compulsory_B_per_cell": 18 * esize}
+    else:
+        for stage in (1, 2):
+            per = STAGE_BYTES_PER_CELL[stage] * esize
+            kernels[f"k_stage<{stage}>"] = {
+                "ms": round(ms[stage - 1], 4), "compulsory_B_per_cell": per,
+                "achieved_GBs": round(per * n * n / (ms[stage - 1] * 1e-3) / 1e9, 1)}
+    kernels["k_finish"] = {"ms": round(ms[2], 4)}
+    if prof:
+        b = [prof.get(f"stage{k}_bytes_per_launch") for k in ((1,) if fused else (1, 2))]
+        if all(x is not None for x in b):
+            traffic = round(sum(b))
+        v = prof.get("stage1_valu")
         if v:
-            # The stage kernels are bound by VALU issue, not HBM: wave64 VALU
-            # instructions per CU per cycle from rocprofv3 counters, against
-            # the FP64 issue limit of 1 (transcendentals counted 4x).
-            valu = {"insts_per_launch": round(v["insts"]), "issue_per_cu_cycle": round(v["issue_per_cu_cycle"], 3),
+            # wave64 VALU instructions per CU per cycle (rocprofv3 counters),
+            # against the FP64 issue limit of 1 (transcendentals counted 4x).
+            valu = {"insts_per_launch": round(v["insts"]),
+                    "issue_per_cu_cycle": round(v["issue_per_cu_cycle"], 3),
                     "issue_incl_trans": round(v["issue_incl_trans_quarter_rate"], 3),
-                    "fp64_issue_limit": 1.0, "source": os.path.relpath(tfile, ROOT)}
+                    "fp64_issue_limit": 1.0}
     roofline = {
         "bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-        "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic, "valu": valu,
-        "kernel": f"{kname} ({per_cell * esize} B/cell compulsory)",
-        "kernel_ms": {"stage1": round(ms[0], 4), "stage2": round(ms[1], 4),
-                      "finish": round(ms[2], 4)},
+        "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+        "unit_of_work": f"one SSPRK2 step of {n}x{n} cells, {STEP_BYTES_PER_CELL * esize} B "
+                        "per cell-update (read U^n, write U^{n+1})",
+        "step_kernel_ms": round(step_ms, 4),
+        "frac_of_value": round(value * STEP_BYTES_PER_CELL * esize / 1e9 / HBM_PEAK_GBS, 4),
+        "traffic_over_algorithmic": round(traffic / alg_bytes, 3) if traffic else None,
+        "traffic_source": os.path.relpath(tfile, ROOT) if traffic else None,
+        "kernels": kernels, "valu": valu,
+        "limiter": "FP64 VALU work and its latency at 2 waves per SIMD, not HBM bandwidth",
     }
     forms = {args.precision: round(value, 1)}
     for form in [f for f in args.forms.split(",") if f and f != args.precision]:
@@ -380,6 +449,9 @@ def main():
         forms["mixed_vs_fp64_fma_speedup"] = round(forms["mixed"] / forms["fp64_fma"], 3)
     if "ref_mixed" in forms and args.precision == "fp64":
         forms["ref_mixed_vs_fp64_speedup"] = round(forms["ref_mixed"] / value, 3)
+    gate = None
+    if args.gate_n:
+        gate = gate_check(mhdglm, args.gate_n, with_floor=not args.no_gate_floor)
     accuracy = None
     if args.accuracy_n:
         accuracy = accuracy_check(mhdglm, args.accuracy_n,
@@ -419,6 +491,7 @@ def main():
         "dtype": {"fp64": "f64", "fp32": "f32", "fp64_fma": "f64",
                   "mixed": "f32 flux / f64 state+update", "mixed_kahan": "f32 flux / f32x2 state",
                   "ref_mixed": "f32 recovery+recon+rhs / f64 flux+state"}[args.precision],
+        "gate": gate,
         "forms": forms,
         "accuracy": accuracy,
         "large": large,

@@ -88,3 +88,46 @@ def test_tile_errors_closed_form():
     # the GLM potential does not enter
     a[8, 5, 5] += 3.0
     assert np.array_equal(so.tile_errors(a, b, 4, 4), e)
+
+
+def test_features_psi_divb_fast_speed_closed_forms(oracle):
+    """Features 6 (|psi|/|B|), 7 (|div B| h/|B|) and 10 (c_f/c_h) on a
+    single-mode field Bx = b0 + a sin(2 pi x), By = c, with a constant psi:
+    the central difference of a sine is cos(2 pi x) sin(2 pi dx)/dx, and c_f
+    of a field perpendicular to x is the reference's fast speed
+    (mhdFastSpeedX, through the pinned physics oracle)."""
+    nx, ny = 48, 30
+    rho, p, psi0, b0, amp, c = 1.1, 0.6, 0.02, 0.3, 0.2, 0.25
+    x = (np.arange(nx) + 0.5) / nx
+    X = np.broadcast_to(x, (ny, nx))
+    bx = b0 + amp * np.sin(2 * np.pi * X)
+    u = np.zeros((9, ny, nx))
+    u[0] = rho
+    u[5], u[6] = bx, c
+    u[4] = p / (so.GAMMA - 1) + 0.5 * (bx * bx + c * c)
+    u[8] = psi0
+    ch = 1.7
+    tx, ty = 16, 15
+    f = so.tile_features(u, tx, ty, ch=ch)
+    bmag = np.sqrt(bx * bx + c * c)
+    divb = amp * np.cos(2 * np.pi * X) * np.sin(2 * np.pi / nx) * nx   # central difference
+    divb_n = np.abs(divb) * (1.0 / nx) / bmag
+    tiles_x = nx // tx
+    for t in range(f.shape[0]):
+        j0, i0 = (t // tiles_x) * ty, (t % tiles_x) * tx
+        sl = (slice(j0, j0 + ty), slice(i0, i0 + tx))
+        assert f[t, 6] == pytest.approx(np.log10(psi0 / (bmag[sl].max() + 1e-12) + 1e-12), abs=1e-12)
+        assert f[t, 7] == pytest.approx(np.log10(divb_n[sl].max() + 1e-12), abs=1e-9)
+    # c_f/c_h with Bx = 0: the x fast speed of the reference's formula
+    u2 = u.copy()
+    u2[5] = 0.0
+    u2[6] = bmag
+    u2[4] = p / (so.GAMMA - 1) + 0.5 * bmag * bmag
+    f2 = so.tile_features(u2, tx, ty, ch=ch)
+    W = np.zeros((nx * ny, 9))
+    W[:, 0], W[:, 4], W[:, 6] = rho, p, bmag.reshape(-1)
+    cf = oracle.fast_speed(W, direction=0).reshape(ny, nx)
+    for t in range(f2.shape[0]):
+        j0, i0 = (t // tiles_x) * ty, (t % tiles_x) * tx
+        sl = (slice(j0, j0 + ty), slice(i0, i0 + tx))
+        assert f2[t, 10] == pytest.approx(cf[sl].max() / ch, rel=1e-12)
