@@ -142,12 +142,14 @@ extern "C" void orc_refmixed_recon_batch(int n, const float* Pm, const float* P0
   }
 }
 
-// The reference's stage-level mixed build (mhd_refmixed.hpp), MUSCL: the
-// same arguments as orc_advance_f64.
-extern "C" int orc_refmixed_advance(int nx, int ny, double* u, int max_steps, double t_end,
-                                    double cfl, double alpha, int hlld, double tol,
-                                    double* t_io, int* steps_out, uint64_t* counters4,
-                                    int* bad_ij, double* last_ch) {
+// The reference's stage-level mixed build (mhd_refmixed.hpp), MUSCL, with
+// the float regions of `mask` (refmixed::Region; kAllRegions: the mixed
+// build, 0: the FP64 step): the same arguments as orc_advance_f64.
+extern "C" int orc_refmixed_advance_regions(int nx, int ny, double* u, int max_steps,
+                                            double t_end, double cfl, double alpha, int hlld,
+                                            double tol, double* t_io, int* steps_out,
+                                            uint64_t* counters4, int* bad_ij, double* last_ch,
+                                            int mask) {
   const Consts<double> c(tol);
   Grid<double> U(nx, ny), U1(nx, ny);
   for (size_t q = 0; q < U.u.size(); q++) U.u[q] = u[q];
@@ -157,7 +159,7 @@ extern "C" int orc_refmixed_advance(int nx, int ny, double* u, int max_steps, do
   int steps = 0, code = 0;
   while (t < t_end && steps < max_steps) {
     StepResult r = refmixed::ssprk2_step(c, U, U1, dx, dy, cfl, alpha, hlld != 0, t, t_end, &st,
-                                         last_ch);
+                                         last_ch, mask);
     if (r.code != 0) {
       code = r.code;
       if (bad_ij) { bad_ij[0] = r.bad_i; bad_ij[1] = r.bad_j; }
@@ -173,4 +175,13 @@ extern "C" int orc_refmixed_advance(int nx, int ny, double* u, int max_steps, do
     counters4[2] = st.degenerate; counters4[3] = st.nonphysical;
   }
   return code;
+}
+
+extern "C" int orc_refmixed_advance(int nx, int ny, double* u, int max_steps, double t_end,
+                                    double cfl, double alpha, int hlld, double tol,
+                                    double* t_io, int* steps_out, uint64_t* counters4,
+                                    int* bad_ij, double* last_ch) {
+  return orc_refmixed_advance_regions(nx, ny, u, max_steps, t_end, cfl, alpha, hlld, tol, t_io,
+                                      steps_out, counters4, bad_ij, last_ch,
+                                      refmixed::kAllRegions);
 }

@@ -164,10 +164,15 @@ class Oracle:
         f(n, _p(a[0]), _p(out))
         return out
 
+    REGIONS = {"recovery": 1, "cfl": 2, "recon": 4, "rhs": 8, "glm": 16}
+
     def refmixed_advance(self, u_aos, max_steps, t_end=0.5, cfl=0.25, alpha=0.18, hlld=True,
-                         tol=1e-4, t0=0.0):
+                         tol=1e-4, t0=0.0, regions=None):
         """The reference's stage-level mixed build (MUSCL), FP64 state:
-        advances a copy of u_aos; returns (u, info dict) as advance()."""
+        advances a copy of u_aos; returns (u, info dict) as advance().
+        `regions` (names of REGIONS) puts only those regions in float; None:
+        all five (the mixed build), (): none (the FP64 step)."""
+        mask = 31 if regions is None else sum(self.REGIONS[r] for r in regions)
         ny, nx, _ = u_aos.shape
         a = np.array(u_aos, dtype=np.float64, copy=True, order="C")
         t = np.array([t0], dtype=np.float64)
@@ -175,12 +180,12 @@ class Oracle:
         cnt = np.zeros(4, dtype=np.uint64)
         bad = np.full(2, -1, dtype=np.int32)
         ch = np.zeros(1, dtype=np.float64)
-        f = self.lib.orc_refmixed_advance
+        f = self.lib.orc_refmixed_advance_regions
         D, I, P = ctypes.c_double, ctypes.c_int, ctypes.c_void_p
-        f.argtypes = [I, I, P, I, D, D, D, I, D, P, P, P, P, P]
+        f.argtypes = [I, I, P, I, D, D, D, I, D, P, P, P, P, P, I]
         f.restype = I
         code = f(nx, ny, _p(a), max_steps, t_end, cfl, alpha, int(hlld), tol, _p(t), _p(steps),
-                 _p(cnt), _p(bad), _p(ch))
+                 _p(cnt), _p(bad), _p(ch), mask)
         return a, {"code": code, "t": t[0], "steps": int(steps[0]), "counters": cnt,
                    "bad": (int(bad[0]), int(bad[1])), "ch": ch[0]}
 

@@ -9,7 +9,8 @@ logs (text tables only; nothing of the reference is imported or run):
       the native FP32 build (ot2d_cmp_nativefp32.txt) and the FP64 build with
       recovery, CFL, reconstruction, rhs and GLM regions truncated to FP32
       mantissas (ot2d_cmp_c_recovery+cfl+recon+rhs+glm.txt, the "keepFU"
-      row of ot2d_table.txt).
+      row of ot2d_table.txt), and the same build with one region at a time
+      truncated (ot2d_cmp_{recovery,cfl,recon,rhs,glm}.txt).
 
     python tests/golden/make_cmp_fixtures.py [/root/reference]
 """
@@ -74,7 +75,14 @@ def main(ref="/root/reference"):
                    "what": "FP64 build with the recovery, cfl, recon, rhs and glm regions "
                            "truncated to FP32 mantissas vs FP64 (the flux and update kept)",
                    "rows": keep, "divB": keep_divb},
+        "regions": {},
     }
+    for reg in ("recovery", "cfl", "recon", "rhs", "glm"):
+        rows, divb = parse_cmp(os.path.join(scan, f"ot2d_cmp_{reg}.txt"))
+        out["regions"][reg] = {
+            "source": f"logs/mhd/raptor_scan/ot2d_cmp_{reg}.txt",
+            "what": f"FP64 build with only the {reg} region truncated to FP32 mantissas vs FP64",
+            "rows": rows, "divB": divb}
     with open(os.path.join(HERE, "cmp_ot192.json"), "w") as f:
         json.dump(out, f, indent=1)
 

@@ -23,6 +23,7 @@ import states
 
 pytestmark = pytest.mark.gpu
 
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 UINT = {np.float64: np.uint64, np.float32: np.uint32}
 PREC = {"fp64": (0, np.float64), "fp32": (1, np.float32)}
 
@@ -296,3 +297,34 @@ def test_full_size_conservation_and_determinism(mhd):
         tot0, tot1 = u0[k].sum(dtype=np.float64), u[k].sum(dtype=np.float64)
         scale = np.abs(u0[k]).sum(dtype=np.float64)
         assert abs(tot1 - tot0) <= 1e-12 * scale, (k, tot0, tot1)
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+def test_cli_dat_header_matches_reference_cpu_build(mhd, tmp_path, prec):
+    """The .dat header of the drop-in CLI: the reference GPU build writes the
+    CPU build's four '#' lines with ' (GPU)' appended to the first
+    (mhd2d_glm_gpu.cu:610-625 against mhd2d_glm_cpu.cpp:628-645).  The CPU
+    side is the oracle's CLI, whose .dat equals the reference CPU binary's
+    byte for byte; the run is bitwise, so t, ch_last and the counters in the
+    header agree too."""
+    import subprocess
+
+    args = ["64", "48", "0.05", "0.25", "muscl", "hlld", "0.18"]
+    gdir, odir = tmp_path / "gpu", tmp_path / "cpu"
+    gdir.mkdir()
+    odir.mkdir()
+    say(f"cli header {prec}")
+    r = mhd.run_cli(args, str(gdir), precision=prec)
+    assert r.returncode == 0, r.stderr
+    exe = os.path.join(ROOT, "oracle", "build", f"mhd2d_glm_oracle_{prec}")
+    ro = subprocess.run([exe] + args, env=dict(os.environ, OUTDIR=str(odir)), capture_output=True,
+                        text=True, timeout=300)
+    assert ro.returncode == 0, ro.stderr
+    (g,) = list(gdir.iterdir())
+    (o,) = list(odir.iterdir())
+    assert g.name == o.name
+    gh = [ln for ln in open(g) if ln.startswith("#")]
+    oh = [ln for ln in open(o) if ln.startswith("#")]
+    assert len(gh) == len(oh) == 4
+    assert gh[0] == oh[0].rstrip("\n") + " (GPU)\n"
+    assert gh[1:] == oh[1:]

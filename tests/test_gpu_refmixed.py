@@ -182,11 +182,11 @@ def test_ot512_distance_from_fp64_as_reference_logged(mhd):
 
 
 def test_ot192_distance_from_fp64_as_truncation_scan(mhd):
-    """192^2 Orszag-Tang to t=0.5: the form against FP64, per field, within a
-    factor 5 of the reference's scan that truncates the same regions
-    (recovery, CFL, reconstruction, rhs, GLM source) to FP32 mantissas while
-    the flux and update stay FP64 (ot2d_table.txt row keepFU); and far below
-    the native FP32 build's distance at the same size."""
+    """192^2 Orszag-Tang to t=0.5: the form against FP64, per field, within
+    25% of the reference's scan that truncates the same regions (recovery,
+    CFL, reconstruction, rhs, GLM source) to FP32 mantissas while the flux
+    and update stay FP64 (ot2d_table.txt row keepFU); and far below the
+    native FP32 build's distance at the same size."""
     g = json.load(open(os.path.join(HERE, "golden", "cmp_ot192.json")))
     n = g["n"]
     say("ref_mixed vs fp64 OT 192^2 to t=0.5")
@@ -205,7 +205,7 @@ def test_ot192_distance_from_fp64_as_truncation_scan(mhd):
     for k in names:
         keep, native = g["keepFU"]["rows"][k]["relL2"], g["native_fp32"]["rows"][k]["relL2"]
         say(f"  {k}: relL2 {errs[k]:.3e} (truncation scan {keep:.3e}, native FP32 {native:.3e})")
-        assert keep / 5 <= errs[k] <= 5 * keep, k
+        assert keep / 1.25 <= errs[k] <= 1.25 * keep, k
         assert errs[k] < native / 10, k
 
 

@@ -10,7 +10,11 @@ mixed build (oracle/mhd_refmixed.hpp, after src/mhd/mhd2d_glm_gpu_mixed.cu).
     NaN);
   * a whole run: the form lies between FP64 and FP32 (its FP32 stages move
     it off FP64 by far less than the all-FP32 build), with the same steps
-    and HLLD fallback counts as FP64 on a short run.
+    and HLLD fallback counts as FP64 on a short run;
+  * regions: with no region in float it is the FP64 oracle bit for bit, and
+    with only the rhs (or only the cfl) region in float its whole-run
+    distance from FP64 at 192^2 is the reference's one-region truncation
+    scan's, per field, to 2%.
 """
 import os
 
@@ -87,3 +91,57 @@ def test_whole_run_between_fp64_and_fp32(oracle):
     assert 0 < e_rm < 0.3 * e_32, (e_rm, e_32)
     assert e_rm < 1e-6
     assert irm["t"] == pytest.approx(i64["t"], rel=1e-6)
+
+
+def test_region_mask_none_is_fp64_oracle(oracle):
+    """With no region in float the restatement is the FP64 oracle's step,
+    bit for bit (state, time and HLLD counters); each single region moves
+    the result."""
+    u = oracle.init("ot", 48, 40)
+    d, idd = oracle.advance(u, 30)
+    c, ic = oracle.refmixed_advance(u, 30, regions=())
+    assert np.array_equal(c.view(np.uint64), d.view(np.uint64))
+    assert ic["t"] == idd["t"] and list(ic["counters"]) == list(idd["counters"])
+    a, _ = oracle.refmixed_advance(u, 30)
+    for r in oracle.REGIONS:
+        e, _ = oracle.refmixed_advance(u, 30, regions=(r,))
+        assert not np.array_equal(e, d), r
+        assert not np.array_equal(e, a), r
+
+
+@pytest.fixture(scope="module")
+def scan192(oracle):
+    """The reference's 192^2 comparison tables and the FP64 whole run."""
+    import json
+
+    g = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "cmp_ot192.json")))
+    u = oracle.init("ot", g["n"], g["n"])
+    ref, iref = oracle.refmixed_advance(u, 100000, t_end=g["t_end"], regions=())
+    assert iref["steps"] == g["steps_fp64"]
+    return g, u, ref
+
+
+@pytest.mark.parametrize("region", ["rhs", "cfl"])
+def test_region_runs_match_reference_scan_192(oracle, scan192, region):
+    """Orszag-Tang 192^2 to t=0.5 with one region in float, against FP64:
+    the relative L2 per field equals the reference's one-region truncation
+    scan (logs/mhd/raptor_scan/ot2d_cmp_{rhs,cfl}.txt) to within 2%.  These
+    regions take FP64 inputs (the face fluxes; the recovered primitives), so
+    casting them and truncating each operation agree; the rhs region is the
+    FP32 narrowing of the face fluxes before they are differenced
+    (mhd2d_glm_gpu_mixed.cu:355-375).  The recovery and recon regions cast
+    the state (f32LoadPrim) where the scan truncates operations on FP64
+    inputs, and differ from it by factors of 0.3-1.9 and 3-9; glm by 0.2;
+    those regions are pinned bitwise against the reference's FP32 header
+    functions instead."""
+    g, u, ref = scan192
+    a, ia = oracle.refmixed_advance(u, 100000, t_end=g["t_end"], regions=(region,))
+    assert ia["code"] == 0 and ia["steps"] == g["steps_fp64"]
+    for k, var in (("rho", 0), ("vx", 1), ("vy", 2), ("E", 4), ("Bx", 5), ("By", 6)):
+        if var in (1, 2):
+            x, y = a[..., var] / a[..., 0], ref[..., var] / ref[..., 0]
+        else:
+            x, y = a[..., var], ref[..., var]
+        rel = np.sqrt(np.mean((x - y) ** 2)) / np.sqrt(np.mean(y ** 2))
+        logged = g["regions"][region]["rows"][k]["relL2"]
+        assert rel == pytest.approx(logged, rel=0.02), (region, k, rel, logged)
