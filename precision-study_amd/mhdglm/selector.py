@@ -8,7 +8,7 @@ exceeds the budget.  The MLP is trained on the device (mhd_mlp_train); the
 model (weights, feature standardisation, threshold, budget) is kept as an
 .npz file.
 
-    python -m mhdglm.selector --out model.npz [--sizes 256,512,1024,2048] [--early-n 4096]
+    python -m mhdglm.selector --out model.npz [--n 512] [--snapshots 190]
 """
 from __future__ import annotations
 
@@ -27,8 +27,8 @@ H = 64
 # difference this many times the label budget.  Misses at the budget itself
 # are round-off variation (every tile exceeds the budget now and then).
 AUDIT_FACTOR = 10.0
-# Model trained by `python -m mhdglm.selector` (Orszag-Tang at 256^2 to
-# 2048^2 and early 4096^2, two variants), shipped with the package.
+# Model trained by `python -m mhdglm.selector` (Orszag-Tang, two variants
+# and both Kelvin-Helmholtz problems at 512^2), shipped with the package.
 DEFAULT_MODEL = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))),
                              "selector_model.npz")
 
@@ -72,14 +72,16 @@ def collect(problem="ot", params=(), n=512, steps=600, every=4, t_end=None, snap
 
 
 def fit(X, E, budget=None, unsafe_fraction=0.05, epochs=30, batch=2048, lr=2e-3, seed=0,
-        log=print):
+        labels=None, log=print):
     """Trains the MLP on (features, pair errors).  budget defaults to the
-    (1 - unsafe_fraction) quantile of the errors.  Returns the model dict."""
+    (1 - unsafe_fraction) quantile of the errors; `labels` (0/1 per tile)
+    replace E > budget when given (budget then only sets the audits').
+    Returns the model dict."""
     X = np.asarray(X, dtype=np.float32)
     E = np.asarray(E, dtype=np.float64)
     if budget is None:
         budget = float(np.quantile(E, 1.0 - unsafe_fraction))
-    y = (E > budget).astype(np.float32)
+    y = (E > budget).astype(np.float32) if labels is None else np.asarray(labels, np.float32)
     mean = X[:, :NFEAT].mean(axis=0)
     std = X[:, :NFEAT].std(axis=0) + 1e-6
     Xs = np.zeros_like(X)
@@ -120,6 +122,38 @@ def load(path):
 # training data of the shipped model.
 TRAIN_VARIANTS = ((0.9, 0.7, 2.1, 0.4, 1.3), (1.15, 3.0, 0.5, 2.2, 4.1))
 
+# The shipped model's training runs: Orszag-Tang (and the two variants) to
+# t = 0.5 and both Kelvin-Helmholtz problems to t = 5, all at 512^2, each
+# sampled at evenly spaced times: (problem, parameters, t_end).
+TRAIN_RUNS = (("ot", (), 0.5), ("otv", TRAIN_VARIANTS[0], 0.5), ("otv", TRAIN_VARIANTS[1], 0.5),
+              ("kh", (), 5.0), ("khb", (), 5.0))
+
+
+def run_labels(E, unsafe_fraction=0.05):
+    """Labels of one run's tiles: the top `unsafe_fraction` of its
+    FP32-vs-FP64 stage differences.  The level of those differences is set
+    by the flow (Kelvin-Helmholtz tiles all lie above the Orszag-Tang 95th
+    percentile, by the FP32 rounding of their larger energies), so one
+    global budget would label a whole problem unsafe; within each run the
+    labels mark the tiles whose FP32 flux is furthest from FP64."""
+    return (E > np.quantile(E, 1.0 - unsafe_fraction)).astype(np.float32)
+
+
+def training_set_512(n=512, snapshots=190, unsafe_fraction=0.05, log=print):
+    """Tiles of TRAIN_RUNS at n^2 (about 1.06e6 tiles at 512^2 with 190
+    snapshots each).  Returns (X, E, labels, E_ot): features, errors,
+    per-run labels and the errors of the plain Orszag-Tang run (the audits'
+    budget reference)."""
+    Xs, Es, Ys, ot = [], [], [], None
+    for problem, params, t_end in TRAIN_RUNS:
+        X, E = collect(problem, params=params, n=n, t_end=t_end, snapshots=snapshots, log=log)
+        Xs.append(X)
+        Es.append(E)
+        Ys.append(run_labels(E, unsafe_fraction))
+        if problem == "ot":
+            ot = E
+    return np.concatenate(Xs), np.concatenate(Es), np.concatenate(Ys), ot
+
 
 def training_set(sizes=(256, 512, 1024, 2048), early_n=4096, log=print):
     """Tiles of Orszag-Tang runs to t=0.5 at each size (24 snapshots), of the
@@ -145,17 +179,26 @@ def training_set(sizes=(256, 512, 1024, 2048), early_n=4096, log=print):
 def main(argv=None) -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--out", required=True)
-    ap.add_argument("--sizes", default="256,512,1024,2048")
+    ap.add_argument("--n", type=int, default=512,
+                    help="grid of the training runs (TRAIN_RUNS: OT and KH)")
+    ap.add_argument("--snapshots", type=int, default=190)
+    ap.add_argument("--ot-sizes", default="",
+                    help="instead: Orszag-Tang only, at these sizes (the earlier training set)")
     ap.add_argument("--early-n", type=int, default=4096)
     ap.add_argument("--epochs", type=int, default=20)
     ap.add_argument("--unsafe-fraction", type=float, default=0.05)
     args = ap.parse_args(argv)
     log = lambda m: print(m, file=sys.stderr, flush=True)
-    X, E, Eot = training_set(tuple(int(x) for x in args.sizes.split(",")), args.early_n, log)
-    # The budget is a quantile of the Orszag-Tang errors (the benchmark
-    # problem); labels elsewhere are relative to it.
+    if args.ot_sizes:
+        X, E, Eot = training_set(tuple(int(x) for x in args.ot_sizes.split(",")), args.early_n,
+                                 log)
+        labels = None
+    else:
+        X, E, labels, Eot = training_set_512(args.n, args.snapshots, args.unsafe_fraction, log)
+    # The audits' budget is a quantile of the Orszag-Tang errors (the
+    # benchmark problem).
     budget = float(np.quantile(Eot, 1.0 - args.unsafe_fraction))
-    model = fit(X, E, budget=budget, epochs=args.epochs, log=log)
+    model = fit(X, E, budget=budget, epochs=args.epochs, labels=labels, log=log)
     save(model, args.out)
     log(f"selector: model written to {args.out}")
     return 0

@@ -93,8 +93,9 @@ def test_compensated_state_io(mhd):
 
 
 @pytest.mark.parametrize("form", ["fp64", "fp32", "mixed_kahan"])
-def test_checkpoint_restart_bitwise(mhd, tmp_path, form):
-    """30 straight steps == 12 steps, checkpoint, new context, 18 steps."""
+def test_checkpoint_restart_bitwise(mhd, oracle, tmp_path, form):
+    """30 straight steps == 12 steps, checkpoint, new context, 18 steps; in
+    the exact forms the restarted run is also the CPU oracle's 30 steps."""
     p = mhd.PRECISIONS[form]
     cfg = mhd.SolverConfig(nx=48, ny=40, precision=p)
     a = mhd.Solver(cfg)
@@ -115,6 +116,15 @@ def test_checkpoint_restart_bitwise(mhd, tmp_path, form):
     c.close()
     assert st.steps == 30
     assert np.array_equal(np.ascontiguousarray(out).view(np.uint8), np.ascontiguousarray(ref).view(np.uint8))
+    if form in ("fp64", "fp32"):
+        import pyoracle
+
+        dt = np.float64 if form == "fp64" else np.float32
+        uo, info = oracle.advance(oracle.init("ot", 48, 40, dtype=dt), 30)
+        assert info["steps"] == 30 and st.t == info["t"]
+        assert np.array_equal(pyoracle.soa_to_aos(out).view(np.uint8), uo.view(np.uint8))
+        assert (st.degenerate, st.nonphysical) == (int(info["counters"][2]),
+                                                  int(info["counters"][3]))
     wrong = mhd.Solver(mhd.SolverConfig(nx=40, ny=48, precision=p))
     with pytest.raises(mhd.MhdError):
         wrong.load(path)
