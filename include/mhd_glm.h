@@ -41,11 +41,18 @@ extern "C" {
  *                SSPRK2 updates in FP64; recovery, CFL reductions, face
  *                reconstruction, rhs differences and psi damping in FP32.
  *                MUSCL only (first order keeps FP64 face states there).
- * MIXED, MIXED_KAHAN and FP64_FMA have no reference counterpart; MIXED has
- * the opposite allocation to REF_MIXED. */
+ *   MIXED_RECON64, MIXED_FLUX64, MIXED_FLUX64_NARROW
+ *                allocation variants of MIXED for locating its error: FP64
+ *                recovery and reconstruction; FP64 GLM-HLLD flux differenced
+ *                in FP64; FP64 flux narrowed to FP32 before the differences
+ *                (as REF_MIXED narrows it).  Single contexts, MUSCL + HLLD,
+ *                16x15 tiles, one stage per launch, no fallback or selector.
+ * MIXED, MIXED_KAHAN, FP64_FMA and the variants have no reference
+ * counterpart; MIXED has the opposite allocation to REF_MIXED. */
 enum mhd_precision {
   MHD_PREC_FP64 = 0, MHD_PREC_FP32 = 1, MHD_PREC_MIXED = 2,
-  MHD_PREC_MIXED_KAHAN = 3, MHD_PREC_FP64_FMA = 4, MHD_PREC_REF_MIXED = 5
+  MHD_PREC_MIXED_KAHAN = 3, MHD_PREC_FP64_FMA = 4, MHD_PREC_REF_MIXED = 5,
+  MHD_PREC_MIXED_RECON64 = 6, MHD_PREC_MIXED_FLUX64 = 7, MHD_PREC_MIXED_FLUX64_NARROW = 8
 };
 
 /* Run configuration.  Mirrors the reference CLI

@@ -11,7 +11,19 @@ namespace mhd {
 const Launchers* launchers_mixed(int mode) {
   static const Launchers m = mixed::make_launchers<mixed::MixedPol>(MODE_MIXED);
   static const Launchers k = mixed::make_launchers<mixed::KahanPol>(MODE_MIXED_KAHAN);
-  return mode == MODE_MIXED_KAHAN ? &k : &m;
+  static const Launchers r = mixed::make_variant_launchers<mixed::MixVarPol<double, float, float>>(
+      MODE_MIXED_RECON64);
+  static const Launchers f = mixed::make_variant_launchers<mixed::MixVarPol<float, double, double>>(
+      MODE_MIXED_FLUX64);
+  static const Launchers n = mixed::make_variant_launchers<mixed::MixVarPol<float, double, float>>(
+      MODE_MIXED_FLUX64_NARROW);
+  switch (mode) {
+    case MODE_MIXED_KAHAN: return &k;
+    case MODE_MIXED_RECON64: return &r;
+    case MODE_MIXED_FLUX64: return &f;
+    case MODE_MIXED_FLUX64_NARROW: return &n;
+    default: return &m;
+  }
 }
 
 }  // namespace mhd

@@ -116,8 +116,19 @@ const mhd::Launchers* launchers_for(int prec) {
     case MHD_PREC_MIXED_KAHAN: return mhd::launchers_mixed(mhd::MODE_MIXED_KAHAN);
     case MHD_PREC_FP64_FMA: return mhd::launchers_fma();
     case MHD_PREC_REF_MIXED: return mhd::launchers_exact(mhd::MODE_REF_MIXED);
+    case MHD_PREC_MIXED_RECON64: return mhd::launchers_mixed(mhd::MODE_MIXED_RECON64);
+    case MHD_PREC_MIXED_FLUX64: return mhd::launchers_mixed(mhd::MODE_MIXED_FLUX64);
+    case MHD_PREC_MIXED_FLUX64_NARROW:
+      return mhd::launchers_mixed(mhd::MODE_MIXED_FLUX64_NARROW);
     default: return nullptr;
   }
+}
+
+// The allocation variants of the mixed form run single grids with MUSCL +
+// HLLD on 16x15 tiles, one stage per launch.
+bool is_variant(int prec) {
+  return prec == MHD_PREC_MIXED_RECON64 || prec == MHD_PREC_MIXED_FLUX64 ||
+         prec == MHD_PREC_MIXED_FLUX64_NARROW;
 }
 
 int fail(mhd_ctx* c, int code, const std::string& msg) {
@@ -598,6 +609,10 @@ int mhd_create(const mhd_cfg* cfg, mhd_ctx** out) {
   if (cfg->precision == MHD_PREC_REF_MIXED && !cfg->muscl) return 1;
   if ((long long)cfg->nx * cfg->ny > (long long)INT_MAX) return 1;
   if (cfg->fuse < 0 || cfg->fuse > 2) return 1;
+  if (is_variant(cfg->precision) &&
+      (!cfg->muscl || !cfg->hlld || cfg->fuse == 2 ||
+       (cfg->tile >= 0 && cfg->tile < mhd::TILE_COUNT && cfg->tile != mhd::TILE_16x15)))
+    return 1;
   const long long ncells = (long long)cfg->nx * cfg->ny;
   int tile = cfg->tile, fuse = cfg->fuse;
   if (tile < 0 || tile >= mhd::TILE_COUNT) {
@@ -1045,6 +1060,10 @@ extern "C" int mhd_run_noalloc(const mhd_cfg* cfg_in, void* state, size_t ld, do
   if (cfg.precision == MHD_PREC_REF_MIXED && !cfg.muscl)
     return noalloc_reject("REF_MIXED runs MUSCL only");
   if (cfg.fuse == 2) return noalloc_reject("one stage per launch only (fuse 0 or 1)");
+  if (is_variant(cfg.precision) &&
+      (!cfg.muscl || !cfg.hlld ||
+       (cfg.tile >= 0 && cfg.tile < mhd::TILE_COUNT && cfg.tile != mhd::TILE_16x15)))
+    return noalloc_reject("the allocation variants run MUSCL + HLLD on 16x15 tiles");
   const mhd::Launchers* L = launchers_for(cfg.precision);
   const size_t cells = (size_t)cfg.nx * cfg.ny;
   if (ld < cells) return noalloc_reject("ld is smaller than nx * ny");
@@ -1364,7 +1383,7 @@ int mhd_ensemble_create(const mhd_cfg* cfg, int n, const int* nx, const int* ny,
   *out = nullptr;
   if (!cfg || n < 1 || !nx || !ny) return 1;
   if (!(cfg->cfl > 0.0 && cfg->cfl < 1.0) || !(cfg->alpha > 0.0)) return 1;
-  if (!launchers_for(cfg->precision)) return 1;
+  if (!launchers_for(cfg->precision) || is_variant(cfg->precision)) return 1;
   if (cfg->precision == MHD_PREC_REF_MIXED && !cfg->muscl) return 1;
   for (int i = 0; i < n; i++)
     if (nx[i] < 2 || ny[i] < 2 || (long long)nx[i] * ny[i] > (long long)INT_MAX) return 1;

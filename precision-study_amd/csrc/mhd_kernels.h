@@ -19,7 +19,11 @@ enum Mode {
   MODE_MIXED_KAHAN = 3,  // as MIXED with an FP32 + FP32 compensated state
   MODE_FP64_FMA = 4,     // FP64 with FMA contraction
   MODE_REF_MIXED = 5,    // the reference's stage-level mixed build (FP64 flux + update)
-  MODE_COUNT = 6
+  // Allocation variants of MIXED (single grids, MUSCL + HLLD, 16x15 tiles):
+  MODE_MIXED_RECON64 = 6,        // FP64 recovery and reconstruction
+  MODE_MIXED_FLUX64 = 7,         // FP64 GLM-HLLD flux, differenced in FP64
+  MODE_MIXED_FLUX64_NARROW = 8,  // FP64 flux narrowed to FP32 before differencing
+  MODE_COUNT = 9
 };
 enum RunState { RUN_ACTIVE = 0, RUN_DONE = 1, RUN_HALTED = 2 };
 
@@ -213,7 +217,7 @@ struct Launchers {
 
 const Launchers* launchers_exact(int mode);  // MODE_FP64, MODE_FP32, MODE_REF_MIXED
 const Launchers* launchers_fma();            // MODE_FP64_FMA
-const Launchers* launchers_mixed(int mode);  // MODE_MIXED, MODE_MIXED_KAHAN
+const Launchers* launchers_mixed(int mode);  // MODE_MIXED, MODE_MIXED_KAHAN, the variants
 
 // Scratch of the no-allocation run (k_noalloc.hip): a fixed device array per
 // device, reserved by the code object's loader.  24 MiB holds the stage-1

@@ -389,6 +389,22 @@ struct MixedPol {
   static constexpr bool kRedo = true;  // FP64 fallback of unusable tiles
   static constexpr bool kRefMixed = false;
 };
+// Allocation variants of the mixed form, for the study of where its error
+// comes from: the mixed form with one of its FP32 parts in FP64 (RP: the
+// recovered primitives and the reconstruction, FP64 recovery in every cell
+// when double; R: the GLM-HLLD/HLL flux; RS: the face fluxes as the rhs
+// differences read them).  No FP64 fallback; MUSCL + HLLD, 16x15 tiles.
+template <class RP_, class R_, class RS_>
+struct MixVarPol {
+  using R = R_;
+  using RP = RP_;
+  using RS = RS_;
+  using T = double;
+  using S = double;
+  static constexpr int kPlanes = 9;
+  static constexpr bool kRedo = false;
+  static constexpr bool kRefMixed = false;
+};
 // Compensated: as Mixed, but the state is FP32 hi + FP32 lo (planes 0-8 and
 // 9-17), accumulated with float-float additions.
 struct KahanPol {
@@ -495,6 +511,9 @@ struct CellIO<MixedPol> {
   }
 };
 
+template <class RP, class R, class RS>
+struct CellIO<MixVarPol<RP, R, RS>> : CellIO<MixedPol> {};
+
 template <>
 struct CellIO<KahanPol> {
   float hi[9], lo[9];
@@ -588,6 +607,20 @@ __device__ __forceinline__ void recover(const Phys<float>& c, const CellIO<Kahan
                                         float ratio, float P[9]) {
   recover_mixed<KahanPol, MUSCL>(c, u, ratio, P);
 }
+// Allocation variants: the mixed recovery, or FP64 recovery in every cell
+// when the primitives are FP64.
+template <bool MUSCL, class RP, class R, class RS>
+__device__ __forceinline__ void recover(const Phys<RP>& c, const CellIO<MixVarPol<RP, R, RS>>& u,
+                                        float ratio, RP P[9]) {
+  if constexpr (sizeof(RP) == 4) {
+    recover_mixed<MixVarPol<RP, R, RS>, MUSCL>(c, u, ratio, P);
+  } else if (MUSCL) {
+    cons_to_prim<double, 9>(c, u.v, P);
+  } else {
+#pragma unroll
+    for (int k = 0; k < 9; k++) P[k] = u.v[k];
+  }
+}
 
 // Face-flux differences of one cell: -(F_r - F_l)/dx - (G_t - G_b)/dy
 // (mhd2d_glm_cpu.cpp:278-281) in type A.
@@ -647,6 +680,24 @@ __device__ __forceinline__ void advance_cell(const CellIO<MixedPol>& src,
                                              const CellIO<MixedPol>& old, const double rhs_flux[9],
                                              double dt, double damping, CellIO<MixedPol>& nu,
                                              double out[9]) {
+#pragma unroll
+  for (int k = 0; k < 9; k++) {
+    double rhs = rhs_flux[k];
+    if (k == PSI) rhs -= damping * src.v[PSI];
+    if (STAGE == 1)
+      out[k] = src.v[k] + dt * rhs;
+    else
+      out[k] = 0.5 * old.v[k] + 0.5 * (src.v[k] + dt * rhs);
+    nu.v[k] = out[k];
+  }
+}
+
+// Allocation variants: as the mixed policy.
+template <int STAGE, class RP, class R, class RS>
+__device__ __forceinline__ void advance_cell(const CellIO<MixVarPol<RP, R, RS>>& src,
+                                             const CellIO<MixVarPol<RP, R, RS>>& old,
+                                             const double rhs_flux[9], double dt, double damping,
+                                             CellIO<MixVarPol<RP, R, RS>>& nu, double out[9]) {
 #pragma unroll
   for (int k = 0; k < 9; k++) {
     double rhs = rhs_flux[k];
@@ -1824,6 +1875,39 @@ static hipError_t ens_set_target(void* ctrls, int n, double t_end, unsigned* que
   hipLaunchKernelGGL(k_set_target_multi<T>, dim3((n + 63) / 64 > 0 ? (n + 63) / 64 : 1), dim3(64), 0,
                      s, (Ctrl<T>*)ctrls, n, t_end, queue);
   return hipGetLastError();
+}
+
+// Launchers of an allocation variant: single grids, MUSCL + HLLD, 16x15
+// tiles, one stage per launch (other configurations are refused).
+template <class Pol>
+static hipError_t launch_step_variant(const GridDesc& g, void* u, void* u1, void* ctrl,
+                                      hipStream_t s, hipEvent_t* ev) {
+  if (!(g.muscl && g.hlld) || g.tile != TILE_16x15) return hipErrorInvalidValue;
+  return launch_step_t<Pol, true, true, 16, 15>(g, u, u1, ctrl, s, ev);
+}
+
+template <class Pol>
+static hipError_t launch_stage_variant(const GridDesc& g, int stage, const void* src,
+                                       const void* uold, void* dst, void* ctrl, hipStream_t s) {
+  if (!(g.muscl && g.hlld) || g.tile != TILE_16x15) return hipErrorInvalidValue;
+  return launch_stage_t<Pol, true, true, 16, 15>(g, stage, src, uold, dst, ctrl, s);
+}
+
+template <class Pol>
+static Launchers make_variant_launchers(int mode) {
+  Launchers L{};
+  L.mode = mode;
+  L.ctrl_bytes = sizeof(Ctrl<double>);
+  L.time_is_float = 0;
+  L.state_elem = sizeof(double);
+  L.state_planes = 9;
+  L.speed = &launch_speed<Pol>;
+  L.commit = &launch_commit<double>;
+  L.set_target = &launch_target<double>;
+  L.step = &launch_step_variant<Pol>;
+  L.stage = &launch_stage_variant<Pol>;
+  L.finish = &launch_finish<double>;
+  return L;
 }
 
 #include "mhd_fused.inc.hip"

@@ -32,12 +32,15 @@ PREC_MIXED = 2
 PREC_MIXED_KAHAN = 3
 PREC_FP64_FMA = 4
 PREC_REF_MIXED = 5
+PREC_MIXED_RECON64 = 6
+PREC_MIXED_FLUX64 = 7
+PREC_MIXED_FLUX64_NARROW = 8
 PRECISIONS = {"fp64": PREC_FP64, "fp32": PREC_FP32, "mixed": PREC_MIXED,
               "mixed_kahan": PREC_MIXED_KAHAN, "fp64_fma": PREC_FP64_FMA,
-              "ref_mixed": PREC_REF_MIXED}
+              "ref_mixed": PREC_REF_MIXED, "mixed_recon64": PREC_MIXED_RECON64,
+              "mixed_flux64": PREC_MIXED_FLUX64, "mixed_flux64_narrow": PREC_MIXED_FLUX64_NARROW}
 # Element type of host state buffers.
-DTYPES = {PREC_FP64: np.float64, PREC_FP32: np.float32, PREC_MIXED: np.float64,
-          PREC_MIXED_KAHAN: np.float64, PREC_FP64_FMA: np.float64, PREC_REF_MIXED: np.float64}
+DTYPES = {p: (np.float32 if p == PREC_FP32 else np.float64) for p in PRECISIONS.values()}
 
 # Public symbols of include/mhd_glm.h.
 EXPORTED = (
