@@ -61,6 +61,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(ROOT, "precision-study_amd"))
 
 HBM_PEAK_GBS = 8000.0
+FP64_PEAK_TFLOPS = 78.6
 METRIC = "cell-updates/s, 2D MHD-GLM HLLD 4096²; mixed vs FP64 speedup at L2≤1e-5"
 # Algorithmic HBM bytes per cell-update (9 variables): read U^n, write
 # U^{n+1}; every intermediate stays on chip in an ideally fused step.
@@ -123,14 +124,17 @@ def host_cpu_model() -> str:
 def make_solver(mhdglm, form, n, tile=-1, problem="ot", fuse=0):
     """n^2 solver of `problem` in an arithmetic form; "selector" is the mixed
     form with the committed selector model deciding FP64 tiles (which runs
-    one stage per launch whatever `fuse` says)."""
-    prec = mhdglm.PREC_MIXED if form == "selector" else mhdglm.PRECISIONS[form]
+    one stage per launch whatever `fuse` says), "selector@X" the same with
+    the network's threshold X."""
+    sel = form.startswith("selector")
+    prec = mhdglm.PREC_MIXED if sel else mhdglm.PRECISIONS[form]
     solver = mhdglm.Solver(mhdglm.SolverConfig(nx=n, ny=n, precision=prec, tile=tile,
                                                fuse=fuse))
     solver.init_problem(problem)
-    if form == "selector":
+    if sel:
         from mhdglm import selector
-        solver.configure_selector(selector.load(selector.DEFAULT_MODEL))
+        thr = float(form.split("@", 1)[1]) if "@" in form else None
+        solver.configure_selector(selector.load(selector.DEFAULT_MODEL), threshold=thr)
     return solver
 
 
@@ -178,7 +182,7 @@ def whole_run(mhdglm, form, n, problem="ot", t_end=0.5):
     wall = time.perf_counter() - t0
     u = s.get_state(0).astype("float64")
     sel = None
-    if form == "selector":
+    if form.startswith("selector"):
         sel = s.selector_stats()
         sel["fp64_tile_fraction"] = round(sel["fp64_tile_steps"] / max(sel["steps"] * s.selector_tiles(), 1), 5)
     s.close()
@@ -424,6 +428,20 @@ def main():
                     "issue_per_cu_cycle": round(v["issue_per_cu_cycle"], 3),
                     "issue_incl_trans": round(v["issue_incl_trans_quarter_rate"], 3),
                     "fp64_issue_limit": 1.0}
+            if "flops_fp64_waves" in v and not fused:
+                # FP64 FLOP of the two stage launches (SQ_INSTS_VALU_FLOPS_FP64,
+                # per wave, x64 lanes) over their live HIP-event time, against
+                # the 78.6 TFLOP/s FP64 vector peak.
+                v2 = prof.get("stage2_valu", {})
+                flops = 64.0 * (v["flops_fp64_waves"] + v2.get("flops_fp64_waves", 0.0))
+                tf = flops / ((ms[0] + ms[1]) * 1e-3) / 1e12
+                valu["fp64_flops_per_step"] = flops
+                valu["fp64_tflops"] = round(tf, 2)
+                valu["fp64_peak_tflops"] = FP64_PEAK_TFLOPS
+                valu["fp64_frac"] = round(tf / FP64_PEAK_TFLOPS, 4)
+                valu["share_of_valu_fp64_arith"] = round(
+                    (v["add_f64"] + v["mul_f64"] + v["fma_f64_counter"] + v["trans_f64_counter"])
+                    / v["insts"], 3)
     roofline = {
         "bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
         "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
