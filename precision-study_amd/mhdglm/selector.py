@@ -45,12 +45,17 @@ def init_params(seed: int = 0) -> np.ndarray:
     return p
 
 
+# Tile shape of the training data (16x15, the stage kernels' default and the
+# autotuned shape from 1024^2 up): features are tile aggregates.
+TRAIN_TILE = 3
+
+
 def collect(problem="ot", params=(), n=512, steps=600, every=4, t_end=None, snapshots=24,
-            log=print):
+            log=print, tile=TRAIN_TILE):
     """Features and pair errors of every tile of a MIXED run of `problem` at
     n x n from t=0: every `every` steps for `steps` steps, or (t_end given)
     at `snapshots` evenly spaced times in [0, t_end)."""
-    s = Solver(SolverConfig(nx=n, ny=n, precision=PREC_MIXED))
+    s = Solver(SolverConfig(nx=n, ny=n, precision=PREC_MIXED, tile=tile))
     if params:
         from . import problem_state_ex
         s.set_state(problem_state_ex(problem, params, n, n))

@@ -1,0 +1,77 @@
+"""Host code under AddressSanitizer + UndefinedBehaviorSanitizer (CPU only).
+
+  * the CPU oracle: every C entry point on small grids and special states
+    (zeros, NaN, infinities, extreme magnitudes), every problem, both
+    precisions, the region masks, a failing run; and the oracle's CLI on a
+    whole small run;
+  * the product's host side (mhd_api.cpp and the CLI, rebuilt with the
+    sanitizers against the same device objects): initial conditions,
+    argument validation, the autotuner's table, and the CLI's usage errors --
+    everything that runs without a device.
+A sanitizer stops the program at the first error it finds; the tests require
+a clean exit and no report.  Leak checking is off for programs that load the
+HIP runtime (its own allocations live until exit)."""
+import os
+import subprocess
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+ORACLE = os.path.join(ROOT, "oracle")
+PKG = os.path.join(ROOT, "precision-study_amd")
+REPORTS = ("ERROR: AddressSanitizer", "runtime error:", "ERROR: LeakSanitizer")
+
+
+def make(directory):
+    r = subprocess.run(["make", "-C", directory, "sanitize"], capture_output=True, text=True,
+                       timeout=600)
+    assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-2000:]
+
+
+def run(args, env_extra, **kw):
+    env = dict(os.environ, **env_extra)
+    r = subprocess.run(args, capture_output=True, text=True, timeout=300, env=env, **kw)
+    for rep in REPORTS:
+        assert rep not in r.stderr, r.stderr[-3000:]
+    return r
+
+
+@pytest.fixture(scope="module")
+def oracle_san():
+    make(ORACLE)
+    return os.path.join(ORACLE, "build", "san")
+
+
+@pytest.fixture(scope="module")
+def host_san():
+    make(PKG)
+    return os.path.join(PKG, "build", "san")
+
+
+def test_oracle_entry_points(oracle_san):
+    r = run([os.path.join(oracle_san, "sanitize_checks")], {"ASAN_OPTIONS": "detect_leaks=1"})
+    assert r.returncode == 0 and "ok" in r.stdout, r.stdout + r.stderr
+
+
+def test_oracle_cli_whole_run(oracle_san, tmp_path):
+    r = run([os.path.join(oracle_san, "mhd2d_glm_oracle_fp64"), "32", "32", "0.05", "0.25",
+             "muscl", "hlld", "0.18"], {"ASAN_OPTIONS": "detect_leaks=1", "OUTDIR": str(tmp_path)})
+    assert r.returncode == 0 and "Done. Steps:" in r.stdout
+    assert len(list(tmp_path.iterdir())) == 1
+
+
+def test_host_entry_points(host_san):
+    r = run([os.path.join(host_san, "host_checks")],
+            {"ASAN_OPTIONS": "detect_leaks=0",
+             "MHD_AUTOTUNE_FILE": os.path.join(PKG, "autotune.txt")})
+    assert r.returncode == 0 and "0 failures" in r.stdout, r.stdout + r.stderr
+
+
+@pytest.mark.parametrize("args", [
+    ["0", "8"], ["8", "8", "0.5", "1.5"], ["8", "8", "0.5", "0.25", "weno"],
+    ["8", "8", "0.5", "0.25", "muscl", "roe"], ["8", "8", "0.5", "0.25", "muscl", "hlld", "0"],
+    ["8", "8", "0.5", "0.25", "muscl", "hlld", "0.18", "otv"], ["1", "8"]])
+def test_host_cli_usage_errors(host_san, tmp_path, args):
+    r = run([os.path.join(host_san, "mhd2d_glm_amd")] + args,
+            {"ASAN_OPTIONS": "detect_leaks=0", "OUTDIR": str(tmp_path)})
+    assert r.returncode == 1 and "Error" in r.stderr
