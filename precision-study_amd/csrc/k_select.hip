@@ -11,9 +11,10 @@
 //   k_tile_features  12 features per tile from the FP64 state
 //   k_sumsq / k_tile_errors   label source (scaled max difference per tile)
 //   k_mlp_infer      probabilities + FP64 flags (with sticky audit flags)
-//   k_mlp_grad       forward (MFMA) + backward (FP32) of a minibatch,
-//                    gradients accumulated into a global buffer
-//   k_adam           Adam update of the FP32 master weights
+//   k_mlp_grad       forward (BF16 MFMA) + backward (FP32 MFMA tiles) of a
+//                    minibatch, one gradient row per block (no atomics)
+//   k_sum_partials   the block rows summed in block order (deterministic)
+//   k_adam           the same sum, then Adam on the FP32 master weights
 //   k_route          FP64 and mixed tile lists of the current flags
 //   k_audit_pick     the step's audit sample (and the control-block snapshot)
 //   k_tile_errors    (audit mode) audited FP32 tiles against their FP64
@@ -360,45 +361,66 @@ __global__ __launch_bounds__(256) void k_mlp_infer(const float* raw, int n, cons
   }
 }
 
-// Forward + backward of up to 64 samples per block; gradients of
-// sum_s w_s BCE(p_s, y_s) / wsum added into grad (FP32 atomics).
-// Samples are idx[offset + s] of the training arrays.
+// One 16x16 tile of C = A B over K in steps of 4 on the FP32 matrix cores
+// (v_mfma_f32_16x16x4_f32: each lane holds A[l % 16][k0 + l / 16] and
+// B[k0 + l / 16][l % 16]; lane l receives C[4 (l / 16) + i][l % 16],
+// i = 0..3).  a(m, k) and b(k, n) read the operands.
+template <class FA, class FB>
+__device__ __forceinline__ f32x4 mfma_tile_f32(int K, FA a, FB b) {
+  const int l = threadIdx.x & 63, r = l & 15, kq = l >> 4;
+  f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+  for (int k0 = 0; k0 < K; k0 += 4)
+    acc = __builtin_amdgcn_mfma_f32_16x16x4f32(a(r, k0 + kq), b(k0 + kq, r), acc, 0, 0, 0);
+  return acc;
+}
+
+// Forward + backward of up to 64 samples per block (4 waves x 16).  The
+// gradients of sum_s w_s BCE(p_s, y_s) / wsum and that loss are written to
+// this block's row of `partial` (kParams + 1 floats per block), no atomics:
+// sel_adam_partials / sel_sum_partials add the rows in block order, so the
+// result does not depend on scheduling.  The three weight gradients and the
+// back-propagation through W2 are 16x16 tiles on the FP32 matrix cores, with
+// the forward's BF16 activations and weights as operands; bias and w3
+// gradients are column sums in FP32.  Samples are idx[offset + s].
 __global__ __launch_bounds__(256) void k_mlp_grad(const float* X, const float* Y, const float* Wt,
                                                   const int* idx, int offset, int count,
-                                                  float wsum, const float* P, float* grad,
-                                                  float* loss) {
+                                                  float wsum, const float* P, float* partial) {
   __shared__ WaveScratch ws[4];
   __shared__ float xs[4][16 * NIN];
   __shared__ float z3[4][16];
   __shared__ float dz3[64];
   __shared__ float dz2[64][H];
   __shared__ float dz1[64][H];
+  __shared__ float lred[64];
   const int w = threadIdx.x >> 6, l = threadIdx.x & 63, tid = threadIdx.x;
+  const int r = l & 15, kq = l >> 4;
   const int b0 = blockIdx.x * 64;
   const int nblk = count - b0 < 0 ? 0 : (count - b0 > 64 ? 64 : count - b0);
   const int nvalid = nblk - w * 16 < 0 ? 0 : (nblk - w * 16 > 16 ? 16 : nblk - w * 16);
-  if (l < nvalid) {
-    const int s = idx[offset + b0 + w * 16 + l];
-    for (int k = 0; k < NIN; k++) xs[w][l * NIN + k] = X[(size_t)s * NIN + k];
+  float* out = partial + (size_t)blockIdx.x * (kParams + 1);
+  if (l < 16) {
+    const int s = l < nvalid ? idx[offset + b0 + w * 16 + l] : -1;
+    for (int k = 0; k < NIN; k++) xs[w][l * NIN + k] = s >= 0 ? X[(size_t)s * NIN + k] : 0.f;
   }
   __syncthreads();
   mlp_forward_wave(xs[w], nvalid, P, ws[w], z3[w]);
-  // dL/dz3 and the loss.
+  // dL/dz3 and each sample's loss.
   if (tid < 64) {
-    float d = 0.f;
+    float d = 0.f, lp = 0.f;
     if (tid < nblk) {
       const int s = idx[offset + b0 + tid];
       const float z = z3[tid >> 4][tid & 15];
       const float p = 1.f / (1.f + __expf(-z));
       const float y = Y[s], wt = Wt[s];
       d = (p - y) * wt / wsum;
-      const float lp = -wt * (y * __logf(fmaxf(p, 1e-12f)) + (1.f - y) * __logf(fmaxf(1.f - p, 1e-12f)));
-      atomicAdd(loss, lp / wsum);
+      lp = -wt * (y * __logf(fmaxf(p, 1e-12f)) + (1.f - y) * __logf(fmaxf(1.f - p, 1e-12f))) /
+           wsum;
     }
     dz3[tid] = d;
+    lred[tid] = lp;
   }
   __syncthreads();
-  // dz2 = dz3 w3 (z2 > 0); gw3 and gb3.
+  // dz2 = dz3 w3 (z2 > 0); gw3, gb3 and the loss as column sums.
   for (int e = tid; e < 64 * H; e += 256) {
     const int s = e / H, nn = e % H;
     const float z2 = ws[s >> 4].z2[s & 15][nn];
@@ -410,57 +432,86 @@ __global__ __launch_bounds__(256) void k_mlp_grad(const float* X, const float* Y
       const float z2 = ws[s >> 4].z2[s & 15][tid];
       g += (z2 > 0.f ? z2 : 0.f) * dz3[s];
     }
-    atomicAdd(&grad[kW3 + tid], g);
+    out[kW3 + tid] = g;
   }
-  if (tid == 0) {
-    float g = 0.f;
-    for (int s = 0; s < nblk; s++) g += dz3[s];
-    atomicAdd(&grad[kB3], g);
+  if (tid == 64) {
+    float g = 0.f, lsum = 0.f;
+    for (int s = 0; s < nblk; s++) {
+      g += dz3[s];
+      lsum += lred[s];
+    }
+    out[kB3] = g;
+    out[kParams] = lsum;
   }
   __syncthreads();
-  // gW2[k][n] = sum_s a1_bf16[s][k] dz2[s][n]; gb2; dz1 = (dz2 W2_bf16^T) (z1 > 0).
-  for (int e = tid; e < H * H; e += 256) {
-    const int k = e / H, nn = e % H;
-    float g = 0.f;
-    for (int s = 0; s < nblk; s++) {
-      const float a1 = __uint_as_float(((unsigned)(unsigned short)ws[s >> 4].a1[s & 15][k]) << 16);
-      g += a1 * dz2[s][nn];
-    }
-    atomicAdd(&grad[kW2 + e], g);
+  // gW2[k][n] = sum_s a1[s][k] dz2[s][n]: wave w takes rows k of tile w.
+  for (int nt = 0; nt < 4; nt++) {
+    const f32x4 acc = mfma_tile_f32(
+        64,
+        [&](int m, int s) {
+          return __uint_as_float(((unsigned)(unsigned short)ws[s >> 4].a1[s & 15][w * 16 + m]) << 16);
+        },
+        [&](int s, int n) { return dz2[s][nt * 16 + n]; });
+    for (int i = 0; i < 4; i++) out[kW2 + (w * 16 + 4 * kq + i) * H + nt * 16 + r] = acc[i];
   }
   if (tid < H) {
     float g = 0.f;
     for (int s = 0; s < nblk; s++) g += dz2[s][tid];
-    atomicAdd(&grad[kB2 + tid], g);
+    out[kB2 + tid] = g;
   }
-  for (int e = tid; e < 64 * H; e += 256) {
-    const int s = e / H, k = e % H;
-    float d = 0.f;
-    if (s < nblk && ws[s >> 4].z1[s & 15][k] > 0.f)
-      for (int nn = 0; nn < H; nn++) d += dz2[s][nn] * bf16_val(P[kW2 + k * H + nn]);
-    dz1[s][k] = d;
+  // dz1[s][k] = (sum_n dz2[s][n] W2[k][n]) (z1 > 0): wave w takes its own
+  // 16 samples.
+  for (int kt = 0; kt < 4; kt++) {
+    const f32x4 acc = mfma_tile_f32(
+        H, [&](int m, int n) { return dz2[w * 16 + m][n]; },
+        [&](int n, int k) { return bf16_val(P[kW2 + (kt * 16 + k) * H + n]); });
+    for (int i = 0; i < 4; i++) {
+      const int sl = 4 * kq + i, k = kt * 16 + r;
+      dz1[w * 16 + sl][k] = ws[w].z1[sl][k] > 0.f ? acc[i] : 0.f;
+    }
   }
   __syncthreads();
-  // gW1[k][n] = sum_s x_bf16[s][k] dz1[s][n]; gb1.
-  for (int e = tid; e < NIN * H; e += 256) {
-    const int k = e / H, nn = e % H;
-    float g = 0.f;
-    for (int s = 0; s < nblk; s++) g += bf16_val(xs[s >> 4][(s & 15) * NIN + k]) * dz1[s][nn];
-    atomicAdd(&grad[kW1 + e], g);
+  // gW1[k][n] = sum_s x[s][k] dz1[s][n] (x as the forward's BF16): wave w
+  // takes columns n of tile w.
+  {
+    const f32x4 acc = mfma_tile_f32(
+        64, [&](int k, int s) { return bf16_val(xs[s >> 4][(s & 15) * NIN + k]); },
+        [&](int s, int n) { return dz1[s][w * 16 + n]; });
+    for (int i = 0; i < 4; i++) out[kW1 + (4 * kq + i) * H + w * 16 + r] = acc[i];
   }
   if (tid < H) {
     float g = 0.f;
     for (int s = 0; s < nblk; s++) g += dz1[s][tid];
-    atomicAdd(&grad[kB1 + tid], g);
+    out[kB1 + tid] = g;
   }
 }
 
-// Adam on the FP32 masters; clears the gradient.
-__global__ void k_adam(float* P, float* grad, float* m, float* v, int step, float lr) {
+// Sum of the per-block rows of `partial` in block order: the gradient
+// (kParams) and the loss (one more entry, added to *loss).
+__global__ void k_sum_partials(const float* partial, int nparts, float* grad, float* loss) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= kParams) return;
+  if (i > kParams) return;
+  float g = 0.f;
+  for (int b = 0; b < nparts; b++) g += partial[(size_t)b * (kParams + 1) + i];
+  if (i < kParams)
+    grad[i] = g;
+  else if (loss)
+    *loss += g;
+}
+
+// Adam on the FP32 masters from the per-block gradient rows (summed in block
+// order); the minibatch loss is added to *loss.
+__global__ void k_adam(float* P, const float* partial, int nparts, float* m, float* v, int step,
+                       float lr, float* loss) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i > kParams) return;
+  float g = 0.f;
+  for (int b = 0; b < nparts; b++) g += partial[(size_t)b * (kParams + 1) + i];
+  if (i == kParams) {
+    if (loss) *loss += g;
+    return;
+  }
   const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
-  const float g = grad[i];
   const float mi = b1 * m[i] + (1.f - b1) * g, vi = b2 * v[i] + (1.f - b2) * g * g;
   m[i] = mi;
   v[i] = vi;
@@ -468,7 +519,6 @@ __global__ void k_adam(float* P, float* grad, float* m, float* v, int step, floa
   float p = P[i] - lr * mh / (sqrtf(vh) + eps);
   if (i >= kW1 + NFEAT * H && i < kB1) p = 0.f;  // padding rows of W1 stay zero
   P[i] = p;
-  grad[i] = 0.f;
 }
 
 // Tile lists of the current flags, by one block of kRouteThreads, each in
@@ -627,18 +677,25 @@ hipError_t sel_mlp_infer(const float* raw, int n, const float* params, const flo
 }
 
 hipError_t sel_mlp_grad(const float* X, const float* Y, const float* W, const int* idx, int offset,
-                        int count, float wsum, const float* params, float* grad, float* loss,
+                        int count, float wsum, const float* params, float* partial,
                         hipStream_t s) {
   if (count <= 0) return hipSuccess;
   hipLaunchKernelGGL(sel::k_mlp_grad, dim3((count + 63) / 64), dim3(256), 0, s, X, Y, W, idx,
-                     offset, count, wsum, params, grad, loss);
+                     offset, count, wsum, params, partial);
   return hipGetLastError();
 }
 
-hipError_t sel_adam(float* params, float* grad, float* m, float* v, int step, float lr,
-                    hipStream_t s) {
-  hipLaunchKernelGGL(sel::k_adam, dim3((sel::kParams + 255) / 256), dim3(256), 0, s, params, grad, m, v,
-                     step, lr);
+hipError_t sel_sum_partials(const float* partial, int nparts, float* grad, float* loss,
+                            hipStream_t s) {
+  hipLaunchKernelGGL(sel::k_sum_partials, dim3((sel::kParams + 1 + 255) / 256), dim3(256), 0, s,
+                     partial, nparts, grad, loss);
+  return hipGetLastError();
+}
+
+hipError_t sel_adam(float* params, const float* partial, int nparts, float* m, float* v, int step,
+                    float lr, float* loss, hipStream_t s) {
+  hipLaunchKernelGGL(sel::k_adam, dim3((sel::kParams + 1 + 255) / 256), dim3(256), 0, s, params,
+                     partial, nparts, m, v, step, lr, loss);
   return hipGetLastError();
 }
 

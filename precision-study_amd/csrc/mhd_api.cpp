@@ -1831,12 +1831,14 @@ int mhd_mlp_forward(const float* params, const float* x, int n, float* prob) {
 int mhd_mlp_gradients(const float* params, const float* x, const float* y, const float* w, int n,
                       float* grad, float* loss) {
   if (!params || !x || !y || !w || !grad || n <= 0) return 1;
-  DevBuf p, xb, yb, wb, ib, gb, lb;
+  DevBuf p, xb, yb, wb, ib, gb, lb, pb;
   const size_t np = mhd::sel::kParams;
+  const int nparts = (n + 63) / 64;
   if (hipMalloc(&p.p, np * 4) != hipSuccess || hipMalloc(&xb.p, (size_t)n * mhd::sel::NIN * 4) != hipSuccess ||
       hipMalloc(&yb.p, (size_t)n * 4) != hipSuccess || hipMalloc(&wb.p, (size_t)n * 4) != hipSuccess ||
       hipMalloc(&ib.p, (size_t)n * 4) != hipSuccess || hipMalloc(&gb.p, np * 4) != hipSuccess ||
-      hipMalloc(&lb.p, 4) != hipSuccess)
+      hipMalloc(&lb.p, 4) != hipSuccess ||
+      hipMalloc(&pb.p, (size_t)nparts * (np + 1) * 4) != hipSuccess)
     return 9;
   std::vector<int> idx(n);
   std::iota(idx.begin(), idx.end(), 0);
@@ -1850,8 +1852,9 @@ int mhd_mlp_gradients(const float* params, const float* x, const float* y, const
       hipMemset(gb.p, 0, np * 4) != hipSuccess || hipMemset(lb.p, 0, 4) != hipSuccess)
     return 9;
   if (mhd::sel_mlp_grad((const float*)xb.p, (const float*)yb.p, (const float*)wb.p, (const int*)ib.p,
-                        0, n, (float)ws, (const float*)p.p, (float*)gb.p, (float*)lb.p,
-                        nullptr) != hipSuccess)
+                        0, n, (float)ws, (const float*)p.p, (float*)pb.p, nullptr) != hipSuccess ||
+      mhd::sel_sum_partials((const float*)pb.p, nparts, (float*)gb.p, (float*)lb.p, nullptr) !=
+          hipSuccess)
     return 9;
   if (hipMemcpy(grad, gb.p, np * 4, hipMemcpyDeviceToHost) != hipSuccess) return 9;
   if (loss && hipMemcpy(loss, lb.p, 4, hipMemcpyDeviceToHost) != hipSuccess) return 9;
@@ -1862,10 +1865,12 @@ int mhd_mlp_train(float* params, const float* x, const float* y, const float* w,
                   int epochs, int batch, float lr, unsigned seed, float* epoch_loss) {
   if (!params || !x || !y || !w || n <= 0 || epochs <= 0 || batch <= 0) return 1;
   const size_t np = mhd::sel::kParams;
-  DevBuf p, xb, yb, wb, ib, gb, mb, vb, lb;
+  DevBuf p, xb, yb, wb, ib, pb, mb, vb, lb;
+  const int maxparts = (std::min(batch, n) + 63) / 64;
   if (hipMalloc(&p.p, np * 4) != hipSuccess || hipMalloc(&xb.p, (size_t)n * mhd::sel::NIN * 4) != hipSuccess ||
       hipMalloc(&yb.p, (size_t)n * 4) != hipSuccess || hipMalloc(&wb.p, (size_t)n * 4) != hipSuccess ||
-      hipMalloc(&ib.p, (size_t)n * 4) != hipSuccess || hipMalloc(&gb.p, np * 4) != hipSuccess ||
+      hipMalloc(&ib.p, (size_t)n * 4) != hipSuccess ||
+      hipMalloc(&pb.p, (size_t)maxparts * (np + 1) * 4) != hipSuccess ||
       hipMalloc(&mb.p, np * 4) != hipSuccess || hipMalloc(&vb.p, np * 4) != hipSuccess ||
       hipMalloc(&lb.p, (size_t)epochs * 4) != hipSuccess)
     return 9;
@@ -1873,8 +1878,8 @@ int mhd_mlp_train(float* params, const float* x, const float* y, const float* w,
       hipMemcpy(xb.p, x, (size_t)n * mhd::sel::NIN * 4, hipMemcpyHostToDevice) != hipSuccess ||
       hipMemcpy(yb.p, y, (size_t)n * 4, hipMemcpyHostToDevice) != hipSuccess ||
       hipMemcpy(wb.p, w, (size_t)n * 4, hipMemcpyHostToDevice) != hipSuccess ||
-      hipMemset(gb.p, 0, np * 4) != hipSuccess || hipMemset(mb.p, 0, np * 4) != hipSuccess ||
-      hipMemset(vb.p, 0, np * 4) != hipSuccess || hipMemset(lb.p, 0, (size_t)epochs * 4) != hipSuccess)
+      hipMemset(mb.p, 0, np * 4) != hipSuccess || hipMemset(vb.p, 0, np * 4) != hipSuccess ||
+      hipMemset(lb.p, 0, (size_t)epochs * 4) != hipSuccess)
     return 9;
   std::mt19937 rng(seed);
   std::vector<int> idx(n);
@@ -1887,13 +1892,15 @@ int mhd_mlp_train(float* params, const float* x, const float* y, const float* w,
       const int cnt = std::min(batch, n - off);
       double ws = 0;
       for (int q = 0; q < cnt; q++) ws += w[idx[off + q]];
-      // The epoch loss is the mean of the minibatch losses (weighted by count).
+      // The epoch loss is the mean of the minibatch losses.  The gradient
+      // rows of the minibatch's blocks are summed in block order, so
+      // training is the same from run to run.
       if (mhd::sel_mlp_grad((const float*)xb.p, (const float*)yb.p, (const float*)wb.p,
-                            (const int*)ib.p, off, cnt, (float)ws, (const float*)p.p, (float*)gb.p,
-                            (float*)lb.p + ep, nullptr) != hipSuccess)
+                            (const int*)ib.p, off, cnt, (float)ws, (const float*)p.p, (float*)pb.p,
+                            nullptr) != hipSuccess)
         return 9;
-      if (mhd::sel_adam((float*)p.p, (float*)gb.p, (float*)mb.p, (float*)vb.p, ++step, lr,
-                        nullptr) != hipSuccess)
+      if (mhd::sel_adam((float*)p.p, (const float*)pb.p, (cnt + 63) / 64, (float*)mb.p,
+                        (float*)vb.p, ++step, lr, (float*)lb.p + ep, nullptr) != hipSuccess)
         return 9;
     }
   }

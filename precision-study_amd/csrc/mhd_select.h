@@ -32,13 +32,19 @@ hipError_t sel_tile_errors(const double* a, const double* b, size_t plane, int n
 hipError_t sel_mlp_infer(const float* raw, int n, const float* params, const float* mean,
                          const float* istd, float threshold, const unsigned char* sticky,
                          float* prob, unsigned char* flags, hipStream_t s);
-// Adds the minibatch gradient (samples idx[offset .. offset+count)) into grad
-// and the weighted loss into *loss.
+// Gradient of the minibatch (samples idx[offset .. offset+count)) as one
+// row of kParams + 1 floats (gradient, then the weighted loss) per block of
+// 64 samples: partial must hold ceil(count / 64) rows.
 hipError_t sel_mlp_grad(const float* X, const float* Y, const float* W, const int* idx, int offset,
-                        int count, float wsum, const float* params, float* grad, float* loss,
+                        int count, float wsum, const float* params, float* partial,
                         hipStream_t s);
-hipError_t sel_adam(float* params, float* grad, float* m, float* v, int step, float lr,
-                    hipStream_t s);
+// grad = sum of the nparts rows in row order; their loss is added to *loss.
+hipError_t sel_sum_partials(const float* partial, int nparts, float* grad, float* loss,
+                            hipStream_t s);
+// Adam step with the gradient summed from the nparts rows in row order
+// (the loss added to *loss when non-null).
+hipError_t sel_adam(float* params, const float* partial, int nparts, float* m, float* v, int step,
+                    float lr, float* loss, hipStream_t s);
 // Tile lists of the current flags, each in tile order: lists[0 .. counts[0])
 // the flagged (FP64) tiles, lists[2 ntiles .. 2 ntiles + counts[2]) the
 // unflagged (mixed) tiles.

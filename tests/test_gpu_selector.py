@@ -2,9 +2,9 @@
 
   * MLP forward on MFMA (BF16 operands, FP32 accumulation) against the CPU
     oracle's BF16 network (oracle/selector_oracle.py), ragged batch sizes;
-  * MLP gradients (device backward, FP32 atomics) against the oracle's
-    float64 gradients of the same BF16 forward; device training lowers the
-    loss on a separable problem;
+  * MLP gradients (device backward on the FP32 matrix cores) against the
+    oracle's float64 gradients of the same BF16 forward; device training
+    lowers the loss on a separable problem and is bitwise repeatable;
   * tile features and FP32-vs-FP64 tile errors against the oracle;
   * routing: every tile FP64 is bitwise the FP64 form, no tile FP64 is
     bitwise the mixed form, a partial selection takes each tile's stage-1
@@ -107,6 +107,26 @@ def test_mlp_training_reduces_loss(mhd):
     assert losses[-1] < 0.3 * losses[0]
     assert acc > 0.97
     assert np.all(p[so.NFEAT * so.H:so.NIN * so.H] == 0.0)
+
+
+def test_mlp_training_is_deterministic(mhd):
+    """The gradient rows of a minibatch's blocks are summed in block order
+    (no atomics): the same data and seed train the same weights, bit for
+    bit, and the gradients of one batch are the same on every call."""
+    from mhdglm import selector
+
+    rng = np.random.default_rng(8)
+    n = 3000
+    x = padded(rng.standard_normal((n, so.NFEAT)).astype(np.float32))
+    y = (rng.random(n) < 0.2).astype(np.float32)
+    w = np.where(y > 0, 4.0, 1.0).astype(np.float32)
+    runs = [mhd.mlp_train(selector.init_params(2), x, y, w, epochs=4, batch=512, lr=2e-3, seed=5)
+            for _ in range(2)]
+    assert np.array_equal(runs[0][0].view(np.uint32), runs[1][0].view(np.uint32))
+    assert np.array_equal(runs[0][1], runs[1][1])
+    g1, l1 = mhd.mlp_gradients(runs[0][0], x, y, w)
+    g2, l2 = mhd.mlp_gradients(runs[0][0], x, y, w)
+    assert np.array_equal(g1.view(np.uint32), g2.view(np.uint32)) and l1 == l2
 
 
 def mixed_solver(mhd, nx, ny, problem="ot", steps=0, **kw):
