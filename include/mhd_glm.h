@@ -213,18 +213,20 @@ int mhd_time_kernels(mhd_ctx* ctx, double t_end, int nsteps, double* avg_ms);
  * already on the device: `state` (device memory, 9 planes `ld` elements
  * apart, element type as for mhd_set_state) starts at t = 0 and is advanced
  * in place until t >= t_end, a failure, or max_steps steps (< 0: no limit),
- * with the same kernels and results as mhd_advance.  The stage-1 state, the
- * control block and the MIXED form's fallback lists live in a fixed scratch
- * array of 24 MiB per device, reserved when the library's code is loaded on
- * that device and held until the process ends (an FP64 grid of up to about
- * 3.4e5 cells, e.g. 512 x 512); a larger grid is rejected with 1 before any
- * launch.  One stage per launch (cfg.fuse 2 is rejected), no selector, no
- * graphs; MIXED_KAHAN is rejected (its device state is two FP32 planes per
- * variable).  stream is a hipStream_t (NULL: the null stream); the call
- * returns once the run has stopped.  Every call on a device shares that
- * device's scratch array: calls that overlap on one device race.  Returns
- * 0, 1 (invalid arguments, message on stderr), 2/3/4 (halted) or 9, and
- * fills *st if non-NULL. */
+ * with the same kernels and results as mhd_advance without rollback.  The
+ * stage-1 state, the control block and the MIXED form's fallback lists live
+ * in a fixed scratch array of 24 MiB per device, reserved when the
+ * library's code is loaded on that device and held until the process ends
+ * (an FP64 grid of up to about 3.4e5 cells, e.g. 512 x 512); a larger grid
+ * is rejected with 1 before any launch.  One stage per launch (cfg.fuse 2
+ * is rejected), no selector, no graphs, no rollback snapshots (a MIXED run
+ * that halts ends with its code 3/4, as mhd_advance does with
+ * cfg.rollback_steps < 0); MIXED_KAHAN is rejected (its device state is two
+ * FP32 planes per variable).  stream is a hipStream_t (NULL: the null
+ * stream); the call returns once the run has stopped.  Every call on a
+ * device shares that device's scratch array: calls that overlap on one
+ * device race.  Returns 0, 1 (invalid arguments, message on stderr), 2/3/4
+ * (halted) or 9, and fills *st if non-NULL. */
 int mhd_run_noalloc(const mhd_cfg* cfg, void* state, size_t ld, double t_end,
                     long long max_steps, void* stream, mhd_status* st);
 

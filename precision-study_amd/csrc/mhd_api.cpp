@@ -1052,7 +1052,7 @@ extern "C" int mhd_run_noalloc(const mhd_cfg* cfg_in, void* state, size_t ld, do
   if (!cfg_in || !state) return noalloc_reject("null configuration or state");
   mhd_cfg cfg = *cfg_in;
   if (cfg.nx < 2 || cfg.ny < 2 || (long long)cfg.nx * cfg.ny > (long long)INT_MAX)
-    return noalloc_reject("grid must be at least 2 x 2");
+    return noalloc_reject("grid must be at least 2 x 2 and at most INT_MAX cells");
   if (!(cfg.cfl > 0.0 && cfg.cfl < 1.0) || !(cfg.alpha > 0.0))
     return noalloc_reject("cfl must lie in (0,1) and alpha be positive");
   if (cfg.precision == MHD_PREC_MIXED_KAHAN || !launchers_for(cfg.precision))
