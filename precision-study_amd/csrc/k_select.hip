@@ -523,46 +523,66 @@ __global__ void k_adam(float* P, const float* partial, int nparts, float* m, flo
 
 // Tile lists of the current flags, by one block of kRouteThreads, each in
 // tile order: flagged tiles -> lists[0 .. counts[0]) (FP64 tiles), the
-// others -> lists[2 ntiles .. 2 ntiles + counts[2]) (mixed tiles).  Each
-// thread takes a contiguous chunk of tiles: it counts, a block-wide
-// exclusive scan gives its offsets, it writes.  Runs when the flags are
-// refreshed; an audit miss later appends its tile to the FP64 list and sets
-// its flag (the mixed stages skip flagged tiles).
+// others -> lists[2 ntiles .. 2 ntiles + counts[2]) (mixed tiles).  The
+// flags are taken in rounds of kRouteThreads x 16: a thread reads its 16
+// consecutive flags with one 16-byte load (coalesced across the block),
+// counts them, a block-wide exclusive scan gives its offsets in both lists,
+// and it writes its tiles.  Runs when the flags are refreshed; an audit miss
+// later appends its tile to the FP64 list and sets its flag (the mixed
+// stages skip flagged tiles).
 constexpr int kRouteThreads = 1024;
 __global__ __launch_bounds__(kRouteThreads) void k_route(const unsigned char* flags, int ntiles,
                                                          int* lists, int* counts) {
-  constexpr int kWaves = kRouteThreads / 64;
+  constexpr int kWaves = kRouteThreads / 64, kPer = 16, kRound = kRouteThreads * kPer;
   __shared__ int wsum[kWaves];
   const int tid = (int)threadIdx.x, lane = tid & 63, w = tid >> 6;
-  const int chunk = (ntiles + kRouteThreads - 1) / kRouteThreads;
-  const int t0 = min(tid * chunk, ntiles), t1 = min(t0 + chunk, ntiles);
-  int nf = 0;
-  for (int t = t0; t < t1; t++) nf += flags[t] ? 1 : 0;
-  int sf = nf;  // inclusive scan within the wave
+  int done_f = 0;  // FP64 tiles in the earlier rounds (block-uniform)
+  for (int base = 0; base < ntiles; base += kRound) {
+    const int t0 = base + tid * kPer;
+    unsigned char f[kPer];
+    if (t0 + kPer <= ntiles) {
+      // flags is 16-byte aligned (checked by the launcher), t0 a multiple of 16.
+      const uint4 v = *(const uint4*)(flags + t0);
+      const unsigned wv[4] = {v.x, v.y, v.z, v.w};
 #pragma unroll
-  for (int d = 1; d < 64; d <<= 1) {
-    const int u = __shfl_up(sf, d);
-    if (lane >= d) sf += u;
-  }
-  if (lane == 63) wsum[w] = sf;
-  __syncthreads();
-  int bf = 0, tf = 0;
+      for (int k = 0; k < kPer; k++) f[k] = (unsigned char)(wv[k >> 2] >> (8 * (k & 3)));
+    } else {
 #pragma unroll
-  for (int k = 0; k < kWaves; k++) {
-    bf += (k < w) ? wsum[k] : 0;
-    tf += wsum[k];
-  }
-  int pf = bf + sf - nf;  // FP64 tiles before the chunk
-  int pm = t0 - pf;       // mixed tiles before the chunk
-  for (int t = t0; t < t1; t++) {
-    if (flags[t])
-      lists[pf++] = t;
-    else
-      lists[2 * ntiles + pm++] = t;
+      for (int k = 0; k < kPer; k++) f[k] = t0 + k < ntiles ? flags[t0 + k] : 0;
+    }
+    int nf = 0;
+#pragma unroll
+    for (int k = 0; k < kPer; k++) nf += f[k] ? 1 : 0;
+    int sf = nf;  // inclusive scan within the wave
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+      const int u = __shfl_up(sf, d);
+      if (lane >= d) sf += u;
+    }
+    if (lane == 63) wsum[w] = sf;
+    __syncthreads();
+    int bf = 0, tf = 0;
+#pragma unroll
+    for (int k = 0; k < kWaves; k++) {
+      bf += (k < w) ? wsum[k] : 0;
+      tf += wsum[k];
+    }
+    int pf = done_f + bf + sf - nf;  // FP64 tiles before t0
+    int pm = t0 - pf;                // mixed tiles before t0
+#pragma unroll
+    for (int k = 0; k < kPer; k++) {
+      if (t0 + k >= ntiles) break;
+      if (f[k])
+        lists[pf++] = t0 + k;
+      else
+        lists[2 * ntiles + pm++] = t0 + k;
+    }
+    done_f += tf;
+    __syncthreads();  // wsum is rewritten by the next round
   }
   if (tid == 0) {
-    counts[0] = tf;
-    counts[2] = ntiles - tf;
+    counts[0] = done_f;
+    counts[2] = ntiles - done_f;
   }
 }
 
@@ -630,6 +650,7 @@ __global__ void k_copy_words(const unsigned* src, unsigned* dst, int nwords) {
 // ---------------------------------------------------------------------------
 hipError_t sel_tile_features(const double* u, size_t plane, int nx, int ny, int tx, int ty,
                              const void* ctrl_ch, float* feats, hipStream_t s) {
+  if (tx < 1 || ty < 1 || tx * ty > 256) return hipErrorInvalidValue;
   const int tiles_x = (nx + tx - 1) / tx, ntiles = tiles_x * ((ny + ty - 1) / ty);
   hipLaunchKernelGGL(sel::k_tile_features, dim3(ntiles), dim3(256), 0, s, u, plane, nx, ny, tx, ty,
                      tiles_x, ntiles, ctrl_ch, feats);
@@ -701,6 +722,7 @@ hipError_t sel_adam(float* params, const float* partial, int nparts, float* m, f
 
 hipError_t sel_route(const unsigned char* flags, int ntiles, int* lists, int* counts,
                      hipStream_t s) {
+  if ((uintptr_t)flags % 16 != 0) return hipErrorInvalidValue;  // 16-byte flag loads
   hipLaunchKernelGGL(sel::k_route, dim3(1), dim3(sel::kRouteThreads), 0, s, flags, ntiles, lists,
                      counts);
   return hipGetLastError();

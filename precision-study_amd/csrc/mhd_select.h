@@ -15,7 +15,8 @@ constexpr int H = 64;       // hidden width
 constexpr int kParams = NIN * H + H + H * H + H + H + 1;  // FP32 master weights
 }  // namespace sel
 
-// feats: ntiles x NIN raw features of the FP64 state (c_h read from *ctrl_ch).
+// feats: ntiles x NIN raw features of the FP64 state (c_h read from *ctrl_ch),
+// tiles of tx x ty <= 256 cells (hipErrorInvalidValue otherwise).
 hipError_t sel_tile_features(const double* u, size_t plane, int nx, int ny, int tx, int ty,
                              const void* ctrl_ch, float* feats, hipStream_t s);
 // Per-plane sums of squares of an FP64 state (the error scale).
@@ -47,7 +48,8 @@ hipError_t sel_adam(float* params, const float* partial, int nparts, float* m, f
                     float lr, float* loss, hipStream_t s);
 // Tile lists of the current flags, each in tile order: lists[0 .. counts[0])
 // the flagged (FP64) tiles, lists[2 ntiles .. 2 ntiles + counts[2]) the
-// unflagged (mixed) tiles.
+// unflagged (mixed) tiles.  flags must be 16-byte aligned (hipErrorInvalidValue
+// otherwise).
 hipError_t sel_route(const unsigned char* flags, int ntiles, int* lists, int* counts,
                      hipStream_t s);
 // The step's audit sample, a fraction `frac` of the mixed list taken at a

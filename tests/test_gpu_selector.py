@@ -137,15 +137,18 @@ def mixed_solver(mhd, nx, ny, problem="ot", steps=0, **kw):
     return s
 
 
-def test_tile_features_match_oracle(mhd):
-    say("tile features OT 96x80 after 40 steps")
-    s = mixed_solver(mhd, 96, 80, steps=40)
+@pytest.mark.parametrize("nx,ny", [(96, 80), (90, 77)])
+def test_tile_features_match_oracle(mhd, nx, ny):
+    """Partial tiles at the high edges (80 and 77 rows, 90 columns) included:
+    their right and top neighbours wrap around the grid."""
+    say(f"tile features OT {nx}x{ny} after 40 steps")
+    s = mixed_solver(mhd, nx, ny, steps=40)
     f = s.tile_features()
     ch = s.status().ch_last
     u = s.get_state(0)
     s.close()
     ref = so.tile_features(u, 16, 15, ch)
-    assert f.shape == (6 * 6, so.NIN)
+    assert f.shape == (-(-nx // 16) * -(-ny // 15), so.NIN)
     assert np.all(f[:, so.NFEAT:] == 0.0)
     d = np.abs(f[:, :so.NFEAT] - ref) / (1.0 + np.abs(ref))
     say(f"  max scaled feature difference {d.max():.2e}")
@@ -243,11 +246,12 @@ def test_selector_counts_only_live_steps(mhd):
     assert st2.steps == st.steps and b == a
 
 
-def test_selector_partial_routing(mhd):
+@pytest.mark.parametrize("nx,ny", [(96, 80), (2048, 2048)])
+def test_selector_partial_routing(mhd, nx, ny):
     """Stage 1 of one step: each tile's update comes from the FP64 kernel if
     flagged, else from the mixed kernel (flags predicted on the host from
-    the same features and network)."""
-    nx, ny = 96, 80
+    the same features and network).  At 2048^2 the 17,536 tiles take the
+    list builder two rounds."""
     model = random_model(4)
     model["mean"] = np.zeros(so.NFEAT, np.float32)
     s = mixed_solver(mhd, nx, ny)
@@ -264,8 +268,10 @@ def test_selector_partial_routing(mhd):
                        audit_frac=0.0)
     assert st["fp64_tile_steps"] == flags.sum()
     assert not np.array_equal(u64, um)
+    tiles_x = -(-nx // 16)
+    assert flags.size == tiles_x * -(-ny // 15)
     for t in range(flags.size):
-        j0, i0 = (t // 6) * 15, (t % 6) * 16
+        j0, i0 = (t // tiles_x) * 15, (t % tiles_x) * 16
         sl = (slice(None), slice(j0, j0 + 15), slice(i0, i0 + 16))
         want = u64[sl] if flags[t] else um[sl]
         assert np.array_equal(us[sl].view(np.uint64), want.view(np.uint64)), t
