@@ -62,83 +62,112 @@ struct Sum { __device__ double operator()(double a, double b) const { return a +
 
 __device__ __forceinline__ int wrapi(int i, int n) { return i < 0 ? i + n : (i >= n ? i - n : i); }
 
-struct CellPrim {
-  double rho, vx, vy, vz, p, bx, by, bz, b2;
+// What a cell contributes to the features, from its conserved FP64 state:
+// density (floored), |m|^2, field, pressure p = (gamma-1)(E - |m|^2/(2 rho) -
+// |B|^2/2).  Pressure and the neighbour differences stay FP64 (they cancel);
+// the ratios built from them are FP32, the features' own precision.
+struct CellFeat {
+  double rho, m2, p, bx, by, bz;
 };
-__device__ __forceinline__ CellPrim prim_at(const double* u, size_t ps, size_t g) {
+__device__ __forceinline__ CellFeat cell_at(const double* u, size_t ps, size_t g) {
   const double gm1 = 5.0 / 3.0 - 1.0;
-  CellPrim c;
+  CellFeat c;
   c.rho = fmax(u[g], 1e-12);
-  c.vx = u[ps + g] / c.rho;
-  c.vy = u[2 * ps + g] / c.rho;
-  c.vz = u[3 * ps + g] / c.rho;
+  const double mx = u[ps + g], my = u[2 * ps + g], mz = u[3 * ps + g];
+  c.m2 = mx * mx + my * my + mz * mz;
   c.bx = u[5 * ps + g];
   c.by = u[6 * ps + g];
   c.bz = u[7 * ps + g];
-  c.b2 = c.bx * c.bx + c.by * c.by + c.bz * c.bz;
-  c.p = gm1 * (u[4 * ps + g] - 0.5 * c.rho * (c.vx * c.vx + c.vy * c.vy + c.vz * c.vz) - 0.5 * c.b2);
+  const double b2 = c.bx * c.bx + c.by * c.by + c.bz * c.bz;
+  c.p = gm1 * (u[4 * ps + g] - 0.5 * c.m2 / c.rho - 0.5 * b2);
   return c;
 }
 
-// Wave reduction of a float (64 lanes, butterfly).
+// Wave reductions of a float: DPP within each 16-lane row (quad xor 1 and
+// 2, row rotations by 4 and 8), then the four rows' lanes 0, 16, 32, 48 by
+// readlane.  Every lane must be active; the result is wave-uniform.
+template <int CTRL>
+__device__ __forceinline__ float dpp_f(float v) {
+  return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, 0xF, 0xF, false));
+}
 template <class Op>
 __device__ __forceinline__ float wave_reduce(float v, Op op) {
-  for (int off = 32; off > 0; off >>= 1) v = op(v, __shfl_xor(v, off, 64));
-  return v;
+  v = op(v, dpp_f<0xB1>(v));   // quad_perm [1,0,3,2]
+  v = op(v, dpp_f<0x4E>(v));   // quad_perm [2,3,0,1]
+  v = op(v, dpp_f<0x124>(v));  // row_ror:4
+  v = op(v, dpp_f<0x128>(v));  // row_ror:8
+  const float r0 = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 0));
+  const float r1 = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 16));
+  const float r2 = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 32));
+  const float r3 = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 48));
+  return op(op(r0, r1), op(r2, r3));
 }
-struct FMax { __device__ float operator()(float a, float b) const { return a < b ? b : a; } };
-struct FMin { __device__ float operator()(float a, float b) const { return b < a ? b : a; } };
+// Max and min of non-negative floats (every reduced feature quantity is one)
+// as unsigned integers: one VALU instruction, no NaN canonicalisation.
+struct FMax {
+  __device__ float operator()(float a, float b) const {
+    return __uint_as_float(max(__float_as_uint(a), __float_as_uint(b)));
+  }
+};
+struct FMin {
+  __device__ float operator()(float a, float b) const {
+    return __uint_as_float(min(__float_as_uint(a), __float_as_uint(b)));
+  }
+};
 struct FSum { __device__ float operator()(float a, float b) const { return a + b; } };
 
-// One block (256 threads) per tile; thread t handles cell t of the tile.
-// Per-cell quantities are computed in FP64 and reduced in FP32 (the features
-// are FP32): 15 wave reductions, one barrier, thread 0 combines the waves.
+// One block (256 threads) per tile; thread t handles cell t of the tile and
+// reads its right and top neighbours itself (they are mostly L2 hits).  14
+// wave reductions, one barrier, thread 0 combines the waves; the cell count
+// is the tile's live extent.
 __global__ __launch_bounds__(256) void k_tile_features(const double* u, size_t ps, int nx, int ny,
                                                        int tx, int ty, int tiles_x, int ntiles,
                                                        const void* ctrl_ch, float* feats) {
-  constexpr int NQ = 15;
+  constexpr int NQ = 14;
   __shared__ float red[4][NQ];
   const int tile = blockIdx.x;
   if (tile >= ntiles) return;
-  const double ch = *(const double*)ctrl_ch;
+  const float ch = (float)*(const double*)ctrl_ch;
   const int j0 = (tile / tiles_x) * ty, i0 = (tile % tiles_x) * tx;
   const int t = threadIdx.x, li = t % tx, lj = t / tx;
   const int i = i0 + li, j = j0 + lj;
   const bool live = t < tx * ty && i < nx && j < ny;
-  const double gamma = 5.0 / 3.0;
+  const float gamma = 5.0f / 3.0f;
   // q: min beta, max beta, drho, dp, db, mach, psi, |B|, div B, min bn,
-  //    v/ch, cf/ch, kinetic, magnetic, cells
-  float q[NQ] = {3e38f, -3e38f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 3e38f, 0.f, 0.f, 0.f, 0.f, 0.f};
+  //    v/ch, cf/ch, kinetic, magnetic
+  float q[NQ] = {3e38f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 3e38f, 0.f, 0.f, 0.f, 0.f};
   if (live) {
     const size_t g = (size_t)j * nx + i;
-    const CellPrim c = prim_at(u, ps, g);
-    const CellPrim r = prim_at(u, ps, (size_t)j * nx + wrapi(i + 1, nx));
-    const CellPrim tp = prim_at(u, ps, (size_t)wrapi(j + 1, ny) * nx + i);
+    const CellFeat c = cell_at(u, ps, g);
+    const CellFeat r = cell_at(u, ps, (size_t)j * nx + wrapi(i + 1, nx));
+    const CellFeat tp = cell_at(u, ps, (size_t)wrapi(j + 1, ny) * nx + i);
     const double bxl = u[5 * ps + (size_t)j * nx + wrapi(i - 1, nx)];
     const double byd = u[6 * ps + (size_t)wrapi(j - 1, ny) * nx + i];
-    const double pp = fmax(c.p, 1e-12), bmag = sqrt(c.b2);
-    const double beta = fmin(fmax(2.0 * pp / fmax(c.b2, 1e-30), 1e-6), 1e6);
-    q[0] = q[1] = (float)beta;
-    q[2] = (float)(fmax(fabs(r.rho - c.rho), fabs(tp.rho - c.rho)) / c.rho);
-    q[3] = (float)(fmax(fabs(fmax(r.p, 1e-12) - pp), fabs(fmax(tp.p, 1e-12) - pp)) / pp);
-    const double dbr = sqrt((r.bx - c.bx) * (r.bx - c.bx) + (r.by - c.by) * (r.by - c.by) +
-                            (r.bz - c.bz) * (r.bz - c.bz));
-    const double dbt = sqrt((tp.bx - c.bx) * (tp.bx - c.bx) + (tp.by - c.by) * (tp.by - c.by) +
-                            (tp.bz - c.bz) * (tp.bz - c.bz));
-    q[4] = (float)(fmax(dbr, dbt) / fmax(bmag, 1e-12));
-    const double v2 = c.vx * c.vx + c.vy * c.vy + c.vz * c.vz, cs2 = gamma * pp / c.rho;
-    q[5] = (float)fmin(sqrt(v2 / cs2), 100.0);
+    const double pp = fmax(c.p, 1e-12);
+    const float rho = (float)c.rho, p = (float)pp, m2 = (float)c.m2;
+    const float b2 = (float)(c.bx * c.bx + c.by * c.by + c.bz * c.bz), bmag = __builtin_amdgcn_sqrtf(b2);
+    const float beta = fminf(fmaxf(__fdividef(2.0f * p, fmaxf(b2, 1e-30f)), 1e-6f), 1e6f);
+    q[0] = q[1] = beta;
+    q[2] = __fdividef((float)fmax(fabs(r.rho - c.rho), fabs(tp.rho - c.rho)), rho);
+    q[3] = __fdividef((float)fmax(fabs(fmax(r.p, 1e-12) - pp), fabs(fmax(tp.p, 1e-12) - pp)), p);
+    const float dxr = (float)(r.bx - c.bx), dyr = (float)(r.by - c.by), dzr = (float)(r.bz - c.bz);
+    const float dxt = (float)(tp.bx - c.bx), dyt = (float)(tp.by - c.by),
+                dzt = (float)(tp.bz - c.bz);
+    const float dbr = __builtin_amdgcn_sqrtf(dxr * dxr + dyr * dyr + dzr * dzr);
+    const float dbt = __builtin_amdgcn_sqrtf(dxt * dxt + dyt * dyt + dzt * dzt);
+    q[4] = __fdividef(fmaxf(dbr, dbt), fmaxf(bmag, 1e-12f));
+    const float v2 = __fdividef(m2, rho * rho), cs2 = __fdividef(gamma * p, rho);
+    q[5] = fminf(__builtin_amdgcn_sqrtf(__fdividef(v2, cs2)), 100.0f);
     q[6] = (float)fabs(u[8 * ps + g]);
-    q[7] = (float)bmag;
-    const double h = 1.0 / (double)(nx > ny ? nx : ny);
-    const double divb = 0.5 * (r.bx - bxl) * nx + 0.5 * (tp.by - byd) * ny;
-    q[8] = (float)(fabs(divb) * h / (bmag + 1e-12));
-    q[9] = (float)(fmin(fabs(c.bx), fabs(c.by)) / (bmag + 1e-12));
-    q[10] = (float)(sqrt(v2) / ch);
-    q[11] = (float)(sqrt(cs2 + c.b2 / c.rho) / ch);
-    q[12] = (float)(0.5 * c.rho * v2);
-    q[13] = (float)(0.5 * c.b2);
-    q[14] = 1.f;
+    q[7] = bmag;
+    const float h = 1.0f / (float)(nx > ny ? nx : ny);
+    const float divb = (float)(0.5 * (r.bx - bxl) * nx + 0.5 * (tp.by - byd) * ny);
+    q[8] = __fdividef(fabsf(divb) * h, bmag + 1e-12f);
+    q[9] = __fdividef(fminf(fabsf((float)c.bx), fabsf((float)c.by)), bmag + 1e-12f);
+    q[10] = __fdividef(__builtin_amdgcn_sqrtf(v2), ch);
+    q[11] = __fdividef(__builtin_amdgcn_sqrtf(cs2 + __fdividef(b2, rho)), ch);
+    q[12] = 0.5f * __fdividef(m2, rho);
+    q[13] = 0.5f * b2;
   }
   const FMax mx;
   const FMin mn;
@@ -164,7 +193,7 @@ __global__ __launch_bounds__(256) void k_tile_features(const double* u, size_t p
         v = (k == 0 || k == 9) ? mn(v, red[ww][k]) : (k >= 12 ? sm(v, red[ww][k]) : mx(v, red[ww][k]));
       f[k] = v;
     }
-    const float cnt = f[14];
+    const float cnt = (float)(min(tx, nx - i0) * min(ty, ny - j0));
     float* o = feats + (size_t)tile * NIN;
     o[0] = log10f(f[0]);
     o[1] = log10f(f[1]);
