@@ -286,9 +286,11 @@ int mhd_mlp_train(float* params, const float* x, const float* y, const float* w,
 
 /* Ensembles (no reference counterpart: the reference runs one grid per
  * process).  n instances of sizes nx[i] x ny[i] share the configuration in
- * cfg (scheme, precision, tile) and advance together: each step is one
- * persistent kernel per SSPRK2 stage pulling (instance, tile) items from a
- * device work queue, then one control kernel; every instance keeps its own
+ * cfg (scheme, precision, tile) and advance together: each SSPRK2 stage is
+ * one kernel over a device list of the running instances' (instance, tile)
+ * work items, rebuilt on the device at the start of the stage
+ * ($MHD_ENSEMBLE_KERNEL=queue: a persistent grid pulling the items from a
+ * work queue), and each step ends with one control kernel; every instance keeps its own
  * dt, t, c_h, counters and halt state, exactly as a single context would
  * (an instance's result is bitwise that of mhd_advance on its own).  Calls
  * mirror the single-context ones with an instance index. */

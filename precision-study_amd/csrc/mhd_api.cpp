@@ -1261,6 +1261,7 @@ struct mhd_ensemble {
   void* desc = nullptr;        // device descriptors
   int* prefix = nullptr;       // device prefix of tile counts
   unsigned* queue = nullptr;   // device work-queue counters
+  void* live = nullptr;        // device live list (rebuilt at every step)
   int* redo_items = nullptr;   // MIXED FP64 fallback: 2 x total work items
   std::vector<Rollback> rb;    // MIXED rollback, per instance
   mhd::EnsLaunch el;
@@ -1372,6 +1373,7 @@ void mhd_ensemble_destroy(mhd_ensemble* e) {
   if (e->desc) (void)hipFree(e->desc);
   if (e->prefix) (void)hipFree(e->prefix);
   if (e->queue) (void)hipFree(e->queue);
+  if (e->live) (void)hipFree(e->live);
   if (e->redo_items) (void)hipFree(e->redo_items);
   if (e->stream) (void)hipStreamDestroy(e->stream);
   delete e;
@@ -1430,6 +1432,8 @@ int mhd_ensemble_create(const mhd_cfg* cfg, int n, const int* nx, const int* ny,
   if (rc == 0) hipok(hipMalloc((void**)&e->prefix, (size_t)(n + 1) * sizeof(int)), "hipMalloc");
   if (rc == 0)
     hipok(hipMalloc((void**)&e->queue, mhd::kEnsQueueWords * sizeof(unsigned)), "hipMalloc");
+  if (rc == 0) hipok(hipMalloc(&e->live, mhd::ens_live_bytes(n)), "hipMalloc");
+  if (rc == 0) hipok(hipMemset(e->live, 0, mhd::ens_live_bytes(n)), "hipMemset");
   if (rc == 0) {
     std::vector<char> hdesc((size_t)n * e->L->desc_bytes);
     std::vector<int> prefix(n + 1, 0);
@@ -1474,6 +1478,10 @@ int mhd_ensemble_create(const mhd_cfg* cfg, int n, const int* nx, const int* ny,
     e->el.tile = e->cfg.tile;
     e->el.grid_blocks = 0;
     if (const char* g = std::getenv("MHD_PERSISTENT_BLOCKS")) e->el.grid_blocks = std::atoi(g);
+    e->el.live = e->live;
+    e->el.kernel = mhd::ENS_FLAT;
+    if (const char* k = std::getenv("MHD_ENSEMBLE_KERNEL"))
+      if (std::strcmp(k, "queue") == 0) e->el.kernel = mhd::ENS_QUEUE;
     e->el.cus = cus;
     if (rc == 0 && cfg->precision == MHD_PREC_MIXED && cfg->fallback != 1) {
       hipok(hipMalloc((void**)&e->redo_items, 2 * (size_t)total * sizeof(int)), "hipMalloc");

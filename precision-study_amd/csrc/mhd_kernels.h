@@ -125,6 +125,25 @@ inline void tile_dims(int tile, int& tx, int& ty) {
   ty = dims[tile][1];
 }
 
+// How an ensemble's stage kernels take the live work items: one block per
+// item, the grid covering every instance's tiles (blocks past the live
+// items exit at once), or a persistent grid claiming items from a work
+// queue.
+enum EnsKernel { ENS_FLAT = 0, ENS_QUEUE = 1 };
+
+// Device live list of an ensemble: a 64-byte header (live work items, live
+// instances), the first live work item of each live instance (n ints,
+// padded to 64 bytes), then one kEnsLiveEntryBytes entry per live instance
+// (its descriptor, its first item in the fixed numbering over every
+// instance, which the FP64 fallback queue uses, and the control-block values
+// the stage reads).
+constexpr size_t kEnsLiveHeaderBytes = 64;
+constexpr size_t kEnsLiveEntryBytes = 128;
+constexpr size_t ens_live_first_bytes(int n) { return ((size_t)n * sizeof(int) + 63) / 64 * 64; }
+constexpr size_t ens_live_bytes(int n) {
+  return kEnsLiveHeaderBytes + ens_live_first_bytes(n) + (size_t)n * kEnsLiveEntryBytes;
+}
+
 // Device description of an ensemble (type-erased; built by mhd_api.cpp).
 struct EnsLaunch {
   const void* desc;     // device array of per-instance descriptors (desc_bytes each)
@@ -136,6 +155,10 @@ struct EnsLaunch {
   int muscl, hlld, tile;
   int grid_blocks;      // persistent grid; <= 0: occupancy x CUs
   int cus;              // compute units of the device
+  // Live list (ens_live_bytes(n) bytes of device memory), rebuilt at the
+  // start of every stage: the running instances' work items numbered from 0.
+  void* live = nullptr;
+  int kernel = ENS_FLAT;  // EnsKernel: how the stage kernels take the items
   // FP64 fallback of the mixed form (as GridDesc): queued work items (global
   // tile indices) per stage, counts in queue[16 + stage - 1], re-evaluated
   // by `redo` (the FP64 launchers' ens_redo).
@@ -208,7 +231,8 @@ struct Launchers {
   // Fills descriptor i of a host array; returns the instance's tile count.
   int (*ens_fill)(void* host_desc, int i, void* u, void* u1, size_t plane, int nx, int ny,
                   void* ctrl, int tile);
-  // One step of every running instance (persistent stage kernels + finish).
+  // One step of every running instance: the live-list kernel, the two stage
+  // kernels (EnsLaunch::kernel), the FP64 fallback launches, the finish.
   hipError_t (*ens_step)(const EnsLaunch&, hipStream_t, hipEvent_t* ev);
   hipError_t (*ens_set_target)(void* ctrls, int n, double t_end, unsigned* queue, hipStream_t);
   // FP64 re-evaluation of an ensemble's queued items (FP64 launchers only).

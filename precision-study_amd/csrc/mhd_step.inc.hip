@@ -870,10 +870,21 @@ struct StageHold {
 // than HOLD_NONE a thread loads its own tile cell first in phase A, and
 // phase C takes that state from registers or from `sU` instead of reading
 // it again.
+// The control-block values a stage reads, as copied at the start of the
+// stage by an ensemble's live-list kernel (so that a block of the flat
+// ensemble kernel finds them beside its instance's descriptor instead of
+// behind the descriptor's control-block pointer).
+template <class T>
+struct StepVals {
+  T ch, dt, dx, dy;
+  int step, exact_until;
+};
+
 template <class Pol, bool MUSCL, bool HLLD, int STAGE, int TX, int TY, int HOLD = HOLD_NONE>
 __device__ __forceinline__ void stage_tile(const StageArgs<Pol>& a, int tile,
                                            TileSharedOf<Pol, TX, TY>& sh,
-                                           typename Pol::S* sU = nullptr) {
+                                           typename Pol::S* sU = nullptr,
+                                           const StepVals<typename Pol::T>* sv = nullptr) {
   using R = typename Pol::R;
   using RP = typename Pol::RP;
   using RS = typename Pol::RS;
@@ -886,15 +897,20 @@ __device__ __forceinline__ void stage_tile(const StageArgs<Pol>& a, int tile,
   RS* sF = sh.sF;
   unsigned long long* sRed = sh.sRed;
 
+  // The step's values: from the control block, or (sv) as copied there at
+  // the start of the stage.
   Ctrl<T>* ctrl = a.ctrl;
   if constexpr (Pol::kRedo) {
     // Rollback window: the whole tile goes to the FP64 kernel unevaluated.
-    if (a.redo_list && uni(ctrl->step) < uni(ctrl->exact_until)) {
+    const bool window = sv ? uni(sv->step) < uni(sv->exact_until)
+                           : uni(ctrl->step) < uni(ctrl->exact_until);
+    if (a.redo_list && window) {
       if (threadIdx.x == 0) a.redo_list[atomicAdd(a.redo_counter, 1)] = a.redo_base + tile;
       return;
     }
   }
-  const T ch_t = uni(ctrl->ch), dt = uni(ctrl->dt), dx = uni(ctrl->dx), dy = uni(ctrl->dy);
+  const T ch_t = uni(sv ? sv->ch : ctrl->ch), dt = uni(sv ? sv->dt : ctrl->dt);
+  const T dx = uni(sv ? sv->dx : ctrl->dx), dy = uni(sv ? sv->dy : ctrl->dy);
   const R ch = (R)ch_t;
   const Phys<R> c = make_phys<R>(a.tol);
   const Phys<RP> cp = make_phys<RP>(a.tol);  // recovery / reconstruction
@@ -1354,7 +1370,110 @@ struct EnsArgs {
   float lowbeta_ratio;
   int* redo_items[2];   // FP64 fallback queues (null: none)
   int redo_all;
+  void* live;           // live list (ens_live_bytes(n) bytes), rebuilt each stage
 };
+
+// Live list of an ensemble (layout in mhd_kernels.h): header, the first live
+// work item of each live instance, and per live instance its descriptor and
+// its first item in the fixed numbering (the fallback queue's).
+struct EnsLiveHdr {
+  int total;  // live work items
+  int n;      // live instances
+  int pad[14];
+};
+static_assert(sizeof(EnsLiveHdr) == kEnsLiveHeaderBytes, "live list header");
+template <class Pol>
+struct LiveEntry {
+  InstDesc<Pol> d;
+  int first;
+  StepVals<typename Pol::T> v;  // the control block's values for this stage
+};
+
+template <class Pol>
+__device__ __forceinline__ int* live_first(const EnsArgs<Pol>& e) {
+  return (int*)((char*)e.live + kEnsLiveHeaderBytes);
+}
+template <class Pol>
+__device__ __forceinline__ LiveEntry<Pol>* live_entries(const EnsArgs<Pol>& e) {
+  static_assert(sizeof(LiveEntry<Pol>) <= kEnsLiveEntryBytes, "live list entry");
+  return (LiveEntry<Pol>*)((char*)e.live + kEnsLiveHeaderBytes + ens_live_first_bytes(e.n));
+}
+
+// Live slot of live work item `item`: the number of live instances whose
+// first item is <= item, less one.  Each lane holds one instance's first
+// item, 64 instances per round; `first0` is the lane's value of the first
+// round, loaded by the caller (so that it can be in flight with other
+// loads).  Wave-uniform.
+template <class Pol>
+__device__ __forceinline__ int live_slot(const EnsArgs<Pol>& e, int nlive, int first0, int item) {
+  const int lane = (int)(threadIdx.x & 63);
+  int count = (int)__popcll(__ballot(lane < nlive && first0 <= item));
+  for (int c0 = 64; c0 < nlive; c0 += 64) {
+    const int p = c0 + lane < nlive ? live_first(e)[c0 + lane] : INT_MAX;
+    count += (int)__popcll(__ballot(p <= item));
+  }
+  return uni(count - 1);
+}
+
+// Rebuilds the live list at the start of a stage (one block of 256): the
+// instances whose control block says RUN_ACTIVE (and, for stage 2, whose
+// stage 1 left no unusable cell), in instance order, their work items
+// numbered consecutively from 0, with the control-block values the stage
+// reads.
+template <class Pol, int STAGE>
+__global__ __launch_bounds__(256) void k_ens_live(EnsArgs<Pol> e) {
+  __shared__ unsigned long long s_wave[4];
+  __shared__ unsigned long long s_base;
+  const int tid = (int)threadIdx.x, lane = tid & 63, wid = tid >> 6;
+  int* first = live_first(e);
+  LiveEntry<Pol>* ent = live_entries(e);
+  if (tid == 0) s_base = 0;
+  __syncthreads();
+  for (int c0 = 0; c0 < e.n; c0 += 256) {
+    const int i = c0 + tid;
+    // (tiles << 32) | 1 for a live instance: one inclusive scan gives both
+    // its live slot (low word) and its first live item (high word).
+    unsigned long long v = 0;
+    if (i < e.n) {
+      const Ctrl<typename Pol::T>* c = e.desc[i].ctrl;
+      if (c->run == RUN_ACTIVE && (STAGE == 1 || c->bad[0] == INT_MAX))
+        v = ((unsigned long long)(unsigned)e.desc[i].ntiles << 32) | 1ull;
+    }
+    unsigned long long incl = v;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+      const unsigned long long t = __shfl_up(incl, d, 64);
+      if (lane >= d) incl += t;
+    }
+    if (lane == 63) s_wave[wid] = incl;
+    __syncthreads();
+    unsigned long long off = s_base;
+    for (int w = 0; w < wid; w++) off += s_wave[w];
+    if (v) {
+      const unsigned long long excl = off + incl - v;
+      const int slot = (int)(unsigned)(excl & 0xffffffffull);
+      first[slot] = (int)(excl >> 32);
+      ent[slot].d = e.desc[i];
+      ent[slot].first = e.prefix[i];
+      const Ctrl<typename Pol::T>* c = e.desc[i].ctrl;
+      StepVals<typename Pol::T>& sv = ent[slot].v;
+      sv.ch = c->ch;
+      sv.dt = c->dt;
+      sv.dx = c->dx;
+      sv.dy = c->dy;
+      sv.step = c->step;
+      sv.exact_until = c->exact_until;
+    }
+    __syncthreads();
+    if (tid == 255) s_base = off + incl;
+    __syncthreads();
+  }
+  if (tid == 0) {
+    EnsLiveHdr* h = (EnsLiveHdr*)e.live;
+    h->total = (int)(s_base >> 32);
+    h->n = (int)(unsigned)(s_base & 0xffffffffull);
+  }
+}
 
 // Stage arguments of one ensemble instance.
 template <class Pol, int STAGE>
@@ -1397,6 +1516,35 @@ __device__ __forceinline__ int find_instance(const int* prefix, int n, int item)
 }
 
 
+// One stage of every live instance, one block per live work item: the grid
+// covers every instance's tiles; block b takes live item (b mod 8) * per +
+// b / 8 with per = ceil(live items / 8), so each XCD (blocks are dealt to the
+// 8 XCDs round-robin) takes a contiguous run of items, whose halos it then
+// shares in its L2, and blocks past the live items exit after one load.
+template <class Pol, bool MUSCL, bool HLLD, int STAGE, int TX, int TY>
+__global__ __launch_bounds__((TX * TY + 63) / 64 * 64)
+__attribute__((amdgpu_waves_per_eu(StageWaves<Pol>::value))) void k_stage_ens(EnsArgs<Pol> e) {
+  constexpr int hold = StageHold<Pol, TX, TY>::value;
+  __shared__ TileSharedOf<Pol, TX, TY> sh;
+  __shared__ typename Pol::S sU[hold == HOLD_LDS ? 9 * TX * TY : 1];
+  const EnsLiveHdr* hdr = (const EnsLiveHdr*)e.live;
+  const int* first = live_first(e);
+  const int lane = (int)(threadIdx.x & 63);
+  const int first0 = lane < e.n ? first[lane] : INT_MAX;  // in flight with the header
+  const int total = uni(hdr->total), nlive = uni(hdr->n);
+  const int b = (int)blockIdx.x, per = (total + 7) >> 3, k = b >> 3;
+  if (k >= per) return;
+  const int item = (b & 7) * per + k;
+  if (item >= total) return;
+  // The live list holds only running instances (for stage 2, those whose
+  // stage 1 passed) and their step values: no control-block load here.
+  const int slot = live_slot(e, nlive, first0, item);
+  const LiveEntry<Pol>& le = live_entries(e)[slot];
+  stage_tile<Pol, MUSCL, HLLD, STAGE, TX, TY, hold>(inst_args<Pol, STAGE>(e, le.d, uni(le.first)),
+                                                    item - uni(first[slot]), sh, sU, &le.v);
+}
+
+// The same as a persistent grid pulling live work items from a queue.
 template <class Pol, bool MUSCL, bool HLLD, int STAGE, int TX, int TY>
 __global__ __launch_bounds__((TX * TY + 63) / 64 * 64)
 __attribute__((amdgpu_waves_per_eu(StageWaves<Pol>::value))) void k_stage_queue(EnsArgs<Pol> e) {
@@ -1408,22 +1556,27 @@ __attribute__((amdgpu_waves_per_eu(StageWaves<Pol>::value))) void k_stage_queue(
   // one instance): fewer queue atomics per tile at the cost of at most
   // kClaim - 1 tiles of imbalance at the end of a stage.
   constexpr int kClaim = 4;
+  const EnsLiveHdr* hdr = (const EnsLiveHdr*)e.live;
+  const int* first = live_first(e);
+  const int lane = (int)(threadIdx.x & 63);
+  const int first0 = lane < e.n ? first[lane] : INT_MAX;
+  const int total = uni(hdr->total), nlive = uni(hdr->n);
   unsigned* q = e.queue + (STAGE - 1) * 8;
   const int home = (int)(blockIdx.x & 7);
   int chunk = 0;
-  // Item range of the instance of the previous item (block-uniform), so the
-  // binary search over instances runs only when an item leaves it.
-  int cur = 0, cur_lo = 0, cur_hi = 0;
+  // Item range of the live instance of the previous item (block-uniform),
+  // so the slot search runs only when an item leaves it.
+  int slot = 0, cur_lo = 0, cur_hi = 0;
   for (;;) {
     if (threadIdx.x == 0) {
       int item = -1, cnt = 0;
       while (chunk < 8) {
         const int x = (home + chunk) & 7;
-        const int lo = (int)((long long)e.total * x / 8);
-        const int hi = (int)((long long)e.total * (x + 1) / 8);
-        const unsigned k = atomicAdd(&q[x], (unsigned)kClaim);
-        if ((long long)lo + k < (long long)hi) {
-          item = lo + (int)k;
+        const int lo = (int)((long long)total * x / 8);
+        const int hi = (int)((long long)total * (x + 1) / 8);
+        const unsigned kk = atomicAdd(&q[x], (unsigned)kClaim);
+        if ((long long)lo + kk < (long long)hi) {
+          item = lo + (int)kk;
           cnt = hi - item < kClaim ? hi - item : kClaim;
           break;
         }
@@ -1433,22 +1586,23 @@ __attribute__((amdgpu_waves_per_eu(StageWaves<Pol>::value))) void k_stage_queue(
       s_count = cnt;
     }
     __syncthreads();
-    // Block-uniform: readfirstlane keeps the item, the instance lookup and
-    // the descriptor in scalar registers.
+    // Block-uniform: readfirstlane keeps the item, the slot and the
+    // descriptor in scalar registers.
     const int first_item = uni(s_item), count = uni(s_count);
     __syncthreads();
     if (first_item < 0) break;  // every chunk drained: every block gets here
     for (int c = 0; c < count; c++) {
       const int item = first_item + c;
       if (item < cur_lo || item >= cur_hi) {
-        cur = uni(find_instance(e.prefix, e.n, item));
-        cur_lo = uni(e.prefix[cur]);
-        cur_hi = uni(e.prefix[cur + 1]);
+        slot = live_slot(e, nlive, first0, item);
+        cur_lo = uni(first[slot]);
+        cur_hi = slot + 1 < nlive ? uni(first[slot + 1]) : total;
       }
-      const InstDesc<Pol>& d = e.desc[cur];
+      const LiveEntry<Pol>& le = live_entries(e)[slot];
+      const InstDesc<Pol>& d = le.d;
       if (uni(d.ctrl->run) != RUN_ACTIVE) continue;
       if (STAGE == 2 && uni(d.ctrl->bad[0]) != INT_MAX) continue;
-      stage_tile<Pol, MUSCL, HLLD, STAGE, TX, TY, hold>(inst_args<Pol, STAGE>(e, d, cur_lo),
+      stage_tile<Pol, MUSCL, HLLD, STAGE, TX, TY, hold>(inst_args<Pol, STAGE>(e, d, uni(le.first)),
                                                         item - cur_lo, sh, sU);
       __syncthreads();  // LDS is reused by the next item
     }
@@ -1779,6 +1933,7 @@ static EnsArgs<Pol> ens_args(const EnsLaunch& L) {
   e.lowbeta_ratio = (float)L.lowbeta_ratio;
   e.redo_items[0] = e.redo_items[1] = nullptr;
   e.redo_all = 0;
+  e.live = L.live;
   return e;
 }
 
@@ -1829,22 +1984,37 @@ static hipError_t ens_step_t(const EnsLaunch& L, hipStream_t s, hipEvent_t* ev) 
     e.redo_items[1] = L.redo_items[1];
     e.redo_all = L.redo_all;
   }
-  int grid = L.grid_blocks;
-  if (grid <= 0) {
-    static int per_cu = 0;
-    if (!per_cu) {
-      if (hipOccupancyMaxActiveBlocksPerMultiprocessor(
-              &per_cu, reinterpret_cast<const void*>(&k_stage_queue<Pol, MUSCL, HLLD, 1, TX, TY>),
-              nthr, 0) != hipSuccess || per_cu < 1)
-        per_cu = 1;
+  if (!L.live || L.total < 1) return hipErrorInvalidValue;
+  const bool queue = L.kernel == ENS_QUEUE;
+  // Flat: one block per work item of every instance, rounded up to whole
+  // rounds of the 8 XCDs (item x * per + k is block 8k + x).
+  int grid = (L.total + 7) / 8 * 8;
+  if (queue) {
+    grid = L.grid_blocks;
+    if (grid <= 0) {
+      static int per_cu = 0;
+      if (!per_cu) {
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(
+                &per_cu, reinterpret_cast<const void*>(&k_stage_queue<Pol, MUSCL, HLLD, 1, TX, TY>),
+                nthr, 0) != hipSuccess || per_cu < 1)
+          per_cu = 1;
+      }
+      grid = per_cu * (L.cus > 0 ? L.cus : 256);
     }
-    grid = per_cu * (L.cus > 0 ? L.cus : 256);
   }
+  hipLaunchKernelGGL((k_ens_live<Pol, 1>), dim3(1), dim3(256), 0, s, e);
   if (ev) (void)hipEventRecord(ev[0], s);
-  hipLaunchKernelGGL((k_stage_queue<Pol, MUSCL, HLLD, 1, TX, TY>), dim3(grid), dim3(nthr), 0, s, e);
+  if (queue)
+    hipLaunchKernelGGL((k_stage_queue<Pol, MUSCL, HLLD, 1, TX, TY>), dim3(grid), dim3(nthr), 0, s, e);
+  else
+    hipLaunchKernelGGL((k_stage_ens<Pol, MUSCL, HLLD, 1, TX, TY>), dim3(grid), dim3(nthr), 0, s, e);
   if (ev) (void)hipEventRecord(ev[1], s);
   if (redo) (void)L.redo(L, 1, s);
-  hipLaunchKernelGGL((k_stage_queue<Pol, MUSCL, HLLD, 2, TX, TY>), dim3(grid), dim3(nthr), 0, s, e);
+  hipLaunchKernelGGL((k_ens_live<Pol, 2>), dim3(1), dim3(256), 0, s, e);
+  if (queue)
+    hipLaunchKernelGGL((k_stage_queue<Pol, MUSCL, HLLD, 2, TX, TY>), dim3(grid), dim3(nthr), 0, s, e);
+  else
+    hipLaunchKernelGGL((k_stage_ens<Pol, MUSCL, HLLD, 2, TX, TY>), dim3(grid), dim3(nthr), 0, s, e);
   if (redo) (void)L.redo(L, 2, s);
   if (ev) (void)hipEventRecord(ev[2], s);
   hipLaunchKernelGGL(k_finish_multi<T>, dim3((L.n + 63) / 64 > 0 ? (L.n + 63) / 64 : 1), dim3(64),

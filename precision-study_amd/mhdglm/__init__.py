@@ -424,8 +424,9 @@ def _cfg(cfg: "SolverConfig", nx: int = 0, ny: int = 0) -> Cfg:
 
 
 class Ensemble:
-    """Many instances of uneven sizes advanced together (persistent
-    work-queue kernels, per-instance dt) behind the C ABI."""
+    """Many instances of uneven sizes advanced together (one kernel per
+    stage over a device list of the running instances' tiles, per-instance
+    dt) behind the C ABI."""
 
     def __init__(self, cfg: SolverConfig, sizes):
         self.lib = load()

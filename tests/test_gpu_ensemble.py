@@ -1,4 +1,6 @@
-"""GPU tests of ensembles (persistent work-queue stage kernels).
+"""GPU tests of ensembles (stage kernels over a per-stage live list of work
+items: one block per item by default, or a persistent work-queue grid with
+MHD_ENSEMBLE_KERNEL=queue).
 
 Contract: an instance advanced inside an ensemble is bitwise the same
 instance advanced alone by a single context (same form, same tile shape),
@@ -56,6 +58,58 @@ def test_ensemble_matches_single(mhd, form):
         assert np.array_equal(bits(u_e), bits(u_s)), f"instance {i} {nx}x{ny}"
     # Different sizes take different step counts to t_end.
     assert len({s.steps for s in sts}) > 1
+    ens.close()
+
+
+@pytest.mark.parametrize("form", ["fp64", "mixed"])
+def test_queue_kernel_matches_flat(mhd, form, monkeypatch):
+    """The persistent work-queue stage kernels give the flat kernels' states:
+    bitwise in FP64, and in the mixed form within rounding of the flat run
+    (the two kernels are separate instantiations of the stage, whose FMA
+    contraction may fall differently)."""
+    cfg = mhd.SolverConfig(nx=2, ny=2, precision=mhd.PRECISIONS[form])
+    sizes = [(64, 64), (48, 80), (96, 32), (33, 57)]
+    params = [(1.0, 0, 0, 0, 0), (0.9, 0.3, 1.1, 2.0, 5.5), (1.2, 6.0, 0.1, 3.3, 0.7),
+              (1.05, 0.5, 0.5, 0.5, 0.5)]
+    out = {}
+    for kernel in ("flat", "queue"):
+        monkeypatch.setenv("MHD_ENSEMBLE_KERNEL", kernel)
+        say(f"ensemble {form}, {kernel} kernel")
+        ens = mhd.Ensemble(cfg, sizes)
+        for i, p in enumerate(params):
+            ens.init_problem(i, "otv", p)
+        sts = ens.advance(0.03)
+        assert all(st.code == 0 and st.done == 1 for st in sts)
+        out[kernel] = ([st.steps for st in sts], [ens.get_state(i) for i in range(len(sizes))])
+        ens.close()
+    assert out["flat"][0] == out["queue"][0]
+    for a, b in zip(out["flat"][1], out["queue"][1]):
+        if form == "fp64":
+            assert np.array_equal(bits(a), bits(b))
+        else:
+            assert np.max(np.abs(a - b) / (np.abs(a) + 1e-3)) < 1e-9
+
+
+def test_many_instances_live_list(mhd):
+    """300 instances (more than one 256-instance round of the live-list
+    kernel and more than 64 live slots per slot search), finishing at
+    different step counts: each equals its single run, bitwise."""
+    cfg = mhd.SolverConfig(nx=2, ny=2)
+    shapes = [(16, 16), (20, 12), (33, 17), (8, 40)]
+    sizes = [shapes[i % len(shapes)] for i in range(300)]
+    ens = mhd.Ensemble(cfg, sizes)
+    for i in range(len(sizes)):
+        ens.init_problem(i, "otv", (0.8 + 0.001 * i, 0.01 * i, 0.02 * i, 0.03 * i, 0.04 * i))
+    say("ensemble of 300 small instances, FP64, to t=0.02")
+    sts = ens.advance(0.02)
+    assert all(st.code == 0 and st.done == 1 for st in sts)
+    assert len({st.steps for st in sts}) > 1
+    for i in range(0, 300, 37):
+        nx, ny = sizes[i]
+        p = (0.8 + 0.001 * i, 0.01 * i, 0.02 * i, 0.03 * i, 0.04 * i)
+        u_s, st_s = single(mhd, cfg, nx, ny, "otv", p, 0.02)
+        assert sts[i].steps == st_s.steps
+        assert np.array_equal(bits(ens.get_state(i)), bits(u_s)), i
     ens.close()
 
 
