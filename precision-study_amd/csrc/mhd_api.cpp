@@ -161,6 +161,13 @@ void init_ctrl(Ctrl<R>* h, const mhd_cfg& cfg) {
   h->bad[1] = INT_MAX;
   h->fail_cell = -1;
   h->rate_f32 = cfg.precision == MHD_PREC_REF_MIXED ? 1 : 0;
+  // Test hook of the mixed form's rollback: $MHD_FAULT_STEP = s makes stage
+  // 1 of step s (counted from the state set) halt with code 3 at cell 0,
+  // as an FP64-confirmed failure would, unless a rollback window covers it.
+  if (const char* f = std::getenv("MHD_FAULT_STEP")) {
+    const int s = std::atoi(f);
+    if (s >= 0 && s < INT_MAX) h->fail_inject = s + 1;
+  }
 }
 
 template <class R>

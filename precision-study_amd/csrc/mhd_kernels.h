@@ -69,7 +69,10 @@ struct Ctrl {
   // by the host after it restored an earlier snapshot of a failed run);
   // exact_steps counts the completed steps so taken.
   int exact_until;
-  int exact_pad;
+  // Injected failure (test hook of the rollback, $MHD_FAULT_STEP; 0: none):
+  // with fail_inject = s + 1, stage 1 of step s of the mixed form counts
+  // cell 0 as an FP64-confirmed unusable cell, outside rollback windows.
+  int fail_inject;
   unsigned long long exact_steps;
   // Per-block partial maxima of the signal speed, spread over kSpeedSlots
   // addresses (block b uses slot b % kSpeedSlots) so that tens of thousands

@@ -908,6 +908,13 @@ __device__ __forceinline__ void stage_tile(const StageArgs<Pol>& a, int tile,
       if (threadIdx.x == 0) a.redo_list[atomicAdd(a.redo_counter, 1)] = a.redo_base + tile;
       return;
     }
+    // Injected failure (test hook): tile 0 reports cell 0 unusable as the
+    // FP64 fallback would, so the step halts with code 3.
+    if (STAGE == 1 && tile == 0 && a.redo_list) {
+      const int inj = uni(ctrl->fail_inject);
+      const int step = sv ? uni(sv->step) : uni(ctrl->step);
+      if (inj != 0 && inj == step + 1 && threadIdx.x == 0) atomicMin(&ctrl->bad[0], 0);
+    }
   }
   const T ch_t = uni(sv ? sv->ch : ctrl->ch), dt = uni(sv ? sv->dt : ctrl->dt);
   const T dx = uni(sv ? sv->dx : ctrl->dx), dy = uni(sv ? sv->dy : ctrl->dy);

@@ -190,8 +190,10 @@ def test_workload_fp64_instances_match_oracle(mhd, oracle):
 
 def test_workload_mixed_finishes_every_instance(mhd):
     """The mixed ensemble of the 64-instance workload to t = 0.5: all 64
-    instances finish (instance 16 through the FP64 rollback), and that
-    instance ends at the mixed form's distance from its FP64 run."""
+    instances finish (any whose trajectory drifts into a state the FP64
+    tile fallback cannot advance through an FP64 rollback window, as
+    instance 16 once did), none by a restart, and instance 16 ends at the
+    mixed form's distance from its FP64 run."""
     from mhdglm import compare
 
     say("64-instance workload, mixed, to t=0.5")
@@ -201,7 +203,7 @@ def test_workload_mixed_finishes_every_instance(mhd):
     say(f"  done {sum(st.done for st in sts)}/64, rolled back {rolled}, "
         f"restarts {sum(st.restarts for st in sts)}")
     assert all(st.code == 0 and st.done == 1 for st in sts)
-    assert 16 in rolled and all(st.restarts == 0 for st in sts)
+    assert all(st.restarts == 0 for st in sts)
     u16 = ens.get_state(16)
     ens.close()
     n = sizes[16]

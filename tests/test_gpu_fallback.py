@@ -8,11 +8,14 @@ nothing and is re-evaluated by the FP64 kernel; only an FP64 failure halts.
   * a low-pressure, high-Mach contact on which the FP32 flux breaks
     positivity within a few steps: with the fallback the mixed form runs on,
     close to FP64;
-  * an Orszag-Tang variant whose mixed trajectory drifts (by t ~ 0.42) into
-    a state from which even the FP64 tile fallback fails: the rollback
-    re-runs an earlier window in FP64 and the run finishes, for a single grid
-    and inside an ensemble alike (without it the run halts, as the other
-    forms do);
+  * a mixed run that halts with an FP64-confirmed unusable cell (injected
+    at a given step through the MHD_FAULT_STEP test hook, as a trajectory
+    that drifted into a state the FP64 tile fallback cannot advance would):
+    the rollback re-runs an earlier window in FP64 and the run finishes, for
+    a single grid and inside an ensemble alike (without it the run halts, as
+    the other forms do);
+  * the Orszag-Tang variant on which such a drift was first seen finishes in
+    the mixed form, close to FP64;
   * a state that the FP64 form cannot advance either: the rollback ends in a
     restart that is the FP64 run, bitwise, halt included.
 """
@@ -23,7 +26,8 @@ import pytest
 
 pytestmark = pytest.mark.gpu
 
-# Ensemble-workload instance (seed 2026, index 16) that halts in FP32 flux.
+# Ensemble-workload instance (seed 2026, index 16) whose mixed trajectory
+# drifted (with the build of that time) into a state that halted in FP64.
 OTV_HARD = (1.067437115364036, 0.8064116573994691, 1.759240013263218, 0.20557563316160354,
             0.5272182398647961)
 
@@ -138,44 +142,83 @@ def run_rb(mhd, n, precision, rollback_steps=0, fallback=0, t_end=0.5, steps=-1,
     return u, st
 
 
-def test_rollback_rescues_trajectory_divergence(mhd):
-    """An Orszag-Tang variant (ensemble instance 16 of seed 2026) at 1024^2:
-    FP64 finishes; the mixed trajectory drifts and by t ~ 0.42 reaches a state
-    from which even the FP64 tile fallback is not physical.  Without the
-    rollback it halts with the FP64 check's code 3 (the reference's
-    contract); with it the run returns to a snapshot 256-512 steps earlier,
-    re-runs that window in FP64 and finishes mixed, at the mixed form's
-    distance from FP64.  Inside an ensemble the instance does the same,
-    bitwise."""
+def test_rollback_rescues_injected_failure(mhd, monkeypatch):
+    """Orszag-Tang 256^2 in the mixed form, with stage 1 of step 1000 made to
+    report cell 0 unusable as the FP64 fallback would (MHD_FAULT_STEP):
+    without the rollback the run halts there with code 3 (the reference's
+    contract); with it the run returns to the newest snapshot at least 256
+    steps earlier, re-runs from there in FP64 until 256 steps past the
+    failure (the injection does not fire inside the window), then finishes
+    mixed, at the mixed form's distance from FP64.  Inside an ensemble both
+    instances (each injected at its step 1000) do the same, bitwise."""
+    from mhdglm import compare
+
+    n, k = 256, 1000
+
+    def ot(prec, rollback_steps=0):
+        s = mhd.Solver(mhd.SolverConfig(nx=n, ny=n, precision=prec, rollback_steps=rollback_steps))
+        s.init_problem("ot")
+        st = s.advance(0.5)
+        u = s.get_state(0)
+        s.close()
+        return u, st
+
+    u64, s64 = ot(mhd.PREC_FP64)
+    assert s64.code == 0 and s64.done == 1
+    monkeypatch.setenv("MHD_FAULT_STEP", str(k))
+    _, f64 = ot(mhd.PREC_FP64)
+    assert f64.code == 0 and f64.steps == s64.steps  # the hook is the mixed form's only
+    _, off = ot(mhd.PREC_MIXED, rollback_steps=-1)
+    say(f"  rollback off: code {off.code} step {off.fail_step} cell ({off.bad_i},{off.bad_j})")
+    assert (off.code, off.fail_step, off.bad_i, off.bad_j, off.steps) == (3, k, 0, 0, k)
+    um, sm = ot(mhd.PREC_MIXED)
+    errs = compare.rel_l2_state(u64, um, ("rho", "p", "|B|"))
+    say(f"  rollback on: code {sm.code} done {sm.done} steps {sm.steps} rollbacks {sm.rollbacks} "
+        f"restarts {sm.restarts} FP64 steps {sm.exact_steps} relL2 {errs}")
+    assert sm.code == 0 and sm.done == 1
+    assert sm.rollbacks == 1 and sm.restarts == 0
+    assert 256 < sm.exact_steps <= 3 * 256
+    assert max(errs.values()) < 1e-3
+    cfg = mhd.SolverConfig(nx=2, ny=2, precision=mhd.PREC_MIXED)
+    ens = mhd.Ensemble(cfg, [(128, 128), (n, n)])
+    for i in range(2):
+        ens.init_problem(i, "ot")
+    sts = ens.advance(0.5)
+    ue = ens.get_state(1)
+    u128 = ens.get_state(0)
+    ens.close()
+    s = mhd.Solver(mhd.SolverConfig(nx=128, ny=128, precision=mhd.PREC_MIXED))
+    s.init_problem("ot")
+    st128 = s.advance(0.5)
+    u128s = s.get_state(0)
+    s.close()
+    assert st128.rollbacks == 1 and st128.code == 0 and st128.done == 1
+    for got, ref, st in ((sts[0], st128, u128), (sts[1], sm, ue)):
+        assert got.code == 0 and got.done == 1
+        assert (got.rollbacks, got.exact_steps, got.steps) == (ref.rollbacks, ref.exact_steps,
+                                                              ref.steps)
+    assert np.array_equal(ue.view(np.uint64), um.view(np.uint64))
+    assert np.array_equal(u128.view(np.uint64), u128s.view(np.uint64))
+
+
+def test_hard_variant_finishes_mixed(mhd):
+    """The Orszag-Tang variant of the ensemble workload (seed 2026, index
+    16) at 1024^2, on which a mixed trajectory was seen to drift into a
+    state the FP64 tile fallback could not advance: the mixed form finishes
+    it (through a rollback window if it drifts so again) at the mixed form's
+    distance from its FP64 run."""
     from mhdglm import compare
 
     n = 1024
     say("hard OT variant 1024^2")
     u64, s64 = run_rb(mhd, n, mhd.PREC_FP64)
     assert s64.code == 0 and s64.done == 1
-    _, off = run_rb(mhd, n, mhd.PREC_MIXED, rollback_steps=-1)
-    say(f"  rollback off: code {off.code} step {off.fail_step} fallback tiles {off.fallback_tiles}")
-    assert off.code == 3 and off.fallback_tiles >= 1
     um, sm = run_rb(mhd, n, mhd.PREC_MIXED)
     errs = compare.rel_l2_state(u64, um, ("rho", "p", "|B|"))
-    say(f"  rollback on: code {sm.code} done {sm.done} steps {sm.steps} rollbacks {sm.rollbacks} "
-        f"restarts {sm.restarts} FP64 steps {sm.exact_steps} relL2 {errs}")
-    assert sm.code == 0 and sm.done == 1
-    assert sm.rollbacks >= 1 and sm.restarts == 0
-    assert 256 < sm.exact_steps <= sm.rollbacks * 3 * 256
+    say(f"  mixed: code {sm.code} done {sm.done} rollbacks {sm.rollbacks} restarts {sm.restarts} "
+        f"relL2 {errs}")
+    assert sm.code == 0 and sm.done == 1 and sm.restarts == 0
     assert max(errs.values()) < 1e-3
-    cfg = mhd.SolverConfig(nx=2, ny=2, precision=mhd.PREC_MIXED)
-    ens = mhd.Ensemble(cfg, [(128, 128), (n, n)])
-    ens.init_problem(0, "ot")
-    ens.init_problem(1, "otv", OTV_HARD)
-    sts = ens.advance(0.5)
-    ue = ens.get_state(1)
-    ens.close()
-    assert sts[0].code == 0 and sts[0].done == 1 and sts[0].rollbacks == 0
-    assert sts[1].code == 0 and sts[1].done == 1
-    assert (sts[1].rollbacks, sts[1].exact_steps, sts[1].steps) == (sm.rollbacks, sm.exact_steps,
-                                                                    sm.steps)
-    assert np.array_equal(ue.view(np.uint64), um.view(np.uint64))
 
 
 def marginal_state(mhd, oracle, n=32):
