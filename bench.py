@@ -37,8 +37,9 @@ reported on the same JSON line:
                   speedup and the selector's FP64 tile share;
   ensemble     -- the batched workload: 64 Orszag-Tang instances (seeded
                   sizes in {128..1024}^2, B amplitude and sine phases varied)
-                  advanced together to t=0.5 by the persistent work-queue
-                  kernels, each with its own dt; aggregate cell-updates/s,
+                  advanced together to t=0.5 by the ensemble stage kernels
+                  (one launch per stage over the device list of the running
+                  instances' tiles), each with its own dt; aggregate cell-updates/s,
                   instances finished and FP64 rollbacks;
   large        -- FP64 and mixed steps at 8192^2 (4.8 GB per FP64 state);
   cpu_baseline -- the reference CPU build (oracle/_ref, serial, one core) on
@@ -49,6 +50,8 @@ Optional sections (off by default; their results are kept in profiles/):
   --kh-n N              Kelvin-Helmholtz (the 1024^2 variant) to t=5;
   --selector-n N        Orszag-Tang N^2 whole runs, FP64 vs selector;
   --selector-breakdown N  the selector's cost per step, by part.
+Profiling mode: --whole-run FORM runs only one whole Orszag-Tang --n^2 run
+to t=0.5 in FORM ("selector" included) and prints its wall time and rate.
 """
 import argparse
 import json
@@ -377,6 +380,9 @@ def main():
     ap.add_argument("--time-budget", type=float, default=430.0,
                     help="seconds this process may run: the gate runs last and leaves out its "
                          "FP64-FMA floor run when that would end past the budget (0: no limit)")
+    ap.add_argument("--whole-run", default="",
+                    help="only run one whole Orszag-Tang --n^2 run to t=0.5 in this form and print "
+                         "its wall time, steps and rate (for profiling the form's kernels)")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="only run a plain device copy of this many MiB (HBM counter calibration)")
     args = ap.parse_args()
@@ -390,6 +396,15 @@ def main():
             b.copy_(a)
         torch.cuda.synchronize()
         print(json.dumps({"copy_probe_bytes_each_way": n * 8, "launches": 3}), flush=True)
+        return
+
+    if args.whole_run:
+        import mhdglm
+        u, st, wall, sel = whole_run(mhdglm, args.whole_run, args.n)
+        print(json.dumps({"whole_run": args.whole_run, "n": args.n, "steps": st.steps,
+                          "code": st.code, "wall_s": round(wall, 3),
+                          "cell_updates_per_s": round(args.n * args.n * st.steps / wall, 1),
+                          "selector": sel}), flush=True)
         return
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
