@@ -116,15 +116,21 @@ struct FMin {
 };
 struct FSum { __device__ float operator()(float a, float b) const { return a + b; } };
 
-// One block (256 threads) per tile; thread t handles cell t of the tile and
-// reads its right and top neighbours itself (they are mostly L2 hits).  14
-// wave reductions, one barrier, thread 0 combines the waves; the cell count
-// is the tile's live extent.
+// The features' LDS holds a tile plus one column and one row: (tx+1)(ty+1)
+// - 1 cells, at most 33 x 9 - 1 over the stage kernels' tile shapes (32x8).
+constexpr int kFeatMaxExt = 33 * 9;
+
+// One block (256 threads) per tile.  First every cell of the tile and of
+// the column and row beyond it (its cells' right and top neighbours) forms
+// its FP64 quantities once, into LDS; then thread t takes cell t of the
+// tile and its neighbours from there.  14 wave reductions, one barrier,
+// thread 0 combines the waves; the cell count is the tile's live extent.
 __global__ __launch_bounds__(256) void k_tile_features(const double* u, size_t ps, int nx, int ny,
                                                        int tx, int ty, int tiles_x, int ntiles,
                                                        const void* ctrl_ch, float* feats) {
   constexpr int NQ = 14;
   __shared__ float red[4][NQ];
+  __shared__ double sf[6][kFeatMaxExt];  // rho, m2, p, bx, by, bz
   const int tile = blockIdx.x;
   if (tile >= ntiles) return;
   const float ch = (float)*(const double*)ctrl_ch;
@@ -133,14 +139,41 @@ __global__ __launch_bounds__(256) void k_tile_features(const double* u, size_t p
   const int i = i0 + li, j = j0 + lj;
   const bool live = t < tx * ty && i < nx && j < ny;
   const float gamma = 5.0f / 3.0f;
+  // The (tx+1) x (ty+1) box at (i0, j0), less its far corner: cells i0+le,
+  // j0+je that some live cell of the tile reads (le <= nx - i0 and je <=
+  // ny - j0, so one wrap reaches them).
+  const int ex = tx + 1, ne = ex * (ty + 1) - 1;
+  for (int e = t; e < ne; e += 256) {
+    const int je = e / ex, le = e - je * ex;
+    if (i0 + le > nx || j0 + je > ny) continue;
+    const CellFeat c = cell_at(u, ps, (size_t)wrapi(j0 + je, ny) * nx + wrapi(i0 + le, nx));
+    sf[0][e] = c.rho;
+    sf[1][e] = c.m2;
+    sf[2][e] = c.p;
+    sf[3][e] = c.bx;
+    sf[4][e] = c.by;
+    sf[5][e] = c.bz;
+  }
+  __syncthreads();
+  auto feat = [&](int e) {
+    CellFeat c;
+    c.rho = sf[0][e];
+    c.m2 = sf[1][e];
+    c.p = sf[2][e];
+    c.bx = sf[3][e];
+    c.by = sf[4][e];
+    c.bz = sf[5][e];
+    return c;
+  };
   // q: min beta, max beta, drho, dp, db, mach, psi, |B|, div B, min bn,
   //    v/ch, cf/ch, kinetic, magnetic
   float q[NQ] = {3e38f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 3e38f, 0.f, 0.f, 0.f, 0.f};
   if (live) {
     const size_t g = (size_t)j * nx + i;
-    const CellFeat c = cell_at(u, ps, g);
-    const CellFeat r = cell_at(u, ps, (size_t)j * nx + wrapi(i + 1, nx));
-    const CellFeat tp = cell_at(u, ps, (size_t)wrapi(j + 1, ny) * nx + i);
+    const int e = lj * ex + li;
+    const CellFeat c = feat(e);
+    const CellFeat r = feat(e + 1);
+    const CellFeat tp = feat(e + ex);
     const double bxl = u[5 * ps + (size_t)j * nx + wrapi(i - 1, nx)];
     const double byd = u[6 * ps + (size_t)wrapi(j - 1, ny) * nx + i];
     const double pp = fmax(c.p, 1e-12);
@@ -679,7 +712,8 @@ __global__ void k_copy_words(const unsigned* src, unsigned* dst, int nwords) {
 // ---------------------------------------------------------------------------
 hipError_t sel_tile_features(const double* u, size_t plane, int nx, int ny, int tx, int ty,
                              const void* ctrl_ch, float* feats, hipStream_t s) {
-  if (tx < 1 || ty < 1 || tx * ty > 256) return hipErrorInvalidValue;
+  if (tx < 1 || ty < 1 || tx * ty > 256 || (tx + 1) * (ty + 1) - 1 > sel::kFeatMaxExt)
+    return hipErrorInvalidValue;
   const int tiles_x = (nx + tx - 1) / tx, ntiles = tiles_x * ((ny + ty - 1) / ty);
   hipLaunchKernelGGL(sel::k_tile_features, dim3(ntiles), dim3(256), 0, s, u, plane, nx, ny, tx, ty,
                      tiles_x, ntiles, ctrl_ch, feats);

@@ -137,18 +137,21 @@ def mixed_solver(mhd, nx, ny, problem="ot", steps=0, **kw):
     return s
 
 
-@pytest.mark.parametrize("nx,ny", [(96, 80), (90, 77)])
-def test_tile_features_match_oracle(mhd, nx, ny):
+@pytest.mark.parametrize("nx,ny,tile", [(96, 80, 3), (90, 77, 3), (90, 77, 1), (12, 10, 3)])
+def test_tile_features_match_oracle(mhd, nx, ny, tile):
     """Partial tiles at the high edges (80 and 77 rows, 90 columns) included:
-    their right and top neighbours wrap around the grid."""
-    say(f"tile features OT {nx}x{ny} after 40 steps")
-    s = mixed_solver(mhd, nx, ny, steps=40)
+    their right and top neighbours wrap around the grid; 16x15 and 32x8 tiles
+    (the largest box of a tile and its neighbour row and column), and a grid
+    smaller than one tile."""
+    tx, ty = {1: (32, 8), 3: (16, 15)}[tile]
+    say(f"tile features OT {nx}x{ny}, {tx}x{ty} tiles, after 40 steps")
+    s = mixed_solver(mhd, nx, ny, steps=40, tile=tile)
     f = s.tile_features()
     ch = s.status().ch_last
     u = s.get_state(0)
     s.close()
-    ref = so.tile_features(u, 16, 15, ch)
-    assert f.shape == (-(-nx // 16) * -(-ny // 15), so.NIN)
+    ref = so.tile_features(u, tx, ty, ch)
+    assert f.shape == (-(-nx // tx) * -(-ny // ty), so.NIN)
     assert np.all(f[:, so.NFEAT:] == 0.0)
     d = np.abs(f[:, :so.NFEAT] - ref) / (1.0 + np.abs(ref))
     say(f"  max scaled feature difference {d.max():.2e}")
