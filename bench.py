@@ -44,7 +44,8 @@ reported on the same JSON line:
   large        -- FP64 and mixed steps at 8192^2 (4.8 GB per FP64 state);
   cpu_baseline -- the reference CPU build (oracle/_ref, serial, one core) on
                   a bounded sample of Orszag-Tang at 512^2 (the first ~50
-                  steps).
+                  steps); `parallel`: the aggregate rate of min(visible
+                  CPUs, 16) such runs at once.
 Optional sections (off by default; their results are kept in profiles/):
   --accuracy-n N        whole runs of every form at N^2 against FP64;
   --kh-n N              Kelvin-Helmholtz (the 1024^2 variant) to t=5;
@@ -87,9 +88,11 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-def cpu_baseline(n: int, t_end: float):
+def cpu_baseline(n: int, t_end: float, parallel: int = 0):
     """Reference CPU build (or the oracle if the reference build is absent),
-    OT n^2 FP64 to t_end (a bounded sample of the whole run), one thread."""
+    OT n^2 FP64 to t_end (a bounded sample of the whole run), one thread;
+    then, with parallel > 1, that many independent single-thread runs at
+    once (the host's aggregate throughput on the same build)."""
     ref = os.path.join(ROOT, "oracle", "_ref", "mhd2d_glm_cpu_fp64")
     kind = "reference"
     if not os.path.exists(ref):
@@ -97,17 +100,18 @@ def cpu_baseline(n: int, t_end: float):
         kind = "port"
     if not os.path.exists(ref):
         return None
+    argv = [ref, str(n), str(n), str(t_end), "0.25", "muscl", "hlld", "0.18"]
+    pat = re.compile(r"Time loop wall time = (\S+) s \((\S+) Mcell-steps/s\)")
     with tempfile.TemporaryDirectory() as d:
         t0 = time.perf_counter()
-        r = subprocess.run([ref, str(n), str(n), str(t_end), "0.25", "muscl", "hlld", "0.18"],
-                           env=dict(os.environ, OUTDIR=d), capture_output=True, text=True,
+        r = subprocess.run(argv, env=dict(os.environ, OUTDIR=d), capture_output=True, text=True,
                            timeout=600)
         wall = time.perf_counter() - t0
     if r.returncode != 0:
         return None
-    m = re.search(r"Time loop wall time = (\S+) s \((\S+) Mcell-steps/s\)", r.stdout)
+    m = pat.search(r.stdout)
     steps = int(re.search(r"Done\. Steps: (\d+)", r.stdout).group(1))
-    return {
+    out = {
         "value": float(m.group(2)) * 1e6,
         "unit": "cell-updates/s",
         "cores": 1,
@@ -118,6 +122,28 @@ def cpu_baseline(n: int, t_end: float):
         "host_cpu": host_cpu_model(),
         "host_cpus_visible": os.cpu_count(),
     }
+    if parallel > 1:
+        # k copies of the same run at once, each in its own output folder:
+        # the aggregate of their loop rates (n^2 * steps / loop wall each).
+        with tempfile.TemporaryDirectory() as d:
+            procs = []
+            for k in range(parallel):
+                od = os.path.join(d, str(k))
+                os.makedirs(od)
+                procs.append(subprocess.Popen(argv, env=dict(os.environ, OUTDIR=od),
+                                              stdout=subprocess.PIPE, stderr=subprocess.DEVNULL,
+                                              text=True))
+            rates = []
+            for p in procs:
+                so, _ = p.communicate(timeout=900)
+                mm = pat.search(so or "")
+                if p.returncode == 0 and mm:
+                    rates.append(float(mm.group(2)) * 1e6)
+        if len(rates) == parallel:
+            out["parallel"] = {"runs": parallel, "value": round(sum(rates), 1),
+                               "per_run_min_max": [round(min(rates), 1), round(max(rates), 1)],
+                               "threading": f"{parallel} independent single-thread runs at once"}
+    return out
 
 
 def host_cpu_model() -> str:
@@ -353,6 +379,9 @@ def main():
                     help="SSPRK2 stages per launch: 1, 2 (16x15 tiles), 0 = the autotuned choice")
     ap.add_argument("--cpu-n", type=int, default=512)
     ap.add_argument("--cpu-t-end", type=float, default=0.006)
+    ap.add_argument("--cpu-parallel", type=int, default=-1,
+                    help="independent single-thread CPU runs at once for the aggregate figure "
+                         "(-1: min(visible CPUs, 16); 0 or 1: none)")
     ap.add_argument("--large-n", type=int, default=8192,
                     help="also time FP64 and mixed steps at this size (0 to skip)")
     ap.add_argument("--no-cpu", action="store_true")
@@ -526,7 +555,8 @@ def main():
         for form in ("fp64", "mixed"):
             large[form] = round(time_form(mhdglm, torch, form, args.large_n, 20, 5)["value"], 1)
         large["mixed_vs_fp64_speedup"] = round(large["mixed"] / large["fp64"], 3)
-    cpu = None if args.no_cpu else cpu_baseline(args.cpu_n, args.cpu_t_end)
+    npar = args.cpu_parallel if args.cpu_parallel >= 0 else min(os.cpu_count() or 1, 16)
+    cpu = None if args.no_cpu else cpu_baseline(args.cpu_n, args.cpu_t_end, npar)
     # The headline's whole runs last: the longest section, and the only one
     # that shortens itself to the time budget.
     gate = None
