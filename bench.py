@@ -41,7 +41,8 @@ reported on the same JSON line:
                   (one launch per stage over the device list of the running
                   instances' tiles), each with its own dt; aggregate cell-updates/s,
                   instances finished and FP64 rollbacks;
-  large        -- FP64 and mixed steps at 8192^2 (4.8 GB per FP64 state);
+  large        -- FP64, mixed and mixed + selector steps at 8192^2 (4.8 GB
+                  per FP64 state);
   cpu_baseline -- the reference CPU build (oracle/_ref, serial, one core) on
                   a bounded sample of Orszag-Tang at 512^2 (the first ~50
                   steps); `parallel`: the aggregate rate of min(visible
@@ -552,9 +553,10 @@ def main():
     large = None
     if args.large_n:
         large = {"n": args.large_n, "steps": 20, "warmup": 5}
-        for form in ("fp64", "mixed"):
+        for form in ("fp64", "mixed", "selector"):
             large[form] = round(time_form(mhdglm, torch, form, args.large_n, 20, 5)["value"], 1)
         large["mixed_vs_fp64_speedup"] = round(large["mixed"] / large["fp64"], 3)
+        large["selector_vs_fp64_speedup"] = round(large["selector"] / large["fp64"], 3)
     npar = args.cpu_parallel if args.cpu_parallel >= 0 else min(os.cpu_count() or 1, 16)
     cpu = None if args.no_cpu else cpu_baseline(args.cpu_n, args.cpu_t_end, npar)
     # The headline's whole runs last: the longest section, and the only one

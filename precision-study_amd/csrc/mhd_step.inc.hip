@@ -1349,11 +1349,12 @@ __global__ void k_finish(Ctrl<T>* ctrl) {
 
 // ---------------------------------------------------------------------------
 // Ensembles: many instances of uneven sizes, each with its own dt and
-// control block, advanced by one persistent kernel per stage that pulls
-// (instance, tile) items from a work queue.  The queue is split into 8
-// contiguous chunks of the item list, one per XCD (blocks are dealt to the
-// XCDs round-robin), so neighbouring tiles share an XCD's L2; a block that
-// drains its chunk steals from the others.
+// control block.  At the start of each stage one block lists the running
+// instances' (instance, tile) work items (the live list); the stage kernel
+// then takes them one block per item (k_stage_ens), each XCD a contiguous
+// run of items so that neighbouring tiles share its L2, or, as a persistent
+// grid, pulls them from a work queue split into 8 such runs (k_stage_queue:
+// a block that drains its run steals from the others).
 // ---------------------------------------------------------------------------
 template <class Pol>
 struct InstDesc {

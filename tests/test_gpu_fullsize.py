@@ -1,6 +1,7 @@
 """GPU tests of the chosen form at the sizes the headline is quoted on:
 Orszag-Tang 4096^2 in the mixed form with the learned tile selector (HIP
-graphs of 32 steps), and 8192^2 (4.8 GB per FP64 state) in FP64 and mixed.
+graphs of 32 steps), and 8192^2 (4.8 GB per FP64 state) in FP64, mixed and
+mixed with the selector.
 Size-independent properties only (the CPU oracle cannot run these sizes):
   * determinism: two runs are bitwise identical;
   * conservation: the periodic flux form keeps the sums of rho, momentum,
@@ -89,4 +90,15 @@ def test_8192_fp64_and_mixed(mhd):
         conserved(u0, out[name])
     errs = compare.rel_l2_state(out["fp64"], out["mixed"], ("rho", "p", "|B|"))
     say(f"  mixed vs FP64 after {steps} steps: {errs}")
+    assert max(errs.values()) < 1e-6
+    # The chosen form at this size: mixed + learned selector (graph-batched).
+    say(f"OT {n}^2 mixed + selector, {steps} steps")
+    u0s, us, st, stats, tiles = selected_run(mhd, n, steps)
+    assert st.code == 0 and st.steps == steps and stats["steps"] == steps
+    assert np.array_equal(u0s.view(np.uint64), u0.view(np.uint64))
+    conserved(u0, us)
+    mixed_tile_steps = steps * tiles - stats["fp64_tile_steps"]
+    assert abs(stats["audits"] - 0.01 * mixed_tile_steps) <= 2 * steps
+    errs = compare.rel_l2_state(out["fp64"], us, ("rho", "p", "|B|"))
+    say(f"  selector vs FP64 after {steps} steps: {errs}; selector {stats}")
     assert max(errs.values()) < 1e-6
