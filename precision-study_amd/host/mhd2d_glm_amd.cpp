@@ -21,6 +21,7 @@
 //
 // This program only talks to the solver through include/mhd_glm.h.
 #include "mhd_glm.h"
+#include "npz_reader.hpp"
 #include "mhd_problems.hpp"
 
 #include <chrono>
@@ -29,6 +30,7 @@
 #include <cstring>
 #include <fstream>
 #include <iomanip>
+#include <map>
 #include <iostream>
 #include <limits>
 #include <sstream>
@@ -163,6 +165,40 @@ int run(const Args& a, int precision, const char* prec_name, double tol) {
   if (const char* s = std::getenv("MHD_FALLBACK")) cfg.fallback = std::atoi(s);
   if (const char* s = std::getenv("MHD_FUSE")) cfg.fuse = std::atoi(s);
   if (const char* s = std::getenv("MHD_ROLLBACK")) cfg.rollback_steps = std::atoi(s);
+  // The learned tile selector (mixed form only): $MHD_SELECTOR names a model
+  // file as mhdglm.selector writes it (uncompressed .npz of float32
+  // arrays: params, mean, std, threshold, audit_budget or budget).
+  std::map<std::string, mhd::NpzArray> model;
+  const char* sel_path = std::getenv("MHD_SELECTOR");
+  const bool selector = sel_path && *sel_path;
+  float sel_threshold = 0.f, sel_budget = 0.f;
+  if (selector) {
+    if (precision != MHD_PREC_MIXED) {
+      std::cerr << "Error: MHD_SELECTOR needs MHD_PRECISION=mixed." << std::endl;
+      return 1;
+    }
+    std::string err;
+    if (!mhd::read_npz_f32(sel_path, model, err)) {
+      std::cerr << "Error: selector model: " << err << std::endl;
+      return 1;
+    }
+    const auto has = [&](const char* k, size_t n) {
+      const auto it = model.find(k);
+      return it != model.end() && it->second.data.size() == n;
+    };
+    if (!has("params", (size_t)mhd_mlp_param_count()) || !has("mean", 12) || !has("std", 12) ||
+        !has("threshold", 1) || !(has("audit_budget", 1) || has("budget", 1))) {
+      std::cerr << "Error: selector model " << sel_path
+                << ": needs params, mean, std, threshold and audit_budget or budget" << std::endl;
+      return 1;
+    }
+    sel_threshold = model["threshold"].data[0];
+    sel_budget = has("audit_budget", 1) ? model["audit_budget"].data[0]
+                                        : 10.0f * model["budget"].data[0];
+    std::cout << "Learned tile selector: " << sel_path << " (threshold " << sel_threshold
+              << ", 1% audits)" << std::endl;
+  }
+
   mhd_ctx* ctx = nullptr;
   int rc = mhd_create(&cfg, &ctx);
   if (rc != 0) {
@@ -174,12 +210,23 @@ int run(const Args& a, int precision, const char* prec_name, double tol) {
     mhd_destroy(ctx);
     return rc;
   }
+  if (selector &&
+      (rc = mhd_selector_configure(ctx, model["params"].data.data(), model["mean"].data.data(),
+                                   model["std"].data.data(), sel_threshold, 0.01f,
+                                   sel_budget)) != 0) {
+    std::cerr << "Error: " << mhd_last_error(ctx) << std::endl;
+    mhd_destroy(ctx);
+    return rc;
+  }
 
   mhd_status st;
   auto w0 = std::chrono::steady_clock::now();
   rc = mhd_advance(ctx, (double)t_end, -1, &st);
   auto w1 = std::chrono::steady_clock::now();
   const double wall = std::chrono::duration<double>(w1 - w0).count();
+  unsigned long long sel_stats[4] = {0, 0, 0, 0};
+  const int sel_tiles = selector ? mhd_selector_tiles(ctx) : 0;
+  if (selector && rc != 9) (void)mhd_selector_stats(ctx, sel_stats);
   if (rc == 9) {
     std::cerr << "Error: " << mhd_last_error(ctx) << std::endl;
     mhd_destroy(ctx);
@@ -252,6 +299,13 @@ int run(const Args& a, int precision, const char* prec_name, double tol) {
   if (st.rollbacks > 0 || st.restarts > 0)
     std::cout << "FP64 rollbacks = " << st.rollbacks << ", restarts = " << st.restarts
               << ", FP64 steps = " << st.exact_steps << std::endl;
+  if (selector) {
+    const double tile_steps = (double)sel_stats[3] * (double)sel_tiles;
+    std::cout << "Selector: FP64 tile-steps = " << sel_stats[2] << " (fraction "
+              << std::scientific << (tile_steps > 0 ? (double)sel_stats[2] / tile_steps : 0.0)
+              << std::defaultfloat << "), audits = " << sel_stats[0]
+              << ", audit misses = " << sel_stats[1] << std::endl;
+  }
 
   const std::string base = a.problem == "ot" ? "orszag_tang" : a.problem;
   const char* out_env = std::getenv("OUTDIR");

@@ -14,8 +14,12 @@
     runs), on held-out Orszag-Tang variants, catches more of the tiles whose
     FP32 flux error is largest than the best single-feature rule (chosen on
     the training tiles) and several times more than a random choice (the
-    audits' share) of the same number of FP64 tiles.
+    audits' share) of the same number of FP64 tiles;
+  * the drop-in executable with MHD_SELECTOR runs the same selected form as
+    the library, bitwise.
 """
+import os
+import re
 import sys
 
 import numpy as np
@@ -340,3 +344,34 @@ def test_selector_gain_over_single_features_and_random(mhd):
         assert k > 200
         assert recall > single
         assert recall >= 3.0 * k / len(Ev)
+
+
+def test_cli_selector_run_is_the_library_run(mhd, tmp_path):
+    """The drop-in executable with MHD_PRECISION=mixed and MHD_SELECTOR set to
+    the shipped model runs the chosen form: its .dat density column and step
+    count are those of the same run through the Python binding, bitwise
+    (%.17e round-trips doubles), and it reports the selector's counters."""
+    from mhdglm import selector
+
+    n, t_end = 128, 0.1
+    env = dict(os.environ, MHD_SELECTOR=selector.DEFAULT_MODEL)
+    r = mhd.run_cli([n, n, t_end, 0.25, "muscl", "hlld", 0.18], str(tmp_path), precision="mixed",
+                    env=env)
+    assert r.returncode == 0, r.stderr
+    m = re.search(r"Selector: FP64 tile-steps = (\d+) .*audits = (\d+), audit misses = (\d+)",
+                  r.stdout)
+    assert m, r.stdout
+    steps = int(re.search(r"Done\. Steps: (\d+)", r.stdout).group(1))
+    (f,) = list(tmp_path.iterdir())
+    assert f.name == f"orszag_tang_glm_muscl_hlld_mixed_N{n}x{n}.dat"
+    rho = np.loadtxt(f, comments="#", usecols=2).reshape(n, n)
+    s = mixed_solver(mhd, n, n)
+    s.configure_selector(selector.load(selector.DEFAULT_MODEL))
+    st = s.advance(t_end)
+    u = s.get_state(0)
+    stats = s.selector_stats()
+    s.close()
+    assert st.steps == steps
+    assert (stats["fp64_tile_steps"], stats["audits"], stats["misses"]) == tuple(
+        int(x) for x in m.groups())
+    assert np.array_equal(rho.view(np.uint64), u[0].view(np.uint64))

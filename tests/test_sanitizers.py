@@ -6,8 +6,9 @@
     whole small run;
   * the product's host side (mhd_api.cpp and the CLI, rebuilt with the
     sanitizers against the same device objects): initial conditions,
-    argument validation, the autotuner's table, and the CLI's usage errors --
-    everything that runs without a device.
+    argument validation, the autotuner's table, the CLI's usage errors and
+    its reading of a selector model file -- everything that runs without a
+    device.
 A sanitizer stops the program at the first error it finds; the tests require
 a clean exit and no report.  Leak checking is off for programs that load the
 HIP runtime (its own allocations live until exit)."""
@@ -75,3 +76,33 @@ def test_host_cli_usage_errors(host_san, tmp_path, args):
     r = run([os.path.join(host_san, "mhd2d_glm_amd")] + args,
             {"ASAN_OPTIONS": "detect_leaks=0", "OUTDIR": str(tmp_path)})
     assert r.returncode == 1 and "Error" in r.stderr
+
+
+def test_host_cli_selector_model(host_san, tmp_path):
+    """MHD_SELECTOR: the model file is read before any device call -- refused
+    outside the mixed form, for a missing, non-zip, compressed or incomplete
+    file; the shipped model is read (the run then needs a device)."""
+    import numpy as np
+
+    cli = os.path.join(host_san, "mhd2d_glm_amd")
+    model = os.path.join(PKG, "selector_model.npz")
+    args = [cli, "16", "16", "0.01", "0.25", "muscl", "hlld", "0.18"]
+    garbage = tmp_path / "garbage.npz"
+    garbage.write_bytes(bytes(range(256)) * 4)
+    d = dict(np.load(model))
+    np.savez_compressed(tmp_path / "compressed.npz", **d)
+    np.savez(tmp_path / "partial.npz", params=d["params"], mean=d["mean"])
+    base = {"ASAN_OPTIONS": "detect_leaks=0", "OUTDIR": str(tmp_path)}
+    cases = [({"MHD_SELECTOR": model}, "needs MHD_PRECISION=mixed"),
+             ({"MHD_PRECISION": "mixed", "MHD_SELECTOR": str(tmp_path / "none.npz")},
+              "cannot open"),
+             ({"MHD_PRECISION": "mixed", "MHD_SELECTOR": str(garbage)}, "not a zip archive"),
+             ({"MHD_PRECISION": "mixed", "MHD_SELECTOR": str(tmp_path / "compressed.npz")},
+              "compressed"),
+             ({"MHD_PRECISION": "mixed", "MHD_SELECTOR": str(tmp_path / "partial.npz")},
+              "needs params, mean, std, threshold")]
+    for env, msg in cases:
+        r = run(args, dict(base, **env))
+        assert r.returncode == 1 and msg in r.stderr, (env, r.stderr)
+    r = run(args, dict(base, MHD_PRECISION="mixed", MHD_SELECTOR=model))
+    assert "Learned tile selector: " + model in r.stdout, r.stdout + r.stderr
