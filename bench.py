@@ -51,7 +51,10 @@ Optional sections (off by default; their results are kept in profiles/):
   --accuracy-n N        whole runs of every form at N^2 against FP64;
   --kh-n N              Kelvin-Helmholtz (the 1024^2 variant) to t=5;
   --selector-n N        Orszag-Tang N^2 whole runs, FP64 vs selector;
-  --selector-breakdown N  the selector's cost per step, by part.
+  --selector-breakdown N  the selector's cost per step, by part;
+  --selector-gain N     the network's routing against random routing at the
+                        same FP64 tile share, over a few steps from one FP64
+                        state (Orszag-Tang t=0.15, Kelvin-Helmholtz t=2).
 Profiling mode: --whole-run FORM runs only one whole Orszag-Tang --n^2 run
 to t=0.5 in FORM ("selector" included) and prints its wall time and rate.
 """
@@ -333,6 +336,80 @@ def selector_breakdown(mhdglm, n, t_end=0.5):
         "all": round(us["selector"] - us["mixed"], 2)}}
 
 
+def selector_gain(mhdglm, n, t0=0.15, problem="ot", ks=(1, 4, 16, 64), shares=(0.05, 0.2)):
+    """What the network's routing buys before a chaotic flow amplifies
+    every rounding difference alike: from one FP64 state of `problem` at
+    n^2 and t0, k steps in FP64 (the reference), in the mixed form, and in
+    the mixed form routed by the shipped network and by a random network
+    at the same FP64 tile share (audits off); relL2 against FP64 of rho, p
+    and |B| after each k.  Shares: the network's own threshold, then the
+    thresholds that flag `shares` of the tiles."""
+    import numpy as np
+
+    from mhdglm import compare, selector
+
+    s = mhdglm.Solver(mhdglm.SolverConfig(nx=n, ny=n))
+    s.init_problem(problem)
+    s.advance(t0)
+    u0 = s.get_state(0)
+    s.close()
+    c = mhdglm.Solver(mhdglm.SolverConfig(nx=n, ny=n, precision=mhdglm.PREC_MIXED))
+    c.set_state(u0)
+    feats = c.tile_features()
+    c.close()
+    shipped = selector.load(selector.DEFAULT_MODEL)
+    rnd = {"params": selector.init_params(7), "mean": np.zeros(12, np.float32),
+           "std": np.ones(12, np.float32), "threshold": np.float32(0.5),
+           "budget": np.float32(1e30), "audit_budget": np.float32(1e30)}
+
+    def prob(model):
+        x = np.zeros_like(feats)
+        x[:, :12] = (feats[:, :12] - model["mean"]) / model["std"]
+        return mhdglm.mlp_forward(model["params"], x)
+
+    ps, pr = prob(shipped), prob(rnd)
+    own = float(np.mean(ps > float(shipped["threshold"])))
+    routes = [("network", float(shipped["threshold"]), own)]
+    routes += [("network", float(np.quantile(ps, 1.0 - f)), f) for f in shares]
+
+    def run(k, model=None, thr=None):
+        prec = mhdglm.PREC_FP64 if model == "fp64" else mhdglm.PREC_MIXED
+        c = mhdglm.Solver(mhdglm.SolverConfig(nx=n, ny=n, precision=prec))
+        c.set_state(u0)
+        if isinstance(model, dict):
+            c.configure_selector(model, threshold=thr, audit_frac=0.0)
+        st = c.advance(10.0, k)
+        u = c.get_state(0)
+        share = None
+        if isinstance(model, dict):
+            sel = c.selector_stats()
+            share = round(sel["fp64_tile_steps"] / max(sel["steps"] * c.selector_tiles(), 1), 4)
+        c.close()
+        return u, st, share
+
+    out = {"problem": problem, "n": n, "t0": t0, "network_own_share": round(own, 4),
+           "rows": []}
+    for k in ks:
+        ref, _, _ = run(k, "fp64")
+        um, _, _ = run(k)
+        em = compare.rel_l2_state(ref, um, ("rho", "p", "|B|"))
+        row = {"k": k, "mixed": {f: float(f"{v:.3e}") for f, v in em.items()}}
+        for name, thr, share in routes:
+            thr_r = float(np.quantile(pr, 1.0 - share)) if share > 0 else 2.0
+            un, _, sn = run(k, shipped, thr)
+            ur, _, sr = run(k, rnd, thr_r)
+            en = compare.rel_l2_state(ref, un, ("rho", "p", "|B|"))
+            er = compare.rel_l2_state(ref, ur, ("rho", "p", "|B|"))
+            row[f"share_{share:.3f}"] = {
+                "network": {"fp64_share": sn, **{f: float(f"{v:.3e}") for f, v in en.items()}},
+                "random": {"fp64_share": sr, **{f: float(f"{v:.3e}") for f, v in er.items()}},
+                "rho_error_removed": {"network": round(1.0 - en["rho"] / em["rho"], 3),
+                                      "random": round(1.0 - er["rho"] / em["rho"], 3)}}
+        out["rows"].append(row)
+        log(f"bench: selector gain {problem} {n}^2 k={k}: {row}")
+    return out
+
+
 def run_ensemble(mhdglm, torch, n, form, t_end):
     from mhdglm.workloads import ENSEMBLE_SEED, ensemble_workload
 
@@ -403,6 +480,9 @@ def main():
                     help="whole runs at this size splitting the selector's cost (0 to skip)")
     ap.add_argument("--selector-n", type=int, default=0,
                     help="Orszag-Tang whole runs to t=0.5, FP64 vs selector (0 to skip)")
+    ap.add_argument("--selector-gain", type=int, default=0,
+                    help="grid of the short-horizon routing comparison: network against random "
+                         "routing at equal FP64 share, Orszag-Tang and Kelvin-Helmholtz (0 to skip)")
     ap.add_argument("--ensemble", type=int, default=64,
                     help="instances of the batched-ensemble workload (0 to skip)")
     ap.add_argument("--ensemble-form", default="mixed")
@@ -546,6 +626,10 @@ def main():
     if args.selector_n:
         selector_run = accuracy_check(mhdglm, args.selector_n, ["fp64_fma", "mixed", "selector"])
     breakdown = selector_breakdown(mhdglm, args.selector_breakdown) if args.selector_breakdown else None
+    gain = None
+    if args.selector_gain:
+        gain = [selector_gain(mhdglm, args.selector_gain),
+                selector_gain(mhdglm, args.selector_gain, t0=2.0, problem="khb")]
     ensemble = None
     if args.ensemble:
         ensemble = run_ensemble(mhdglm, torch, args.ensemble, args.ensemble_form,
@@ -586,6 +670,7 @@ def main():
         "kh": kh,
         "selector_run": selector_run,
         "selector_breakdown": breakdown,
+        "selector_gain": gain,
         "ensemble": ensemble,
         "data": "synthetic: Orszag-Tang initial condition evaluated on the host",
         "config": {

@@ -14,7 +14,10 @@
     runs), on held-out Orszag-Tang variants, catches more of the tiles whose
     FP32 flux error is largest than the best single-feature rule (chosen on
     the training tiles) and several times more than a random choice (the
-    audits' share) of the same number of FP64 tiles;
+    audits' share) of the same number of FP64 tiles; from one FP64
+    Orszag-Tang state, the shipped network's routing removes several times
+    more of the mixed form's error over the next steps than random routing
+    of as many tiles;
   * the drop-in executable with MHD_SELECTOR runs the same selected form as
     the library, bitwise.
 """
@@ -344,6 +347,67 @@ def test_selector_gain_over_single_features_and_random(mhd):
         assert k > 200
         assert recall > single
         assert recall >= 3.0 * k / len(Ev)
+
+
+def test_shipped_selector_lowers_short_horizon_error(mhd):
+    """What the routing buys before chaos: from one FP64 Orszag-Tang state
+    (512^2, t=0.15), k steps in FP64, in the mixed form, and in the mixed
+    form routed by the shipped network and by a random network at the same
+    FP64 tile share (audits off).  Over 1 and 16 steps the network's routing
+    removes at least 10% of the mixed form's density error against FP64 at
+    its own threshold and at a 20% share, and at least twice what random
+    routing of as many tiles removes."""
+    from mhdglm import compare, selector
+
+    n = 512
+    s = mhd.Solver(mhd.SolverConfig(nx=n, ny=n))
+    s.init_problem("ot")
+    assert s.advance(0.15).code == 0
+    u0 = s.get_state(0)
+    s.close()
+    c = mhd.Solver(mhd.SolverConfig(nx=n, ny=n, precision=mhd.PREC_MIXED))
+    c.set_state(u0)
+    feats = c.tile_features()
+    c.close()
+    shipped = selector.load(selector.DEFAULT_MODEL)
+    rnd = random_model(7)
+
+    def prob(model):
+        x = np.zeros_like(feats)
+        x[:, :so.NFEAT] = (feats[:, :so.NFEAT] - model["mean"]) / model["std"]
+        return mhd.mlp_forward(model["params"], x)
+
+    ps, pr = prob(shipped), prob(rnd)
+
+    def run(k, precision, model=None, thr=None):
+        r = mhd.Solver(mhd.SolverConfig(nx=n, ny=n, precision=precision))
+        r.set_state(u0)
+        if model is not None:
+            r.configure_selector(model, threshold=thr, audit_frac=0.0)
+        assert r.advance(10.0, k).code == 0
+        u = r.get_state(0)
+        share = None
+        if model is not None:
+            st = r.selector_stats()
+            share = st["fp64_tile_steps"] / (st["steps"] * r.selector_tiles())
+        r.close()
+        return u, share
+
+    own = float(np.mean(ps > float(shipped["threshold"])))
+    assert 0.02 < own < 0.5
+    for k in (1, 16):
+        ref, _ = run(k, mhd.PREC_FP64)
+        e_mixed = compare.rel_l2_state(ref, run(k, mhd.PREC_MIXED)[0], ("rho",))["rho"]
+        for thr, frac in ((float(shipped["threshold"]), own), (float(np.quantile(ps, 0.8)), 0.2)):
+            un, sn = run(k, mhd.PREC_MIXED, shipped, thr)
+            ur, sr = run(k, mhd.PREC_MIXED, rnd, float(np.quantile(pr, 1.0 - frac)))
+            e_net = compare.rel_l2_state(ref, un, ("rho",))["rho"]
+            e_rnd = compare.rel_l2_state(ref, ur, ("rho",))["rho"]
+            say(f"k={k} share {frac:.3f}: FP64 shares network {sn:.3f} random {sr:.3f}; relL2 rho "
+                f"mixed {e_mixed:.3e} network {e_net:.3e} random {e_rnd:.3e}")
+            assert abs(sn - sr) < 0.25 * frac
+            assert e_net < 0.9 * e_mixed
+            assert e_mixed - e_net > 2.0 * (e_mixed - e_rnd)
 
 
 def test_cli_selector_run_is_the_library_run(mhd, tmp_path):
