@@ -87,12 +87,22 @@ inline bool parse_npy(const unsigned char* p, size_t n, NpzArray& out, bool& wan
       v = v * 10 + (size_t)(dims[q] - '0');
       q++;
       digit = true;
+      if (v > n) {  // no dimension exceeds the member's byte count
+        err = "bad shape in .npy header";
+        return false;
+      }
     }
     if (!digit) {
       err = "bad shape in .npy header";
       return false;
     }
     out.shape.push_back(v);
+    // A shape whose byte count does not fit the member is malformed; the
+    // check before the product keeps it from wrapping.
+    if (v != 0 && count > n / 4 / v) {
+      err = "truncated .npy data";
+      return false;
+    }
     count *= v;
   }
   const size_t data_off = off + hlen;

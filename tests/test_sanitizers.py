@@ -14,6 +14,7 @@ a clean exit and no report.  Leak checking is off for programs that load the
 HIP runtime (its own allocations live until exit)."""
 import os
 import subprocess
+import zipfile
 
 import pytest
 
@@ -81,7 +82,8 @@ def test_host_cli_usage_errors(host_san, tmp_path, args):
 def test_host_cli_selector_model(host_san, tmp_path):
     """MHD_SELECTOR: the model file is read before any device call -- refused
     outside the mixed form, for a missing, non-zip, compressed or incomplete
-    file; the shipped model is read (the run then needs a device)."""
+    file and for a member whose shape overflows its byte count; the shipped
+    model is read (the run then needs a device)."""
     import numpy as np
 
     cli = os.path.join(host_san, "mhd2d_glm_amd")
@@ -92,6 +94,14 @@ def test_host_cli_selector_model(host_san, tmp_path):
     d = dict(np.load(model))
     np.savez_compressed(tmp_path / "compressed.npz", **d)
     np.savez(tmp_path / "partial.npz", params=d["params"], mean=d["mean"])
+    # Stored members over 16 bytes of data whose headers claim 2^62 + 1
+    # floats (the byte count wraps to 4 in 64 bits), or 100^10.
+    for name, shape in (("huge", "(4611686018427387905,)"), ("many", "(" + "100, " * 10 + ")")):
+        hdr = "{'descr': '<f4', 'fortran_order': False, 'shape': %s, }" % shape
+        hdr += " " * ((64 - (10 + len(hdr) + 1) % 64) % 64) + "\n"
+        blob = b"\x93NUMPY\x01\x00" + len(hdr).to_bytes(2, "little") + hdr.encode() + bytes(16)
+        with zipfile.ZipFile(tmp_path / f"{name}.npz", "w", zipfile.ZIP_STORED) as z:
+            z.writestr("mean.npy", blob)
     base = {"ASAN_OPTIONS": "detect_leaks=0", "OUTDIR": str(tmp_path)}
     cases = [({"MHD_SELECTOR": model}, "needs MHD_PRECISION=mixed"),
              ({"MHD_PRECISION": "mixed", "MHD_SELECTOR": str(tmp_path / "none.npz")},
@@ -100,7 +110,11 @@ def test_host_cli_selector_model(host_san, tmp_path):
              ({"MHD_PRECISION": "mixed", "MHD_SELECTOR": str(tmp_path / "compressed.npz")},
               "compressed"),
              ({"MHD_PRECISION": "mixed", "MHD_SELECTOR": str(tmp_path / "partial.npz")},
-              "needs params, mean, std, threshold")]
+              "needs params, mean, std, threshold"),
+             ({"MHD_PRECISION": "mixed", "MHD_SELECTOR": str(tmp_path / "huge.npz")},
+              "mean.npy: bad shape in .npy header"),
+             ({"MHD_PRECISION": "mixed", "MHD_SELECTOR": str(tmp_path / "many.npz")},
+              "mean.npy: truncated .npy data")]
     for env, msg in cases:
         r = run(args, dict(base, **env))
         assert r.returncode == 1 and msg in r.stderr, (env, r.stderr)
