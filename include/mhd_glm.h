@@ -186,8 +186,10 @@ int mhd_enqueue_steps(mhd_ctx* ctx, double t_end, int nsteps);
  * U bitwise, mhd2d_glm_cpu.cpp:648-672).  The file holds a header (magic,
  * version, nx, ny, precision), the device control block (t, dt, c_h, step,
  * counters) and the raw state planes; loading it into a context of the same
- * nx, ny and precision continues the run bitwise.  Returns 0, 1 (mismatch or
- * I/O error) or 9. */
+ * nx, ny and precision continues the run bitwise.  A control block that no
+ * save writes (a run state outside RunState, a first-bad-cell index outside
+ * the grid) is refused; per-step queue counts in it are cleared.  Returns 0,
+ * 1 (mismatch, out-of-range control block or I/O error) or 9. */
 int mhd_checkpoint_save(mhd_ctx* ctx, const char* path);
 int mhd_checkpoint_load(mhd_ctx* ctx, const char* path);
 

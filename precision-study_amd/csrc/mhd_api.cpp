@@ -589,6 +589,28 @@ void problem_soa(const char* problem, int nx, int ny, R* out, size_t ld,
     }
 }
 
+// A checkpoint's control block as the device may take it: the run state a
+// RunState, each first-bad-cell index INT_MAX or a cell of the grid; the
+// per-step queue counts and two-stage flags cleared (a halted run's are not
+// reset by the next target), the state in the context's own buffer (cur 0,
+// as mhd_checkpoint_save leaves it) and the loading context's own test hook.
+// Returns false for a block no save could have written.
+template <class R>
+bool sane_ctrl(Ctrl<R>* h, const Ctrl<R>* mine, size_t cells) {
+  if (h->run != mhd::RUN_ACTIVE && h->run != mhd::RUN_DONE && h->run != mhd::RUN_HALTED)
+    return false;
+  for (int k = 0; k < 2; k++)
+    if (h->bad[k] != INT_MAX && (h->bad[k] < 0 || (size_t)h->bad[k] >= cells)) return false;
+  if (h->cur != 0 && h->cur != 1) return false;
+  h->cur = 0;
+  h->redo_count[0] = 0;
+  h->redo_count[1] = 0;
+  h->fz_fail = 0;
+  h->fz_count = 0;
+  h->fail_inject = mine->fail_inject;
+  return true;
+}
+
 }  // namespace
 
 extern "C" {
@@ -980,14 +1002,20 @@ int mhd_checkpoint_load(mhd_ctx* c, const char* path) {
             h.version == 2 && h.nx == c->cfg.nx && h.ny == c->cfg.ny &&
             h.precision == c->cfg.precision && h.planes == c->L->state_planes &&
             h.elem == (int)es && h.ctrl_bytes == (int)c->L->ctrl_bytes;
-  std::vector<char> buf;
+  std::vector<char> buf, ctl;
   if (ok) {
     buf.resize((size_t)h.planes * cells * es);
-    ok = std::fread(c->ctrl_host, c->L->ctrl_bytes, 1, f) == 1 &&
+    ctl.resize(c->L->ctrl_bytes);
+    ok = std::fread(ctl.data(), ctl.size(), 1, f) == 1 &&
          std::fread(buf.data(), 1, buf.size(), f) == buf.size();
   }
   std::fclose(f);
   if (!ok) return fail(c, 1, "mhd_checkpoint_load: file does not match this context");
+  ok = c->L->time_is_float
+           ? sane_ctrl((Ctrl<float>*)ctl.data(), (const Ctrl<float>*)c->ctrl_host, cells)
+           : sane_ctrl((Ctrl<double>*)ctl.data(), (const Ctrl<double>*)c->ctrl_host, cells);
+  if (!ok) return fail(c, 1, "mhd_checkpoint_load: control block out of range");
+  std::memcpy(c->ctrl_host, ctl.data(), ctl.size());
   for (int k = 0; k < h.planes; k++)
     HIP_OR_FAIL(c, hipMemcpy((char*)c->u + k * c->plane * es, buf.data() + k * cells * es,
                              cells * es, hipMemcpyHostToDevice));

@@ -131,6 +131,43 @@ def test_checkpoint_restart_bitwise(mhd, oracle, tmp_path, form):
     wrong.close()
 
 
+def test_checkpoint_control_block_checked(mhd, tmp_path):
+    """A checkpoint whose control block holds a run state that is not one,
+    or a first-bad-cell index outside the grid, is refused (the device would
+    otherwise take it); the file as saved still loads and continues."""
+    import struct
+
+    nx, ny = 48, 40
+    cfg = mhd.SolverConfig(nx=nx, ny=ny, precision=mhd.PREC_FP64)
+    a = mhd.Solver(cfg)
+    a.init_problem("ot")
+    st = a.advance(0.5, 7)
+    path = tmp_path / "ck.bin"
+    a.save(str(path))
+    a.close()
+    raw = bytearray(path.read_bytes())
+    # 40-byte header, then the FP64 control block: seven doubles, four
+    # 64-bit counters, valid_cur, valid_acc, step, run, code, bad[2].
+    base = 40
+    step_off, run_off, bad_off = base + 96, base + 100, base + 108
+    assert struct.unpack_from("<i", raw, step_off)[0] == st.steps == 7
+    assert struct.unpack_from("<i", raw, run_off)[0] in (0, 1)
+    assert struct.unpack_from("<ii", raw, bad_off) == (2**31 - 1, 2**31 - 1)
+    for off, val in ((run_off, 7), (run_off, -1), (bad_off, nx * ny), (bad_off + 4, -5)):
+        bad = bytearray(raw)
+        struct.pack_into("<i", bad, off, val)
+        q = tmp_path / "bad.bin"
+        q.write_bytes(bytes(bad))
+        b = mhd.Solver(cfg)
+        with pytest.raises(mhd.MhdError, match="control block out of range"):
+            b.load(str(q))
+        b.close()
+    c = mhd.Solver(cfg)
+    c.load(str(path))
+    assert c.advance(0.5, 3).steps == 10
+    c.close()
+
+
 @pytest.mark.parametrize("form,tol", [("mixed", 1e-6), ("mixed_kahan", 1e-6), ("fp64_fma", 1e-12)])
 @pytest.mark.parametrize("nx,ny", [(64, 48), (70, 33)])
 def test_forms_with_every_tile_shape(mhd, form, tol, nx, ny):
