@@ -52,6 +52,9 @@ Optional sections (off by default; their results are kept in profiles/):
   --kh-n N              Kelvin-Helmholtz (the 1024^2 variant) to t=5;
   --selector-n N        Orszag-Tang N^2 whole runs, FP64 vs selector;
   --selector-breakdown N  the selector's cost per step, by part;
+  --slab-rows R         one GPU's share of a row-slab split of the --n grid
+                        (R owned rows + 2-row halos): its stage kernels at
+                        that shape against the full grid;
   --selector-gain N     the network's routing against random routing at the
                         same FP64 tile share, over a few steps from one FP64
                         state (Orszag-Tang t=0.15, Kelvin-Helmholtz t=2).
@@ -410,6 +413,41 @@ def selector_gain(mhdglm, n, t0=0.15, problem="ot", ks=(1, 4, 16, 64), shares=(0
     return out
 
 
+def slab_share(mhdglm, n, rows, forms=("fp64", "mixed"), steps=40):
+    """One GPU's share of a row-slab split of the n^2 grid, at its real
+    shape: the stage kernels on an n x (rows + 4) grid holding rows
+    0 .. rows + 3 of the Orszag-Tang state after 200 steps (`rows` owned
+    rows and a 2-row halo on each side, as a slab would hold them after its
+    halo exchange), against the full grid.  Rates count owned cells only.
+    The kernels are the single-grid ones (periodic in y), so the halo rows'
+    own updates are discarded work, as they would be on a slab."""
+    import numpy as np
+
+    full = mhdglm.Solver(mhdglm.SolverConfig(nx=n, ny=n))
+    full.init_problem("ot")
+    full.advance(0.5, 200)
+    u = full.get_state(0)
+    full.close()
+    out = {"n": n, "rows": rows, "state": "Orszag-Tang after 200 FP64 steps", "forms": {}}
+    for form in forms:
+        r = {}
+        for ny, owned in ((n, n * n), (rows + 4, n * rows)):
+            s = mhdglm.Solver(mhdglm.SolverConfig(nx=n, ny=ny, precision=mhdglm.PRECISIONS[form]))
+            s.set_state(np.ascontiguousarray(u[:, :ny, :]))
+            s.advance(0.5, 5)
+            ms = s.time_kernels(0.5, steps)
+            s.close()
+            key = "full" if ny == n else "slab"
+            r[key] = {"shape": [n, ny], "stage_ms": [round(x, 4) for x in ms],
+                      "step_ms": round(sum(ms), 4),
+                      "owned_cell_updates_per_s": round(owned / (sum(ms) * 1e-3))}
+        r["slab_vs_full"] = round(r["slab"]["owned_cell_updates_per_s"]
+                                  / r["full"]["owned_cell_updates_per_s"], 3)
+        out["forms"][form] = r
+        log(f"bench: slab share {n}x{rows} {form}: {r}")
+    return out
+
+
 def run_ensemble(mhdglm, torch, n, form, t_end):
     from mhdglm.workloads import ENSEMBLE_SEED, ensemble_workload
 
@@ -480,6 +518,9 @@ def main():
                     help="whole runs at this size splitting the selector's cost (0 to skip)")
     ap.add_argument("--selector-n", type=int, default=0,
                     help="Orszag-Tang whole runs to t=0.5, FP64 vs selector (0 to skip)")
+    ap.add_argument("--slab-rows", type=int, default=0,
+                    help="rows of one GPU's share of a row-slab split of the --n grid: its stage "
+                         "kernels at that shape (plus 2-row halos) against the full grid (0 to skip)")
     ap.add_argument("--selector-gain", type=int, default=0,
                     help="grid of the short-horizon routing comparison: network against random "
                          "routing at equal FP64 share, Orszag-Tang and Kelvin-Helmholtz (0 to skip)")
@@ -626,6 +667,7 @@ def main():
     if args.selector_n:
         selector_run = accuracy_check(mhdglm, args.selector_n, ["fp64_fma", "mixed", "selector"])
     breakdown = selector_breakdown(mhdglm, args.selector_breakdown) if args.selector_breakdown else None
+    slab = slab_share(mhdglm, n, args.slab_rows) if args.slab_rows else None
     gain = None
     if args.selector_gain:
         gain = [selector_gain(mhdglm, args.selector_gain),
@@ -671,6 +713,7 @@ def main():
         "selector_run": selector_run,
         "selector_breakdown": breakdown,
         "selector_gain": gain,
+        "slab_share": slab,
         "ensemble": ensemble,
         "data": "synthetic: Orszag-Tang initial condition evaluated on the host",
         "config": {
