@@ -43,6 +43,16 @@ reported on the same JSON line:
                   instances finished and FP64 rollbacks;
   large        -- FP64, mixed and mixed + selector steps at 8192^2 (4.8 GB
                   per FP64 state);
+  selector_gain -- what the learned selector's routing buys: from one FP64
+                  state at 512^2 (Orszag-Tang t=0.15, Kelvin-Helmholtz
+                  t=2), k = 1..64 steps in FP64, in the mixed form, and in
+                  the mixed form routed by the shipped network and by a
+                  random network at the same FP64 tile share; relL2 against
+                  FP64 and the share of the mixed form's error removed
+                  (--selector-gain, 0 to skip);
+  slab_share   -- one GPU's share of an 8-way row-slab split of the grid at
+                  its real shape (n x 512 owned rows + 2-row halos): stage
+                  kernels against the full grid (--slab-rows, 0 to skip);
   cpu_baseline -- the reference CPU build (oracle/_ref, serial, one core) on
                   a bounded sample of Orszag-Tang at 512^2 (the first ~50
                   steps); `parallel`: the aggregate rate of min(visible
@@ -51,13 +61,7 @@ Optional sections (off by default; their results are kept in profiles/):
   --accuracy-n N        whole runs of every form at N^2 against FP64;
   --kh-n N              Kelvin-Helmholtz (the 1024^2 variant) to t=5;
   --selector-n N        Orszag-Tang N^2 whole runs, FP64 vs selector;
-  --selector-breakdown N  the selector's cost per step, by part;
-  --slab-rows R         one GPU's share of a row-slab split of the --n grid
-                        (R owned rows + 2-row halos): its stage kernels at
-                        that shape against the full grid;
-  --selector-gain N     the network's routing against random routing at the
-                        same FP64 tile share, over a few steps from one FP64
-                        state (Orszag-Tang t=0.15, Kelvin-Helmholtz t=2).
+  --selector-breakdown N  the selector's cost per step, by part.
 Profiling mode: --whole-run FORM runs only one whole Orszag-Tang --n^2 run
 to t=0.5 in FORM ("selector" included) and prints its wall time and rate.
 """
@@ -518,10 +522,10 @@ def main():
                     help="whole runs at this size splitting the selector's cost (0 to skip)")
     ap.add_argument("--selector-n", type=int, default=0,
                     help="Orszag-Tang whole runs to t=0.5, FP64 vs selector (0 to skip)")
-    ap.add_argument("--slab-rows", type=int, default=0,
+    ap.add_argument("--slab-rows", type=int, default=512,
                     help="rows of one GPU's share of a row-slab split of the --n grid: its stage "
                          "kernels at that shape (plus 2-row halos) against the full grid (0 to skip)")
-    ap.add_argument("--selector-gain", type=int, default=0,
+    ap.add_argument("--selector-gain", type=int, default=512,
                     help="grid of the short-horizon routing comparison: network against random "
                          "routing at equal FP64 share, Orszag-Tang and Kelvin-Helmholtz (0 to skip)")
     ap.add_argument("--ensemble", type=int, default=64,
