@@ -631,7 +631,7 @@ int mhd_create(const mhd_cfg* cfg, mhd_ctx** out) {
   if (!out) return 1;
   *out = nullptr;
   if (!cfg) return 1;
-  if (cfg->nx < 2 || cfg->ny < 2) return 1;
+  if (cfg->nx < 1 || cfg->ny < 1) return 1;
   if (!(cfg->cfl > 0.0 && cfg->cfl < 1.0)) return 1;
   if (!(cfg->alpha > 0.0)) return 1;
   if (!launchers_for(cfg->precision)) return 1;
@@ -1086,8 +1086,8 @@ extern "C" int mhd_run_noalloc(const mhd_cfg* cfg_in, void* state, size_t ld, do
                                long long max_steps, void* stream, mhd_status* st) {
   if (!cfg_in || !state) return noalloc_reject("null configuration or state");
   mhd_cfg cfg = *cfg_in;
-  if (cfg.nx < 2 || cfg.ny < 2 || (long long)cfg.nx * cfg.ny > (long long)INT_MAX)
-    return noalloc_reject("grid must be at least 2 x 2 and at most INT_MAX cells");
+  if (cfg.nx < 1 || cfg.ny < 1 || (long long)cfg.nx * cfg.ny > (long long)INT_MAX)
+    return noalloc_reject("grid must be at least 1 x 1 and at most INT_MAX cells");
   if (!(cfg.cfl > 0.0 && cfg.cfl < 1.0) || !(cfg.alpha > 0.0))
     return noalloc_reject("cfl must lie in (0,1) and alpha be positive");
   if (cfg.precision == MHD_PREC_MIXED_KAHAN || !launchers_for(cfg.precision))
@@ -1423,7 +1423,7 @@ int mhd_ensemble_create(const mhd_cfg* cfg, int n, const int* nx, const int* ny,
   if (!launchers_for(cfg->precision) || is_variant(cfg->precision)) return 1;
   if (cfg->precision == MHD_PREC_REF_MIXED && !cfg->muscl) return 1;
   for (int i = 0; i < n; i++)
-    if (nx[i] < 2 || ny[i] < 2 || (long long)nx[i] * ny[i] > (long long)INT_MAX) return 1;
+    if (nx[i] < 1 || ny[i] < 1 || (long long)nx[i] * ny[i] > (long long)INT_MAX) return 1;
   mhd_ensemble* e = new mhd_ensemble();
   e->cfg = *cfg;
   e->L = launchers_for(cfg->precision);

@@ -57,8 +57,8 @@ int main() {
   // Configurations refused before any device call.
   mhd_ctx* ctx = nullptr;
   expect(mhd_create(nullptr, &ctx) == 1 && ctx == nullptr, "null cfg");
-  mhd_cfg c = base(1, 8, MHD_PREC_FP64);
-  expect(mhd_create(&c, &ctx) == 1, "nx < 2");
+  mhd_cfg c = base(0, 8, MHD_PREC_FP64);
+  expect(mhd_create(&c, &ctx) == 1, "nx < 1");
   c = base(8, 8, MHD_PREC_FP64);
   c.cfl = 1.5;
   expect(mhd_create(&c, &ctx) == 1, "cfl");
@@ -78,7 +78,7 @@ int main() {
   expect(mhd_create(&c, &ctx) == 1, "fuse");
   expect(mhd_create(&c, nullptr) == 1, "null out");
   mhd_ensemble* e = nullptr;
-  const int nxs[2] = {8, 1}, nys[2] = {8, 8};
+  const int nxs[2] = {8, 0}, nys[2] = {8, 8};
   c = base(2, 2, MHD_PREC_FP64);
   expect(mhd_ensemble_create(&c, 2, nxs, nys, &e) == 1 && e == nullptr, "ensemble size");
   c = base(2, 2, MHD_PREC_MIXED_RECON64);

@@ -120,10 +120,6 @@ int run(const Args& a, int precision, const char* prec_name, double tol) {
     std::cerr << "Error: problem must be 'ot', 'kh' or 'khb'." << std::endl;
     return 1;
   }
-  if (a.nx < 2 || a.ny < 2) {
-    std::cerr << "Error: the periodic 2-cell halo needs nx, ny >= 2." << std::endl;
-    return 1;
-  }
   const bool use_muscl = a.order == "muscl", use_hlld = a.solver == "hlld";
   if (precision == MHD_PREC_REF_MIXED && !use_muscl) {
     std::cerr << "Error: the reference-mixed form needs 'muscl'." << std::endl;

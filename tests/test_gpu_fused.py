@@ -70,7 +70,7 @@ def check_oracle(mhd, oracle, u, steps, prec, **kw):
 
 
 @pytest.mark.parametrize("prec", ["fp64", "fp32"])
-@pytest.mark.parametrize("nx,ny", [(64, 48), (50, 37), (10, 9), (33, 70)])
+@pytest.mark.parametrize("nx,ny", [(64, 48), (50, 37), (10, 9), (33, 70), (1, 9), (3, 2)])
 def test_fused_steps_match_oracle(mhd, oracle, prec, nx, ny):
     _, dt = PREC[prec]
     say(f"fused steps OT {nx}x{ny} {prec}, 13 steps")

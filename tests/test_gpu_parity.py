@@ -126,7 +126,8 @@ def test_steps_ot_all_schemes(mhd, oracle, prec, muscl, hlld):
 
 
 @pytest.mark.parametrize("tile", [0, 1, 2, 3, 4, 5])
-@pytest.mark.parametrize("nx,ny", [(64, 64), (50, 37), (33, 70), (2, 5)])
+@pytest.mark.parametrize("nx,ny", [(64, 64), (50, 37), (33, 70), (2, 5), (1, 9), (7, 1), (1, 1),
+                                   (3, 2)])
 def test_steps_tiles_and_odd_sizes(mhd, oracle, tile, nx, ny):
     say(f"steps OT {nx}x{ny} tile={tile}")
     u = oracle.init("ot", nx, ny)
@@ -297,6 +298,34 @@ def test_full_size_conservation_and_determinism(mhd):
         tot0, tot1 = u0[k].sum(dtype=np.float64), u[k].sum(dtype=np.float64)
         scale = np.abs(u0[k]).sum(dtype=np.float64)
         assert abs(tot1 - tot0) <= 1e-12 * scale, (k, tot0, tot1)
+
+
+@pytest.mark.parametrize("nx,ny", [(1, 8), (7, 1), (1, 1), (3, 2)])
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+def test_cli_degenerate_grids_match_oracle(mhd, tmp_path, prec, nx, ny):
+    """Grids narrower than the 2-cell halo through the drop-in CLI: the
+    .dat data rows, the step count, the fallback counters and the divB
+    line equal the oracle CLI's (whose .dat equals the reference CPU
+    binary's byte for byte at these sizes)."""
+    import subprocess
+
+    args = [str(nx), str(ny), "0.05", "0.3", "muscl", "hlld", "0.18"]
+    gdir, odir = tmp_path / "gpu", tmp_path / "cpu"
+    gdir.mkdir()
+    odir.mkdir()
+    say(f"cli degenerate {nx}x{ny} {prec}")
+    r = mhd.run_cli(args, str(gdir), precision=prec)
+    assert r.returncode == 0, r.stderr
+    exe = os.path.join(ROOT, "oracle", "build", f"mhd2d_glm_oracle_{prec}")
+    ro = subprocess.run([exe] + args, env=dict(os.environ, OUTDIR=str(odir)), capture_output=True,
+                        text=True, timeout=300)
+    assert ro.returncode == 0, ro.stderr
+    (g,) = list(gdir.iterdir())
+    (o,) = list(odir.iterdir())
+    assert g.name == o.name
+    assert data_md5(g) == data_md5(o)
+    for pat in (r"Done\. Steps: (\d+)", r"HLLD->HLL fallbacks = (.*)", r"divB L1 = (.*)"):
+        assert re.search(pat, r.stdout).group(1) == re.search(pat, ro.stdout).group(1), pat
 
 
 @pytest.mark.parametrize("prec", ["fp64", "fp32"])

@@ -63,7 +63,7 @@ def run_cli(mhd, args, tmp_path, precision="fp64"):
     (["8", "8", "0.5", "0.25", "muscl", "hlld", "0"], "alpha must be positive"),
     (["8", "8", "0.5", "0.25", "muscl", "hlld", "0.18", "sod"], "problem must be"),
     (["8", "8", "0.5", "0.25", "muscl", "hlld", "0.18", "otv"], "problem must be"),
-    (["1", "8"], "nx, ny >= 2"),
+    (["8", "-3"], "invalid grid or final time"),
 ])
 def test_cli_usage_errors(mhd, tmp_path, args, msg):
     r = run_cli(mhd, args, tmp_path)

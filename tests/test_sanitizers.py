@@ -72,7 +72,7 @@ def test_host_entry_points(host_san):
 @pytest.mark.parametrize("args", [
     ["0", "8"], ["8", "8", "0.5", "1.5"], ["8", "8", "0.5", "0.25", "weno"],
     ["8", "8", "0.5", "0.25", "muscl", "roe"], ["8", "8", "0.5", "0.25", "muscl", "hlld", "0"],
-    ["8", "8", "0.5", "0.25", "muscl", "hlld", "0.18", "otv"], ["1", "8"]])
+    ["8", "8", "0.5", "0.25", "muscl", "hlld", "0.18", "otv"], ["8", "-3"]])
 def test_host_cli_usage_errors(host_san, tmp_path, args):
     r = run([os.path.join(host_san, "mhd2d_glm_amd")] + args,
             {"ASAN_OPTIONS": "detect_leaks=0", "OUTDIR": str(tmp_path)})
