@@ -1,7 +1,8 @@
 """GPU tests of the chosen form at the sizes the headline is quoted on:
 Orszag-Tang 4096^2 in the mixed form with the learned tile selector (HIP
-graphs of 32 steps), and 8192^2 (4.8 GB per FP64 state) in FP64, mixed and
-mixed with the selector.
+graphs of 32 steps), 8192^2 (4.8 GB per FP64 state) in FP64, mixed and
+mixed with the selector, and 16384^2 (19 GB per FP64 state; the largest
+power-of-two square below the 2^31-cell index limit) in FP64 and mixed.
 Size-independent properties only (the CPU oracle cannot run these sizes):
   * determinism: two runs are bitwise identical;
   * conservation: the periodic flux form keeps the sums of rho, momentum,
@@ -101,4 +102,28 @@ def test_8192_fp64_and_mixed(mhd):
     assert abs(stats["audits"] - 0.01 * mixed_tile_steps) <= 2 * steps
     errs = compare.rel_l2_state(out["fp64"], us, ("rho", "p", "|B|"))
     say(f"  selector vs FP64 after {steps} steps: {errs}; selector {stats}")
+    assert max(errs.values()) < 1e-6
+
+
+def test_16384_fp64_and_mixed(mhd):
+    """268M cells: about 39 GB of HBM per context (state and U1), one
+    context at a time; the cell index, plane offsets and the first-bad-cell
+    index stay within their types."""
+    from mhdglm import compare
+
+    n, steps = 16384, 2
+    out = {}
+    for name, prec in (("fp64", mhd.PREC_FP64), ("mixed", mhd.PREC_MIXED)):
+        say(f"OT {n}^2 {name}, {steps} steps")
+        s = mhd.Solver(mhd.SolverConfig(nx=n, ny=n, precision=prec))
+        s.init_problem("ot")
+        if name == "fp64":
+            u0 = s.get_state(0)
+        st = s.advance(0.5, steps)
+        assert st.code == 0 and st.steps == steps
+        out[name] = s.get_state(0)
+        s.close()
+        conserved(u0, out[name])
+    errs = compare.rel_l2_state(out["fp64"], out["mixed"], ("rho", "p", "|B|"))
+    say(f"  mixed vs FP64 after {steps} steps: {errs}")
     assert max(errs.values()) < 1e-6
