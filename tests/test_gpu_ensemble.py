@@ -41,9 +41,11 @@ def bits(u):
 @pytest.mark.parametrize("form", ["fp64", "fp32", "mixed", "mixed_kahan"])
 def test_ensemble_matches_single(mhd, form):
     cfg = mhd.SolverConfig(nx=2, ny=2, precision=mhd.PRECISIONS[form])
-    sizes = [(64, 64), (48, 80), (96, 32), (40, 40), (33, 57)]
+    # The last four are narrower than the 2-cell halo.
+    sizes = [(64, 64), (48, 80), (96, 32), (40, 40), (33, 57), (1, 9), (7, 1), (1, 1), (3, 2)]
     params = [(1.0, 0, 0, 0, 0), (0.9, 0.3, 1.1, 2.0, 5.5), (1.2, 6.0, 0.1, 3.3, 0.7),
-              (0.8, 1.0, 2.0, 3.0, 4.0), (1.05, 0.5, 0.5, 0.5, 0.5)]
+              (0.8, 1.0, 2.0, 3.0, 4.0), (1.05, 0.5, 0.5, 0.5, 0.5), (1.1, 0.2, 0.4, 0.6, 0.8),
+              (0.95, 1.5, 2.5, 3.5, 4.5), (1.0, 0.7, 0.7, 0.7, 0.7), (0.85, 5.0, 4.0, 3.0, 2.0)]
     say(f"ensemble {form} {sizes}")
     ens = mhd.Ensemble(cfg, sizes)
     for i, p in enumerate(params):
