@@ -18,7 +18,9 @@ import pytest
 pytestmark = pytest.mark.gpu
 
 UINT = {np.dtype(np.float64): np.uint64, np.dtype(np.float32): np.uint32}
-CASES = ((90, 70, 40), (50, 40, 25))  # second call smaller than the first
+# Each call smaller than the one before; the last two are narrower than
+# the 2-cell halo.
+CASES = ((90, 70, 40), (50, 40, 25), (3, 2, 12), (1, 9, 12))
 
 
 def say(msg):
