@@ -97,6 +97,11 @@ struct mhd_ctx {
     unsigned long long* counters = nullptr;  // audits, misses, FP64 tile-steps, steps
     void* u1x = nullptr;            // FP64 scratch stage output
     void* ctrl_scratch = nullptr;   // control-block snapshot for side evaluations
+    // Concurrent schedule ($MHD_SELECTOR_CONCURRENT=1): the FP64 list
+    // launch of the flagged and audit tiles runs on `side` beside the mixed
+    // stage kernel (fork and join events, captured into the step's graph).
+    hipStream_t side = nullptr;
+    hipEvent_t fork[2] = {nullptr, nullptr}, join[2] = {nullptr, nullptr};
   } sel;
   Rollback rb;
 };
@@ -451,8 +456,33 @@ hipError_t enqueue_step_selected(mhd_ctx* c) {
     e = mhd::sel_audit_pick(S.ntiles, (const int*)(ctrl + offsetof(Ctrl<double>, step)),
                             (const int*)(ctrl + offsetof(Ctrl<double>, run)), S.audit_frac, S.lists, S.counts, S.counters, c->ctrl, S.ctrl_scratch,
                             c->L->ctrl_bytes, c->stream);
+  // Concurrent schedule: the flagged (and, in stage 1, audit) tiles on the
+  // side stream beside the mixed stage kernel.  Their cells are disjoint
+  // from the mixed list's, they read only what the previous stage left, and
+  // they meet the mixed kernel only in the control block's order-independent
+  // atomics (maximum speed, first bad cell, counters), so the step's results
+  // are those of the serial schedule.  The fallback queue, which the mixed
+  // kernel fills, runs after it on the main stream.
+  const bool conc = S.side != nullptr;
+  if (conc && e == hipSuccess) {
+    e = hipEventRecord(S.fork[0], c->stream);
+    if (e == hipSuccess) e = hipStreamWaitEvent(S.side, S.fork[0], 0);
+    if (e == hipSuccess) {
+      const mhd::ListJob jobs[2] = {{c->u, c->u, c->u1, c->ctrl, fp64_list, fp64_count},
+                                    {c->u, c->u, S.u1x, S.ctrl_scratch, audit_list, audit_count}};
+      e = F64->stage_listn(c->grid, 1, jobs, 2, S.side);
+    }
+    if (e == hipSuccess) e = hipEventRecord(S.join[0], S.side);
+  }
   if (e == hipSuccess) e = c->L->stage(gm, 1, c->u, c->u, c->u1, c->ctrl, c->stream);
-  if (e == hipSuccess) {
+  if (e == hipSuccess && conc) {
+    if (c->grid.redo_list[0]) {
+      const mhd::ListJob jobs[1] = {
+          {c->u, c->u, c->u1, c->ctrl, c->grid.redo_list[0], redo_count(1)}};
+      e = F64->stage_listn(c->grid, 1, jobs, 1, c->stream);
+    }
+    if (e == hipSuccess) e = hipStreamWaitEvent(c->stream, S.join[0], 0);
+  } else if (e == hipSuccess) {
     const mhd::ListJob jobs[3] = {
         {c->u, c->u, c->u1, c->ctrl, fp64_list, fp64_count},
         {c->u, c->u, S.u1x, S.ctrl_scratch, audit_list, audit_count},
@@ -463,8 +493,24 @@ hipError_t enqueue_step_selected(mhd_ctx* c) {
     e = mhd::sel_audit((const double*)c->u1, (const double*)S.u1x, c->plane, c->cfg.nx, c->cfg.ny,
                        S.tx, S.ty, S.sumsq, S.lists, S.counts, S.audit_budget, S.sticky, S.flags,
                        S.err, S.counters, c->stream);
+  if (conc && e == hipSuccess) {
+    e = hipEventRecord(S.fork[1], c->stream);
+    if (e == hipSuccess) e = hipStreamWaitEvent(S.side, S.fork[1], 0);
+    if (e == hipSuccess) {
+      const mhd::ListJob jobs[1] = {{c->u1, c->u, c->u, c->ctrl, fp64_list, fp64_count}};
+      e = F64->stage_listn(c->grid, 2, jobs, 1, S.side);
+    }
+    if (e == hipSuccess) e = hipEventRecord(S.join[1], S.side);
+  }
   if (e == hipSuccess) e = c->L->stage(gm, 2, c->u1, c->u, c->u, c->ctrl, c->stream);
-  if (e == hipSuccess) {
+  if (e == hipSuccess && conc) {
+    if (c->grid.redo_list[1]) {
+      const mhd::ListJob jobs[1] = {
+          {c->u1, c->u, c->u, c->ctrl, c->grid.redo_list[1], redo_count(2)}};
+      e = F64->stage_listn(c->grid, 2, jobs, 1, c->stream);
+    }
+    if (e == hipSuccess) e = hipStreamWaitEvent(c->stream, S.join[1], 0);
+  } else if (e == hipSuccess) {
     const mhd::ListJob jobs[2] = {
         {c->u1, c->u, c->u, c->ctrl, fp64_list, fp64_count},
         {c->u1, c->u, c->u, c->ctrl, c->grid.redo_list[1], redo_count(2)}};
@@ -761,6 +807,11 @@ void mhd_destroy(mhd_ctx* c) {
   if (c->ctrl) (void)hipFree(c->ctrl);
   if (c->ctrl_host) (void)hipHostFree(c->ctrl_host);
   rb_free(c->rb);
+  for (int k = 0; k < 2; k++) {
+    if (c->sel.fork[k]) (void)hipEventDestroy(c->sel.fork[k]);
+    if (c->sel.join[k]) (void)hipEventDestroy(c->sel.join[k]);
+  }
+  if (c->sel.side) (void)hipStreamDestroy(c->sel.side);
   if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
 }
@@ -1716,6 +1767,14 @@ int sel_alloc(mhd_ctx* c) {
   HIP_OR_FAIL(c, hipMalloc(&S.ctrl_scratch, c->L->ctrl_bytes));
   HIP_OR_FAIL(c, hipMemset(S.sticky, 0, nt));
   HIP_OR_FAIL(c, hipMemset(S.counters, 0, 4 * sizeof(unsigned long long)));
+  const char* conc = std::getenv("MHD_SELECTOR_CONCURRENT");
+  if (conc && std::atoi(conc) == 1) {
+    HIP_OR_FAIL(c, hipStreamCreateWithFlags(&S.side, hipStreamNonBlocking));
+    for (int k = 0; k < 2; k++) {
+      HIP_OR_FAIL(c, hipEventCreateWithFlags(&S.fork[k], hipEventDisableTiming));
+      HIP_OR_FAIL(c, hipEventCreateWithFlags(&S.join[k], hipEventDisableTiming));
+    }
+  }
   return 0;
 }
 
@@ -1806,6 +1865,12 @@ int mhd_selector_configure(mhd_ctx* c, const float* params, const float* mean, c
   }
   if (c->cfg.precision != MHD_PREC_MIXED) return fail(c, 1, "selector needs the MIXED form");
   if (!c->have_state) return fail(c, 1, "mhd_selector_configure: set a state first");
+  // The refresh every kSelectorRefresh steps is decided when a step is
+  // enqueued, so a captured batch replays the refreshes it was captured
+  // with: a batch that is a multiple of the refresh period keeps them every
+  // kSelectorRefresh steps across replays and the steps enqueued one by one.
+  if (c->cfg.batch_steps % kSelectorRefresh != 0)
+    c->cfg.batch_steps += kSelectorRefresh - c->cfg.batch_steps % kSelectorRefresh;
   // The selector routes tiles of the single-stage step: two-stage steps
   // end for this context.
   if (int rc = settle_state(c)) return rc;

@@ -410,6 +410,42 @@ def test_shipped_selector_lowers_short_horizon_error(mhd):
             assert e_mixed - e_net > 2.0 * (e_mixed - e_rnd)
 
 
+def test_concurrent_schedule_is_the_serial_one(mhd, monkeypatch):
+    """MHD_SELECTOR_CONCURRENT=1 runs the flagged and audit tiles' FP64 list
+    launch on a second stream beside the mixed stage kernel (fork and join
+    inside the captured step).  From a mid-flow Orszag-Tang state (tiles
+    flagged and audited), 40 steps in graph batches of 32 (then 8 steps
+    enqueued one by one) and of 1, which the selector rounds up to its
+    refresh period of 8: the state, the status and the selector's counters
+    are the same in every case, bit for bit."""
+    from mhdglm import selector
+
+    n = 256
+    s = mhd.Solver(mhd.SolverConfig(nx=n, ny=n))
+    s.init_problem("ot")
+    assert s.advance(0.2).code == 0
+    u0 = s.get_state(0)
+    s.close()
+    model = selector.load(selector.DEFAULT_MODEL)
+    out = {}
+    for conc in ("0", "1"):
+        monkeypatch.setenv("MHD_SELECTOR_CONCURRENT", conc)
+        for batch in (0, 1):
+            r = mhd.Solver(mhd.SolverConfig(nx=n, ny=n, precision=mhd.PREC_MIXED,
+                                            batch_steps=batch))
+            r.set_state(u0)
+            r.configure_selector(model, threshold=float(model["threshold"]) * 0.5)
+            st = r.advance(10.0, 40)
+            out[(conc, batch)] = (r.get_state(0), st.as_dict(), r.selector_stats())
+            r.close()
+    u, st, stats = out[("0", 0)]
+    say(f"serial: {stats}")
+    assert st["code"] == 0 and stats["fp64_tile_steps"] > 0 and stats["audits"] > 0
+    for key, (u2, st2, stats2) in out.items():
+        assert np.array_equal(u.view(np.uint64), u2.view(np.uint64)), key
+        assert st2 == st and stats2 == stats, key
+
+
 def test_cli_selector_run_is_the_library_run(mhd, tmp_path):
     """The drop-in executable with MHD_PRECISION=mixed and MHD_SELECTOR set to
     the shipped model runs the chosen form: its .dat density column and step
