@@ -793,6 +793,9 @@ void mhd_destroy(mhd_ctx* c) {
   if (!c) return;
   if (c->graph) (void)hipGraphExecDestroy(c->graph);
   if (c->stream) (void)hipStreamSynchronize(c->stream);
+  // Side work is joined into the main stream, except after a failed launch
+  // between a fork and its join.
+  if (c->sel.side) (void)hipStreamSynchronize(c->sel.side);
   for (void* p : {(void*)c->sel.params, (void*)c->sel.mean, (void*)c->sel.istd,
                   (void*)c->sel.feats, (void*)c->sel.err, (void*)c->sel.flags,
                   (void*)c->sel.sticky, (void*)c->sel.lists, (void*)c->sel.counts,
