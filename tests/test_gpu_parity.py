@@ -328,6 +328,37 @@ def test_cli_degenerate_grids_match_oracle(mhd, tmp_path, prec, nx, ny):
         assert re.search(pat, r.stdout).group(1) == re.search(pat, ro.stdout).group(1), pat
 
 
+@pytest.mark.parametrize("problem", ["kh", "khb"])
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+def test_cli_kelvin_helmholtz_matches_oracle(mhd, tmp_path, prec, problem):
+    """The drop-in CLI on the Kelvin-Helmholtz problems: the file is named
+    and titled as the reference GPU driver names and titles its KH output
+    (mhd2d_glm_gpu.cu:592-611: base "kh", "# Kelvin-Helmholtz ideal MHD
+    with mixed GLM (GPU)"; `khb`, this build's variant, takes its own
+    name), and its data rows, step count, fallback counters and divB line
+    equal the oracle CLI's on the same problem."""
+    import subprocess
+
+    args = ["48", "32", "0.05", "0.3", "muscl", "hlld", "0.18", problem]
+    gdir, odir = tmp_path / "gpu", tmp_path / "cpu"
+    gdir.mkdir()
+    odir.mkdir()
+    say(f"cli {problem} 48x32 {prec}")
+    r = mhd.run_cli(args, str(gdir), precision=prec)
+    assert r.returncode == 0, r.stderr
+    exe = os.path.join(ROOT, "oracle", "build", f"mhd2d_glm_oracle_{prec}")
+    ro = subprocess.run([exe] + args, env=dict(os.environ, OUTDIR=str(odir)), capture_output=True,
+                        text=True, timeout=300)
+    assert ro.returncode == 0, ro.stderr
+    (g,) = list(gdir.iterdir())
+    (o,) = list(odir.iterdir())
+    assert g.name == o.name == f"{problem}_glm_muscl_hlld_{prec}_N48x32.dat"
+    assert open(g).readline() == "# Kelvin-Helmholtz ideal MHD with mixed GLM (GPU)\n"
+    assert data_md5(g) == data_md5(o)
+    for pat in (r"Done\. Steps: (\d+)", r"HLLD->HLL fallbacks = (.*)", r"divB L1 = (.*)"):
+        assert re.search(pat, r.stdout).group(1) == re.search(pat, ro.stdout).group(1), pat
+
+
 @pytest.mark.parametrize("prec", ["fp64", "fp32"])
 def test_cli_dat_header_matches_reference_cpu_build(mhd, tmp_path, prec):
     """The .dat header of the drop-in CLI: the reference GPU build writes the
