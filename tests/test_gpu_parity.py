@@ -359,21 +359,24 @@ def test_cli_kelvin_helmholtz_matches_oracle(mhd, tmp_path, prec, problem):
         assert re.search(pat, r.stdout).group(1) == re.search(pat, ro.stdout).group(1), pat
 
 
+@pytest.mark.parametrize("order,solver", [("muscl", "hlld"), ("muscl", "hll"), ("first", "hlld"),
+                                          ("first", "hll")])
 @pytest.mark.parametrize("prec", ["fp64", "fp32"])
-def test_cli_dat_header_matches_reference_cpu_build(mhd, tmp_path, prec):
-    """The .dat header of the drop-in CLI: the reference GPU build writes the
-    CPU build's four '#' lines with ' (GPU)' appended to the first
-    (mhd2d_glm_gpu.cu:610-625 against mhd2d_glm_cpu.cpp:628-645).  The CPU
-    side is the oracle's CLI, whose .dat equals the reference CPU binary's
-    byte for byte; the run is bitwise, so t, ch_last and the counters in the
-    header agree too."""
+def test_cli_dat_header_matches_reference_cpu_build(mhd, tmp_path, prec, order, solver):
+    """The .dat of the drop-in CLI for every order and solver: the reference
+    GPU build writes the CPU build's '#' lines with ' (GPU)' appended to the
+    first (mhd2d_glm_gpu.cu:610-625 against mhd2d_glm_cpu.cpp:628-645), the
+    HLLD statistics line for the HLLD solver only.  The CPU side is the
+    oracle's CLI, whose .dat equals the reference CPU binary's byte for byte
+    for all four schemes; the run is bitwise, so t, ch_last, the counters
+    and the data rows agree too."""
     import subprocess
 
-    args = ["64", "48", "0.05", "0.25", "muscl", "hlld", "0.18"]
+    args = ["64", "48", "0.05", "0.25", order, solver, "0.18"]
     gdir, odir = tmp_path / "gpu", tmp_path / "cpu"
     gdir.mkdir()
     odir.mkdir()
-    say(f"cli header {prec}")
+    say(f"cli header {prec} {order} {solver}")
     r = mhd.run_cli(args, str(gdir), precision=prec)
     assert r.returncode == 0, r.stderr
     exe = os.path.join(ROOT, "oracle", "build", f"mhd2d_glm_oracle_{prec}")
@@ -385,6 +388,8 @@ def test_cli_dat_header_matches_reference_cpu_build(mhd, tmp_path, prec):
     assert g.name == o.name
     gh = [ln for ln in open(g) if ln.startswith("#")]
     oh = [ln for ln in open(o) if ln.startswith("#")]
-    assert len(gh) == len(oh) == 4
+    # The HLLD statistics line is written for the HLLD solver only.
+    assert len(gh) == len(oh) == (4 if solver == "hlld" else 3)
     assert gh[0] == oh[0].rstrip("\n") + " (GPU)\n"
     assert gh[1:] == oh[1:]
+    assert data_md5(g) == data_md5(o)
