@@ -21,6 +21,7 @@
 //                    recomputation; misses become sticky FP64 tiles
 #include <hip/hip_runtime.h>
 
+#include "mhd_kernels.h"
 #include "mhd_select.h"
 
 namespace mhd {
@@ -280,6 +281,8 @@ struct AuditOut {
   int* fp64_list;
   int* fp64_count;
   unsigned long long* counters;
+  int* slots;      // the mixed stage launches' slots (-1: none)
+  const int* pos;  // slot of each mixed tile
 };
 
 __global__ __launch_bounds__(256) void k_tile_errors(const double* a, const double* b, size_t ps,
@@ -315,6 +318,7 @@ __global__ __launch_bounds__(256) void k_tile_errors(const double* a, const doub
         if ((float)m > audit.budget) {
           audit.sticky[tile] = 1;
           audit.flags[tile] = 1;
+          audit.slots[audit.pos[tile]] = -1;
           audit.fp64_list[atomicAdd(audit.fp64_count, 1)] = tile;
           atomicAdd(&audit.counters[1], 1ull);
         }
@@ -589,9 +593,11 @@ __global__ void k_adam(float* P, const float* partial, int nparts, float* m, flo
 // flags are taken in rounds of kRouteThreads x 16: a thread reads its 16
 // consecutive flags with one 16-byte load (coalesced across the block),
 // counts them, a block-wide exclusive scan gives its offsets in both lists,
-// and it writes its tiles.  Runs when the flags are refreshed; an audit miss
-// later appends its tile to the FP64 list and sets its flag (the mixed
-// stages skip flagged tiles).
+// and it writes its tiles.  Then it deals the mixed tiles to the mixed
+// stage launches' blocks in XCD order (lists + 3 ntiles) and records each
+// tile's slot (lists + 4 ntiles).  Runs when the flags are refreshed; an
+// audit miss later appends its tile to the FP64 list, sets its flag and
+// strikes its slot out (-1), so the mixed stages skip it.
 constexpr int kRouteThreads = 1024;
 __global__ __launch_bounds__(kRouteThreads) void k_route(const unsigned char* flags, int ntiles,
                                                          int* lists, int* counts) {
@@ -645,6 +651,21 @@ __global__ __launch_bounds__(kRouteThreads) void k_route(const unsigned char* fl
   if (tid == 0) {
     counts[0] = done_f;
     counts[2] = ntiles - done_f;
+  }
+  // The mixed stage launches' slots: block b of a one-block-per-tile grid
+  // takes the mixed tile at xcd_item(b, mixed count), -1 past the list, and
+  // pos[tile] is that tile's slot (for an audit miss to strike it out).
+  // The mixed list was written by this block above.
+  __syncthreads();
+  const int nm = ntiles - done_f;
+  const int* mixed = lists + 2 * ntiles;
+  int* slots = lists + 3 * ntiles;
+  int* pos = lists + 4 * ntiles;
+  for (int b = tid; b < ntiles; b += kRouteThreads) {
+    const int item = xcd_item(b, nm);
+    const int t = item < 0 ? -1 : mixed[item];
+    slots[b] = t;
+    if (t >= 0) pos[t] = b;
   }
 }
 
@@ -745,7 +766,8 @@ hipError_t sel_audit(const double* a, const double* b, size_t plane, int nx, int
                      unsigned long long* counters, hipStream_t s) {
   const int tiles_x = (nx + tx - 1) / tx, ntiles = tiles_x * ((ny + ty - 1) / ty);
   const int grid = ntiles < 1024 ? ntiles : 1024;
-  sel::AuditOut o{budget, sticky, flags, lists, counts, counters};
+  sel::AuditOut o{budget, sticky, flags, lists, counts, counters, lists + 3 * ntiles,
+                  lists + 4 * ntiles};
   hipLaunchKernelGGL(sel::k_tile_errors, dim3(grid), dim3(256), 0, s, a, b, plane, nx, ny, tx, ty,
                      tiles_x, ntiles, sumsq9, lists + ntiles, counts + 1, err, o);
   return hipGetLastError();

@@ -91,7 +91,8 @@ struct mhd_ctx {
     float* err = nullptr;           // ntiles
     unsigned char* flags = nullptr;   // FP64 routing of the step
     unsigned char* sticky = nullptr;  // tiles forced to FP64 by an audit miss
-    int* lists = nullptr;           // 3 x ntiles: the step's FP64 tiles, audit sample, mixed tiles
+    int* lists = nullptr;           // 5 x ntiles: the step's FP64 tiles, audit sample, mixed
+                                    // tiles, the mixed launches' slots, each mixed tile's slot
     int* counts = nullptr;          // lengths of the three lists
     double* sumsq = nullptr;        // error scale (per-plane sum of squares)
     unsigned long long* counters = nullptr;  // audits, misses, FP64 tile-steps, steps
@@ -433,8 +434,9 @@ hipError_t enqueue_step_selected(mhd_ctx* c) {
   }
   // FP64 tiles and mixed tiles as device lists (rebuilt with the flags), and
   // the step's audit sample (from the pick, which also snapshots the control
-  // block for the audits).  The mixed stages walk the mixed list, each XCD
-  // an equal part of it, so flagged tiles cost the mixed launch nothing
+  // block for the audits).  The mixed stages take the mixed tiles from the
+  // slots the routing dealt them (each XCD an equal part of the mixed list,
+  // one load per block), so flagged tiles cost the mixed launch nothing
   // wherever they cluster; they only queue their fallback tiles here.  One
   // list launch per stage evaluates in FP64, over a small fixed grid: the
   // flagged tiles, the tiles the mixed stage queued for its fallback, and
@@ -444,9 +446,7 @@ hipError_t enqueue_step_selected(mhd_ctx* c) {
   const int* fp64_count = S.counts;
   const int* audit_count = S.counts + 1;
   mhd::GridDesc gm = c->grid;
-  gm.flags = S.flags;
-  gm.tiles = S.lists + 2 * S.ntiles;
-  gm.tile_count = S.counts + 2;
+  gm.tiles = S.lists + 3 * S.ntiles;
   gm.redo = nullptr;
   auto redo_count = [&](int stage) {
     return (const int*)((char*)c->ctrl + offsetof(Ctrl<double>, redo_count) +
@@ -1762,7 +1762,7 @@ int sel_alloc(mhd_ctx* c) {
   HIP_OR_FAIL(c, hipMalloc((void**)&S.err, nt * sizeof(float)));
   HIP_OR_FAIL(c, hipMalloc((void**)&S.flags, nt));
   HIP_OR_FAIL(c, hipMalloc((void**)&S.sticky, nt));
-  HIP_OR_FAIL(c, hipMalloc((void**)&S.lists, 3 * nt * sizeof(int)));
+  HIP_OR_FAIL(c, hipMalloc((void**)&S.lists, 5 * nt * sizeof(int)));
   HIP_OR_FAIL(c, hipMalloc((void**)&S.counts, 3 * sizeof(int)));
   HIP_OR_FAIL(c, hipMalloc((void**)&S.sumsq, 9 * sizeof(double)));
   HIP_OR_FAIL(c, hipMalloc((void**)&S.counters, 4 * sizeof(unsigned long long)));
@@ -1831,9 +1831,7 @@ int mhd_pair_tile_errors(mhd_ctx* c, float* err) {
   auto& S = c->sel;
   const mhd::Launchers* F64 = mhd::launchers_exact(mhd::MODE_FP64);
   mhd::GridDesc g = c->grid;
-  g.flags = nullptr;
   g.tiles = nullptr;
-  g.tile_count = nullptr;
   g.redo_list[0] = g.redo_list[1] = nullptr;  // both updates of every tile
   // Both stage-1 updates of the current state, against a prepared snapshot
   // of the control block (the run's own counters and verdicts are

@@ -487,9 +487,7 @@ static hipError_t launch_step_fused(const GridDesc& g, void* u, void* ualt, void
   const int* cur = (const int*)((char*)ctrl + offsetof(Ctrl<T>, cur));
   FusedArgs<Pol> fa;
   fa.a = stage_args<Pol, TX, TY>(g, ctrl);
-  fa.a.flags = nullptr;
   fa.a.tiles = nullptr;
-  fa.a.tile_count = nullptr;
   fa.a.src = fa.a.uold = (const S*)u;
   fa.a.dst = (S*)ualt;
   fa.a.redo_list = vote ? g.redo_list[0] : nullptr;

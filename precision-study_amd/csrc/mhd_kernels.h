@@ -82,6 +82,16 @@ struct Ctrl {
   unsigned long long speed_slot[kSpeedSlots];
 };
 
+// The XCD banding of a list of n items walked by a grid of at least n
+// blocks (blocks are dealt to the 8 XCDs round-robin): block bid takes item
+// xcd_item(bid, n), so XCD x takes the x-th eighth of the list; -1 for the
+// blocks left over (the highest-numbered ones, so no XCD waits on another).
+constexpr int xcd_item(int bid, int n) {
+  const int q = n >> 3, r = n & 7, x = bid & 7, s = bid >> 3;
+  if (s >= q + (x < r ? 1 : 0)) return -1;
+  return x * q + (x < r ? x : r) + s;
+}
+
 // Launch parameters of one solver instance (one grid).
 struct GridDesc {
   int nx, ny;
@@ -91,14 +101,12 @@ struct GridDesc {
   double tol;            // HLLD degeneracy tolerance
   double lowbeta_ratio;  // mixed forms: FP64 recovery where E > ratio * e_int
   int tile;              // tile-shape id
-  // Per-tile precision routing (learned selector).  With `tiles` non-null a
-  // stage kernel evaluates the listed tiles tiles[0 .. *tile_count) (each
-  // XCD a contiguous part of the list) instead of the whole grid; with
-  // `flags` non-null it skips a tile whose flag is set (a tile the stage-1
-  // audit moved to FP64 after the list was made).
-  const unsigned char* flags = nullptr;
+  // Per-tile precision routing (learned selector).  With `tiles` non-null,
+  // block b of a stage launch evaluates tile tiles[b] (-1: none) instead of
+  // the b-th tile of the whole grid: the routing kernel writes the mixed
+  // tiles there in each XCD's block order (xcd_item), and an audit miss
+  // strikes its tile out.
   const int* tiles = nullptr;
-  const int* tile_count = nullptr;
   // FP64 fallback of the mixed form.  With redo_list[stage-1] non-null, a
   // mixed stage tile with an unusable cell (the positivity check) writes
   // nothing and appends its index to that list (count in the control

@@ -246,14 +246,6 @@ __device__ __forceinline__ int xcd_tile(int bid, int nblocks) {
   return x * q + (x < r ? x : r) + (bid >> 3);
 }
 
-// The same banding over a list of n items walked by a grid of at least n
-// blocks: XCD x takes the x-th eighth of the list; -1 for the blocks left
-// over (the highest-numbered ones, so no XCD waits on another).
-__device__ __forceinline__ int xcd_item(int bid, int n) {
-  const int q = n >> 3, r = n & 7, x = bid & 7, s = bid >> 3;
-  if (s >= q + (x < r ? 1 : 0)) return -1;
-  return x * q + (x < r ? x : r) + s;
-}
 
 // Publishes a block's signal-speed maximum: one slot per block residue, and
 // the atomic only when the slot does not already hold a larger value (the
@@ -431,9 +423,7 @@ struct StageArgs {
   double alpha;
   float lowbeta_ratio;      // mixed: FP64 recovery where E > ratio * e_internal
   Ctrl<typename Pol::T>* ctrl;
-  const unsigned char* flags;  // skip tiles whose flag is set (null: none)
-  const int* tiles;            // evaluate tiles[0 .. *tile_count) (null: every tile)
-  const int* tile_count;
+  const int* tiles;            // block b evaluates tiles[b], -1: none (null: every tile)
   // FP64 fallback (mixed policy): a tile with an unusable cell appends
   // redo_base + tile at redo_list[atomicAdd(redo_counter, 1)].
   int* redo_list;              // null: no fallback
@@ -1145,17 +1135,13 @@ __attribute__((amdgpu_waves_per_eu(StageWaves<Pol>::value))) void k_stage(StageA
   __shared__ TileSharedOf<Pol, TX, TY> sh;
   __shared__ typename Pol::S sU[hold == HOLD_LDS ? 9 * TX * TY : 1];
   Ctrl<typename Pol::T>* ctrl = a.ctrl;
+  // The routed tile does not depend on the control block: its load is in
+  // flight with the control block's (the grid has one block per slot).
+  const int routed = a.tiles ? a.tiles[blockIdx.x] : 0;
   if (ctrl->run != RUN_ACTIVE) return;
   if (STAGE == 2 && ctrl->bad[0] != INT_MAX) return;
-  int tile;
-  if (a.tiles) {
-    const int item = xcd_item((int)blockIdx.x, uni(*a.tile_count));
-    if (item < 0) return;
-    tile = uni(a.tiles[item]);
-  } else {
-    tile = xcd_tile((int)blockIdx.x, a.ntiles);
-  }
-  if (a.flags && a.flags[tile] != 0) return;
+  const int tile = a.tiles ? uni(routed) : xcd_tile((int)blockIdx.x, a.ntiles);
+  if (tile < 0) return;
   stage_tile<Pol, MUSCL, HLLD, STAGE, TX, TY, hold>(a, tile, sh, sU);
 }
 
@@ -1500,9 +1486,7 @@ __device__ __forceinline__ StageArgs<Pol> inst_args(const EnsArgs<Pol>& e, const
   a.alpha = e.alpha;
   a.lowbeta_ratio = e.lowbeta_ratio;
   a.ctrl = uni_ptr(d.ctrl);
-  a.flags = nullptr;
   a.tiles = nullptr;
-  a.tile_count = nullptr;
   a.redo_list = e.redo_items[STAGE - 1];
   a.redo_counter = (int*)&e.queue[16 + STAGE - 1];
   a.redo_base = first_item;
@@ -1681,9 +1665,7 @@ static StageArgs<Pol> stage_args(const GridDesc& g, void* ctrl) {
   a.alpha = g.alpha;
   a.lowbeta_ratio = (float)g.lowbeta_ratio;
   a.ctrl = (Ctrl<T>*)ctrl;
-  a.flags = g.flags;
   a.tiles = g.tiles;
-  a.tile_count = g.tile_count;
   a.redo_list = nullptr;
   a.redo_counter = redo_counter_of<T>(ctrl, 1);
   a.redo_base = 0;
@@ -1771,9 +1753,7 @@ static hipError_t launch_list_t(const GridDesc& g, int stage, const ListJob* job
   auto seg = [&](const ListJob& j) {
     ListSeg<Pol> q;
     q.a = stage_args<Pol, TX, TY>(g, j.ctrl);
-    q.a.flags = nullptr;
     q.a.tiles = nullptr;
-    q.a.tile_count = nullptr;
     q.a.redo_all = 0;
     if (Pol::kRedo && queue && j.ctrl) {
       q.a.redo_list = queue;
