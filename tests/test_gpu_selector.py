@@ -137,6 +137,7 @@ def test_mlp_training_is_deterministic(mhd):
 
 
 def mixed_solver(mhd, nx, ny, problem="ot", steps=0, **kw):
+    kw.setdefault("tile", 3)  # 16x15, the tile grid the tests count
     s = mhd.Solver(mhd.SolverConfig(nx=nx, ny=ny, precision=mhd.PREC_MIXED, **kw))
     s.init_problem(problem)
     if steps:
@@ -186,7 +187,7 @@ def test_pair_tile_errors_match_oracle(mhd):
 
 
 def run_steps(mhd, precision, nx, ny, steps, model=None, problem="ot", which=0, **sel):
-    s = mhd.Solver(mhd.SolverConfig(nx=nx, ny=ny, precision=precision))
+    s = mhd.Solver(mhd.SolverConfig(nx=nx, ny=ny, precision=precision, tile=3))
     s.init_problem(problem)
     if model is not None:
         s.configure_selector(model, **sel)
@@ -236,12 +237,13 @@ def test_selector_counts_only_live_steps(mhd):
     them (audits there would compare stale buffers); advancing a finished
     run again changes nothing."""
     say("selector to t_end inside a batch")
-    s = mhd.Solver(mhd.SolverConfig(nx=96, ny=80, precision=mhd.PREC_MIXED))
+    s = mhd.Solver(mhd.SolverConfig(nx=96, ny=80, precision=mhd.PREC_MIXED, tile=3))
     s.init_problem("ot")
     steps = s.advance(0.05).steps
     s.close()
     batch = 32 if steps % 32 else 31
-    s = mhd.Solver(mhd.SolverConfig(nx=96, ny=80, precision=mhd.PREC_MIXED, batch_steps=batch))
+    s = mhd.Solver(mhd.SolverConfig(nx=96, ny=80, precision=mhd.PREC_MIXED, batch_steps=batch,
+                                    tile=3))
     s.init_problem("ot")
     s.configure_selector(random_model(), threshold=2.0, audit_frac=0.3, audit_budget=1e30)
     st = s.advance(0.05)
@@ -465,7 +467,7 @@ def test_cli_selector_run_is_the_library_run(mhd, tmp_path):
     (f,) = list(tmp_path.iterdir())
     assert f.name == f"orszag_tang_glm_muscl_hlld_mixed_N{n}x{n}.dat"
     rho = np.loadtxt(f, comments="#", usecols=2).reshape(n, n)
-    s = mixed_solver(mhd, n, n)
+    s = mixed_solver(mhd, n, n, tile=-1)  # the CLI's tile: the tuned one
     s.configure_selector(selector.load(selector.DEFAULT_MODEL))
     st = s.advance(t_end)
     u = s.get_state(0)
